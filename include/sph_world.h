@@ -1,0 +1,143 @@
+/*
+ * sph_world.h - C-ABI of the MI355X (gfx950) SPH substep library, libsph_hip.so.
+ *
+ * Drop-in for the per-substep compute chain that the reference records in
+ * recordCommands (src/renderer/commands.c:193-402) over the vk_context particle buffers
+ * (src/renderer/vk_types.h:127-159).  The reference has no FFI; its boundary is that
+ * dispatch chain plus the UBO physics fields.  Each entry point below names the reference
+ * code it replaces.
+ *
+ * Contract:
+ *   - Every int-returning call returns SPH_OK (0) or a negative sph_status; the reference
+ *     prints and exit(1)s instead (src/renderer/buffer.c:30-47).  sph_last_error() gives
+ *     the message of the calling thread's last failure.
+ *   - The library owns device memory; callers own host arrays.
+ *   - Calls on one handle are stream-ordered and must come from one thread at a time.
+ *   - Particle arrays are the reference's vec4 AoS layout (x, y, z, w), 16 B per particle
+ *     (predictPositions.comp:27-30).  After a step they are in the reference's permuted
+ *     (cell-key-sorted) order, exactly as the reference's buffers are.
+ */
+#ifndef SPH_WORLD_H
+#define SPH_WORLD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum sph_status {
+    SPH_OK = 0,
+    SPH_ERR_INVALID_ARG = -1,  /* bad size, NULL pointer, NaN or out-of-box particle      */
+    SPH_ERR_OUT_OF_MEMORY = -2,
+    SPH_ERR_DEVICE = -3,       /* a HIP call failed                                       */
+    SPH_ERR_STATE = -4,        /* call not valid in the handle's current state            */
+    SPH_ERR_CAPACITY = -5      /* request does not fit the configured budget / maximum    */
+} sph_status;
+
+/* Physics parameters: the UBO fields the step shaders read
+ * (UniformBufferObject, src/renderer/vk_types.h:52-65, filled by updateUniforms,
+ * src/main.c:53-63), plus the table size and substep count the host code fixes
+ * (renderer.c:741-754 sizes the table to numParticles; main.c:16 NUM_SUBSTEPS). */
+typedef struct sph_params {
+    float particle_radius;          /* wall margin of the predict pass; +0.5 in integrate */
+    float smoothing_radius;         /* R: cell edge and kernel support                     */
+    float g;                        /* gravity magnitude, applied along -z                 */
+    float min_xyz[3];               /* box                                                 */
+    float max_xyz[3];
+    float dt;                       /* per-substep dt (main.c:57: frame dt / substeps)     */
+    float particle_mass;
+    float pressure_multiplier;
+    float target_density;
+    float viscosity_strength;       /* accepted and ignored, as the reference's shader
+                                       never reads it (applyPressure...comp:174,313)       */
+    float volume;                   /* density normaliser 2*pi*R^5/15 (main.c:349)         */
+    float near_pressure_multiplier;
+    uint32_t table_cells;           /* T, cell-table size; keys are hash & (T-1).
+                                       0 means T = particle count, the reference's choice. */
+    uint32_t substeps;              /* substeps per sph_world_step frame                   */
+} sph_params;
+
+/* Fills dt and volume the way the reference's host does: dt = (float)(frame_dt / substeps)
+ * (main.c:57) and volume = (float)(2*pi*R^5/15) evaluated in double from the double
+ * radius literal (main.c:349).  Also sets p->substeps. */
+void sph_params_derive(sph_params *p, double frame_dt, uint32_t substeps,
+                       double smoothing_radius);
+
+typedef struct sph_world sph_world;
+
+/* Replaces initVulkan's buffer creation (renderer.c:851-888) for the particle, cell and
+ * table buffers of one scene of n particles on the current HIP device. */
+int sph_world_create(const sph_params *params, uint32_t n, sph_world **out);
+
+/* Replaces updateUniforms (main.c:37-82) for the physics fields.  n and table_cells are
+ * fixed at creation; a different table_cells returns SPH_ERR_INVALID_ARG. */
+int sph_world_set_params(sph_world *w, const sph_params *params);
+
+/* Replaces uploadDataToBufferViaStaging of the position and velocity buffers
+ * (buffer.c:102, renderer.c:569-645).  Host arrays of n vec4.  Positions must be finite
+ * and inside the box (the hash of an outside position is undefined in the reference). */
+int sph_world_upload_state(sph_world *w, const float *pos_vec4, const float *vel_vec4);
+
+/* Enqueues frames * substeps substeps; replaces the substep loop of recordCommands
+ * (commands.c:193-402).  Asynchronous: returns once enqueued. */
+int sph_world_step(sph_world *w, uint32_t frames);
+
+/* Waits for all enqueued work of the handle. */
+int sph_world_sync(sph_world *w);
+
+/* Copies the state back (synchronising first).  Any pointer may be NULL.  color_vec4
+ * receives updateVelocities.comp's speed ramp (:120-141) computed from the velocities. */
+int sph_world_download_state(sph_world *w, float *pos_vec4, float *vel_vec4, float *color_vec4);
+
+/* Intermediate buffers of the most recent substep, in the reference's formats, for parity
+ * checks.  Stages other than CELL_KEYS, SORTED_*, START, END, PRED_POS and DENSITY need
+ * debug capture (sph_world_set_debug) to have been on during that substep. */
+typedef enum sph_dump_stage {
+    SPH_DUMP_CELL_KEYS = 0,    /* u32[n]  CellEntry.key in the substep's input order       */
+    SPH_DUMP_SORTED_KEYS = 1,  /* u32[n]  sorted CellEntry.key                             */
+    SPH_DUMP_SORTED_INDEX = 2, /* u32[n]  sorted CellEntry.particleIndex (input index)     */
+    SPH_DUMP_START = 3,        /* i32[T]  startIndices, -1 = empty                         */
+    SPH_DUMP_END = 4,          /* i32[T]  endIndices, -1 = empty                           */
+    SPH_DUMP_PRED_POS = 5,     /* f32[4n] predictedPositions, sorted order                 */
+    SPH_DUMP_PRED_VEL = 6,     /* f32[4n] predictedVelocities, sorted order (debug)        */
+    SPH_DUMP_DENSITY = 7,      /* f32[n]  densities                                        */
+    SPH_DUMP_NEAR_DENSITY = 8, /* f32[n]  nearDensities                                    */
+    SPH_DUMP_VEL_FORCE = 9     /* f32[4n] velocities after the force pass (debug)          */
+} sph_dump_stage;
+
+int sph_world_set_debug(sph_world *w, int enable);
+int sph_world_debug_dump(sph_world *w, int stage, void *dst, size_t bytes);
+
+/* Per-pass device time in ms averaged over `substeps` eagerly launched substeps, the
+ * hipEvent counterpart of the reference's timestamp table (commands.c:3-18,
+ * main.c:170-198).  The substeps advance the state like sph_world_step. */
+typedef struct sph_pass_times {
+    double hash_ms;      /* updateCellEntries (+ table clear, histogram)                   */
+    double sort_ms;      /* multiRadixSort (all passes)                                    */
+    double reorder_ms;   /* predictPositions + scatterPass + updateStartIndices            */
+    double density_ms;   /* calculateDensities                                             */
+    double force_ms;     /* applyPressureAndViscosityForce + updateVelocities              */
+    double total_ms;
+} sph_pass_times;
+int sph_world_pass_times(sph_world *w, uint32_t substeps, sph_pass_times *out);
+
+/* Device footprint in bytes of a world of n particles and table_cells cells (0 = n). */
+uint64_t sph_world_footprint(uint32_t n, uint32_t table_cells);
+
+void sph_world_destroy(sph_world *w);
+
+const char *sph_last_error(void);
+
+/* Seeded scene initialisers (host only).  Counterparts of initializePositionsBuffer /
+ * initializeVelocitiesBuffer (renderer.c:569-645) for synthetic workloads: PCG32 streams,
+ * w = 0 like the reference's initial buffers.
+ *   block:   uniform in [lo, hi] per axis;  vel ~ N(0, vel_sigma) per component. */
+int sph_scene_fill_block(uint32_t n, uint64_t seed, const float lo[3], const float hi[3],
+                         float vel_sigma, float *pos_vec4, float *vel_vec4);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,97 @@
+// Device-side definitions shared by the SPH kernels (gfx950, wave64).
+//
+// All float arithmetic in this library is compiled with -ffp-contract=off and IEEE
+// division / sqrt, and every expression keeps the operand order of the reference's GLSL,
+// so results are reproducible bit for bit and independent of how particles are packed.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sph {
+
+// Per-scene constants.  One copy per scene: a kernel argument for a single world, an
+// array indexed by scene id for a batch.
+struct SceneConst {
+    float minx, miny, minz;
+    float maxx, maxy, maxz;
+    float R;                 // smoothing radius
+    float R2;                // R*R (force cut-off, applyPressure...comp:282)
+    float R2_pre;            // d2 > R2_pre implies sqrt(d2) >= R (density pre-filter)
+    float g, dt, m;
+    float rho0, k, kn;       // target density, pressure and near-pressure multipliers
+    float volume;            // 2*pi*R^5/15
+    float slope_scale;       // 15/(PI*pow(R,5))
+    float near_slope_scale;  // -45/(PI*pow(R,6))
+    float near_vol_inv;      // 15/(PI*pow(R,6))
+    float margin_predict;    // particleRadius
+    float margin_integrate;  // particleRadius + 0.5
+    uint32_t mask;           // table_cells - 1
+    uint32_t table_base;     // first cell of this scene in the table
+    uint32_t p_begin;        // first particle of this scene
+    uint32_t p_end;          // one past the last particle
+    uint32_t pad;
+};
+
+// cellCoordToHash (updateCellEntries.comp:36-52): u32 arithmetic, wraps mod 2^32.
+__device__ __forceinline__ uint32_t cell_hash(uint32_t x, uint32_t y, uint32_t z) {
+    uint32_t h = (x * 73856093u) ^ (y * 19349663u) ^ (z * 83492791u);
+    h ^= h >> 16;
+    h *= 0x7feb352du;
+    h ^= h >> 15;
+    h *= 0x846ca68bu;
+    h ^= h >> 16;
+    return h;
+}
+
+// floor((v - lo) / R) as a cell coordinate; true division, as the reference divides.
+__device__ __forceinline__ int32_t cell_of(float v, float lo, float R) {
+    return (int32_t)floorf((v - lo) / R);
+}
+
+// Wall clamp of predictPositions.comp:81-113 / updateVelocities.comp:83-115.
+// `vel *= 1.00` in the reference is exact and omitted.
+__device__ __forceinline__ void wall_clamp(const SceneConst &c, float margin, float3 &p, float3 &v) {
+    if (p.x < c.minx + margin) { p.x = c.minx + margin; v.x = v.x * -1.0f; }
+    else if (p.x > c.maxx - margin) { p.x = c.maxx - margin; v.x = v.x * -1.0f; }
+    if (p.y < c.miny + margin) { p.y = c.miny + margin; v.y = v.y * -1.0f; }
+    else if (p.y > c.maxy - margin) { p.y = c.maxy - margin; v.y = v.y * -1.0f; }
+    if (p.z < c.minz + margin) { p.z = c.minz + margin; v.z = v.z * -1.0f; v.z = v.z * 0.95f; }
+    else if (p.z > c.maxz - margin) { p.z = c.maxz - margin; v.z = v.z * -1.0f; }
+}
+
+// Gravity kick + drift + clamp: predictPositions.comp:74-113 (margin r, dt_scale absent)
+// and updateVelocities.comp:75-115 (margin r + 0.5, dt_scale = 1.0).
+template <bool kIntegrate>
+__device__ __forceinline__ void kick_drift(const SceneConst &c, float3 &p, float3 &v) {
+    const float dt = c.dt;
+    v.x = v.x + 0.0f * dt;
+    v.y = v.y + 0.0f * dt;
+    v.z = v.z + (-c.g) * dt;
+    if (kIntegrate) {
+        p.x = p.x + v.x * dt * 1.0f;
+        p.y = p.y + v.y * dt * 1.0f;
+        p.z = p.z + v.z * dt * 1.0f;
+        wall_clamp(c, c.margin_integrate, p, v);
+    } else {
+        p.x = p.x + v.x * dt;
+        p.y = p.y + v.y * dt;
+        p.z = p.z + v.z * dt;
+        wall_clamp(c, c.margin_predict, p, v);
+    }
+}
+
+// Scene lookup policies.  A single world passes its constants by value; a batch reads
+// them from a device table through the particle's scene id.
+struct OneScene {
+    SceneConst c;
+    __device__ __forceinline__ const SceneConst &scene(uint32_t) const { return c; }
+};
+
+struct ManyScenes {
+    const SceneConst *__restrict__ table;
+    const uint16_t *__restrict__ scene_of;  // scene id per particle slot
+    __device__ __forceinline__ SceneConst scene(uint32_t i) const { return table[scene_of[i]]; }
+};
+
+}  // namespace sph

@@ -1,0 +1,59 @@
+// Host-side pieces shared by the single-world and batched C-ABI entry points.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/sph_world.h"
+#include "sph_kernels.hpp"
+
+namespace sph {
+
+int fail(int code, const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what);
+const char *last_error();
+
+SceneConst make_scene_const(const sph_params &p, uint32_t table_cells, uint32_t table_base,
+                            uint32_t p_begin, uint32_t p_end);
+int validate_params(const sph_params *p);
+int validate_state(const sph_params &p, uint32_t n, const float *pos4, const float *vel4);
+uint32_t key_bits_for(uint64_t table_total);
+void speed_ramp(uint32_t n, const float *vel4, float *color4);
+
+// hipEvents bracketing the five passes of one substep.
+struct PassEvents {
+    hipEvent_t ev[6] = {};
+    int create();
+    void destroy();
+};
+
+// Buffers and launch sequence of the substep chain over `n` slots and `table_total` cells.
+struct Chain {
+    ChainBuffers b{};
+    uint32_t n = 0, table_total = 0, key_bits = 0, passes = 0, tiles = 0, lookback_words = 0;
+
+    int allocate(uint32_t n, uint32_t table_total);
+    int set_debug(bool on);
+    void release();
+    const uint32_t *sorted_keys() const;
+    const uint32_t *sorted_idx() const;  // nullptr means identity (no sort pass ran)
+
+    template <class Scenes>
+    hipError_t enqueue(hipStream_t s, const Scenes &sc, PassEvents *ev);
+};
+
+// One instantiated graph of the substep chain; dropped whenever what it captured changes.
+struct GraphCache {
+    hipGraphExec_t exec = nullptr;
+    int ensure_begin(hipStream_t s);
+    int end(hipStream_t s, hipError_t launch_status);
+    void drop();
+};
+
+// Copies one intermediate buffer of the last substep into `dst` in the reference's format,
+// restricted to slots [p_begin, p_end) and table cells [t_begin, t_end); slot indices and
+// keys are made scene-local.
+int dump_stage(const Chain &c, hipStream_t s, int stage, uint32_t p_begin, uint32_t p_end,
+               uint32_t t_begin, uint32_t t_end, void *dst, size_t bytes);
+
+}  // namespace sph

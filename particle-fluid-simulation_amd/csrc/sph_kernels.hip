@@ -1,0 +1,271 @@
+// SPH substep kernels for gfx950: hash, reorder + cell table, density, force + integrate.
+//
+// One lane per particle slot; 256-lane workgroups (4 wave64).  Neighbour sums visit the
+// 27 cells in the reference's (dz, dy, dx) order and each cell's run in ascending slot
+// order, so every particle's floating-point sum has the reference's association order and
+// does not depend on launch geometry or on which other scenes share the launch.
+#include "sph_device.hpp"
+#include "sph_kernels.hpp"
+
+namespace sph {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// ---------------------------------------------------------------------------------------
+// Hash: updateCellEntries.comp:69-80 (key of the CURRENT position, value = slot index,
+// implicit here), plus the digit histograms of the radix passes and the clears that
+// clearStartIndices.comp:47-52 performs, moved to a point where the table is dead.
+template <class Scenes>
+__global__ __launch_bounds__(kBlock) void k_hash(
+    Scenes sc, const float4 *__restrict__ pos, uint32_t *__restrict__ keys, uint32_t n,
+    uint32_t passes, uint32_t *__restrict__ ghist, uint32_t hash_blocks,
+    int2 *__restrict__ table, uint32_t table_total, uint32_t *__restrict__ lookback,
+    uint32_t lookback_words, uint32_t *__restrict__ tile_ctr) {
+    __shared__ uint32_t s_hist[4][256];
+    const uint32_t tid = threadIdx.x;
+    if (blockIdx.x < hash_blocks) {
+        for (uint32_t i = tid; i < 4u * 256u; i += kBlock) (&s_hist[0][0])[i] = 0u;
+        __syncthreads();
+        const uint32_t per_block = ((n + hash_blocks - 1u) / hash_blocks + kBlock - 1u) / kBlock * kBlock;
+        const uint32_t begin = blockIdx.x * per_block;
+        const uint32_t end = min(n, begin + per_block);
+        for (uint32_t i = begin + tid; i < end; i += kBlock) {
+            const SceneConst c = sc.scene(i);
+            const float4 p = pos[i];
+            const uint32_t cx = (uint32_t)cell_of(p.x, c.minx, c.R);
+            const uint32_t cy = (uint32_t)cell_of(p.y, c.miny, c.R);
+            const uint32_t cz = (uint32_t)cell_of(p.z, c.minz, c.R);
+            const uint32_t key = c.table_base + (cell_hash(cx, cy, cz) & c.mask);
+            keys[i] = key;
+            for (uint32_t q = 0; q < passes; ++q) atomicAdd(&s_hist[q][(key >> (8u * q)) & 255u], 1u);
+        }
+        __syncthreads();
+        for (uint32_t q = 0; q < passes; ++q) {
+            const uint32_t v = s_hist[q][tid];
+            if (v) atomicAdd(&ghist[q * 256u + tid], v);
+        }
+    } else {
+        const uint32_t b = blockIdx.x - hash_blocks;
+        const uint32_t nb = gridDim.x - hash_blocks;
+        if (b == 0 && tid < 4) tile_ctr[tid] = 0u;
+        for (uint32_t i = b * kBlock + tid; i < table_total; i += nb * kBlock) table[i] = make_int2(-1, -1);
+        for (uint32_t i = b * kBlock + tid; i < lookback_words; i += nb * kBlock) lookback[i] = 0u;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Reorder: scatterPass.comp:50-65 gathers the state into key order; predictPositions.comp
+// is a pure per-particle function, so it is evaluated here on the gathered state instead
+// of being gathered; updateStartIndices.comp:40-64 marks the run boundaries.
+template <class Scenes>
+__global__ __launch_bounds__(kBlock) void k_reorder(
+    Scenes sc, const uint32_t *__restrict__ skeys, const uint32_t *__restrict__ sidx,
+    uint32_t n, const float4 *__restrict__ pos, const float4 *__restrict__ vel,
+    float4 *__restrict__ pos_sorted, float4 *__restrict__ vel_sorted,
+    float4 *__restrict__ pred_pos, float4 *__restrict__ dbg_pred_vel, int2 *__restrict__ table,
+    uint32_t *__restrict__ ghist) {
+    const uint32_t tid = threadIdx.x;
+    if (blockIdx.x == 0)
+        for (uint32_t i = tid; i < 4u * 256u; i += kBlock) ghist[i] = 0u;
+    const uint32_t j = blockIdx.x * kBlock + tid;
+    if (j >= n) return;
+    const uint32_t key = skeys[j];
+    const uint32_t idx = sidx ? sidx[j] : j;
+    const float4 P = pos[idx];
+    const float4 V = vel[idx];
+    const SceneConst c = sc.scene(j);
+    float3 p = make_float3(P.x, P.y, P.z), v = make_float3(V.x, V.y, V.z);
+    kick_drift<false>(c, p, v);
+    pos_sorted[j] = P;
+    vel_sorted[j] = V;
+    pred_pos[j] = make_float4(p.x, p.y, p.z, 1.0f);
+    if (dbg_pred_vel) dbg_pred_vel[j] = make_float4(v.x, v.y, v.z, 1.0f);
+    int *tbl = reinterpret_cast<int *>(table);
+    if (j == 0) {
+        tbl[2u * key] = 0;
+    } else {
+        const uint32_t prev = skeys[j - 1u];
+        if (key != prev) {
+            tbl[2u * key] = (int)j;
+            tbl[2u * prev + 1u] = (int)j;
+        }
+    }
+    if (j == n - 1u) tbl[2u * key + 1u] = (int)n;
+}
+
+// ---------------------------------------------------------------------------------------
+// Density: calculateDensities.comp:96-155.
+template <class Scenes>
+__global__ __launch_bounds__(kBlock) void k_density(
+    Scenes sc, const float4 *__restrict__ pred_pos, const int2 *__restrict__ table,
+    float2 *__restrict__ dens, uint32_t n) {
+    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= n) return;
+    const SceneConst c = sc.scene(j);
+    const float4 s = pred_pos[j];
+    float M = 0.0f, Mn = 0.0f;
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                // samplePoint + smoothingRadius * vec3(dx, dy, dz), then samplePointToKey
+                const float qx = s.x + c.R * (float)dx;
+                const float qy = s.y + c.R * (float)dy;
+                const float qz = s.z + c.R * (float)dz;
+                if (qx < c.minx || qy < c.miny || qz < c.minz) continue;
+                if (qx > c.maxx || qy > c.maxy || qz > c.maxz) continue;
+                const uint32_t key = c.table_base +
+                    (cell_hash((uint32_t)cell_of(qx, c.minx, c.R), (uint32_t)cell_of(qy, c.miny, c.R),
+                               (uint32_t)cell_of(qz, c.minz, c.R)) & c.mask);
+                const int2 se = table[key];
+                if (se.x == -1) continue;
+                for (int i = se.x; i < se.y; ++i) {
+                    const float4 p = pred_pos[i];
+                    const float ex = p.x - s.x, ey = p.y - s.y, ez = p.z - s.z;
+                    const float d2 = ex * ex + ey * ey + ez * ez;
+                    if (d2 > c.R2_pre) continue;  // sqrt(d2) >= R: the kernels are 0
+                    const float dist = sqrtf(d2);
+                    if (dist >= c.R) continue;
+                    const float t = c.R - dist;
+                    M = M + c.m * (t * t);
+                    Mn = Mn + c.m * (t * t * t);
+                }
+            }
+    const float density = M / c.volume;
+    const float near_density = Mn * c.near_vol_inv;
+    dens[j] = make_float2(fmaxf(0.01f, density), fmaxf(0.01f, near_density));
+}
+
+// ---------------------------------------------------------------------------------------
+// Force + integrate: applyPressureAndViscosityForce.comp:275-337 followed by
+// updateVelocities.comp:71-119.  Neighbour velocities come from vel_sorted, which no lane
+// writes (the reference races on that buffer; here every read sees the pre-force value).
+template <class Scenes>
+__global__ __launch_bounds__(kBlock) void k_force(
+    Scenes sc, const float4 *__restrict__ pred_pos, const float4 *__restrict__ pos_sorted,
+    const float4 *__restrict__ vel_sorted, const float2 *__restrict__ dens,
+    const int2 *__restrict__ table, float4 *__restrict__ pos_out, float4 *__restrict__ vel_out,
+    float4 *__restrict__ dbg_vel_force, uint32_t n) {
+    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= n) return;
+    const SceneConst c = sc.scene(j);
+    const float4 s = pred_pos[j];
+    const float4 P = pos_sorted[j];
+    const float4 V = vel_sorted[j];
+    float3 ps = make_float3(P.x, P.y, P.z), u = make_float3(V.x, V.y, V.z);
+    kick_drift<false>(c, ps, u);  // u = predictedVelocities[j]
+    const float2 myd = dens[j];
+    const float my_density = myd.x;
+    const float near_pressure = myd.y * c.kn;
+    const float my_pressure = (my_density - c.rho0) * c.k;
+    const bool outside = s.x < c.minx || s.y < c.miny || s.z < c.minz || s.x > c.maxx ||
+                         s.y > c.maxy || s.z > c.maxz;
+    const int32_t cx = outside ? -1 : cell_of(s.x, c.minx, c.R);
+    const int32_t cy = outside ? -1 : cell_of(s.y, c.miny, c.R);
+    const int32_t cz = outside ? -1 : cell_of(s.z, c.minz, c.R);
+    float3 Fp = make_float3(0.0f, 0.0f, 0.0f), Fv = make_float3(0.0f, 0.0f, 0.0f);
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const uint32_t key = c.table_base +
+                    (cell_hash((uint32_t)(cx + dx), (uint32_t)(cy + dy), (uint32_t)(cz + dz)) & c.mask);
+                const int2 se = table[key];
+                if (se.x == -1) continue;
+                for (int i = se.x; i < se.y; ++i) {
+                    if ((uint32_t)i == j) continue;
+                    const float4 pi = pred_pos[i];
+                    const float ex = pi.x - s.x, ey = pi.y - s.y, ez = pi.z - s.z;
+                    const float d2 = ex * ex + ey * ey + ez * ez;
+                    if (d2 > c.R2) continue;
+                    const float dist = sqrtf(d2);
+                    if (fabsf(dist) < 1e-7f) continue;
+                    const float dirx = ex / dist, diry = ey / dist, dirz = ez / dist;
+                    const float2 di = dens[i];
+                    const float slope = dist >= c.R ? 0.0f : (dist - c.R) * c.slope_scale;
+                    const float shared = ((di.x - c.rho0) * c.k + my_pressure) / 2.0f;
+                    const float a = shared * slope * c.m / di.x;
+                    Fp.x = Fp.x + a * dirx;
+                    Fp.y = Fp.y + a * diry;
+                    Fp.z = Fp.z + a * dirz;
+                    const float infl = dist >= c.R ? 0.0f : (c.R - dist) * (c.R - dist);
+                    const float b = c.m * infl / di.x;
+                    const float4 vi = vel_sorted[i];
+                    Fv.x = Fv.x + b * (vi.x - u.x);
+                    Fv.y = Fv.y + b * (vi.y - u.y);
+                    Fv.z = Fv.z + b * (vi.z - u.z);
+                    const float shared_near = (near_pressure + di.y * c.kn) / 2.0f;
+                    const float x = c.R - dist;
+                    const float nslope = dist >= c.R ? 0.0f : x * x * c.near_slope_scale;
+                    const float cn = shared_near * nslope * c.m / di.y;
+                    Fp.x = Fp.x + cn * dirx;
+                    Fp.y = Fp.y + cn * diry;
+                    Fp.z = Fp.z + cn * dirz;
+                }
+            }
+    float3 v = make_float3(V.x + (Fp.x + Fv.x + 0.0f) / my_density * c.dt,
+                           V.y + (Fp.y + Fv.y + 0.0f) / my_density * c.dt,
+                           V.z + (Fp.z + Fv.z + 0.0f) / my_density * c.dt);
+    if (dbg_vel_force) dbg_vel_force[j] = make_float4(v.x, v.y, v.z, V.w + 1.0f);
+    float3 p = make_float3(P.x, P.y, P.z);
+    kick_drift<true>(c, p, v);
+    pos_out[j] = make_float4(p.x, p.y, p.z, 1.0f);
+    vel_out[j] = make_float4(v.x, v.y, v.z, 1.0f);
+}
+
+inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1u) / kBlock; }
+
+}  // namespace
+
+template <class Scenes>
+hipError_t launch_hash(hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n,
+                       uint32_t table_total, uint32_t passes, uint32_t lookback_words) {
+    // ~8K keys per hashing workgroup keeps the global histogram atomics few; the clearing
+    // workgroups run beside them.
+    const uint32_t hash_blocks = (n + 8191u) / 8192u;
+    const uint32_t clear_work = table_total > lookback_words ? table_total : lookback_words;
+    uint32_t clear_blocks = (clear_work + 4u * kBlock - 1u) / (4u * kBlock);
+    if (clear_blocks < 1u) clear_blocks = 1u;
+    if (clear_blocks > 2048u) clear_blocks = 2048u;
+    hipLaunchKernelGGL(k_hash<Scenes>, dim3(hash_blocks + clear_blocks), dim3(kBlock), 0, stream,
+                       sc, b.pos, b.keys, n, passes, b.ghist, hash_blocks, b.table, table_total,
+                       b.lookback, lookback_words, b.tile_ctr);
+    return hipGetLastError();
+}
+
+template <class Scenes>
+hipError_t launch_reorder(hipStream_t stream, const Scenes &sc, const ChainBuffers &b,
+                          const uint32_t *sorted_keys, const uint32_t *sorted_idx, uint32_t n) {
+    hipLaunchKernelGGL(k_reorder<Scenes>, dim3(blocks_for(n)), dim3(kBlock), 0, stream, sc,
+                       sorted_keys, sorted_idx, n, b.pos, b.vel, b.pos_sorted, b.vel_sorted,
+                       b.pred_pos, b.dbg_pred_vel, b.table, b.ghist);
+    return hipGetLastError();
+}
+
+template <class Scenes>
+hipError_t launch_density(hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n) {
+    hipLaunchKernelGGL(k_density<Scenes>, dim3(blocks_for(n)), dim3(kBlock), 0, stream, sc,
+                       b.pred_pos, b.table, b.dens, n);
+    return hipGetLastError();
+}
+
+template <class Scenes>
+hipError_t launch_force(hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n) {
+    hipLaunchKernelGGL(k_force<Scenes>, dim3(blocks_for(n)), dim3(kBlock), 0, stream, sc,
+                       b.pred_pos, b.pos_sorted, b.vel_sorted, b.dens, b.table, b.pos, b.vel,
+                       b.dbg_vel_force, n);
+    return hipGetLastError();
+}
+
+#define SPH_INSTANTIATE(S)                                                                       \
+    template hipError_t launch_hash<S>(hipStream_t, const S &, const ChainBuffers &, uint32_t,   \
+                                       uint32_t, uint32_t, uint32_t);                            \
+    template hipError_t launch_reorder<S>(hipStream_t, const S &, const ChainBuffers &,          \
+                                          const uint32_t *, const uint32_t *, uint32_t);         \
+    template hipError_t launch_density<S>(hipStream_t, const S &, const ChainBuffers &, uint32_t); \
+    template hipError_t launch_force<S>(hipStream_t, const S &, const ChainBuffers &, uint32_t);
+
+SPH_INSTANTIATE(OneScene)
+SPH_INSTANTIATE(ManyScenes)
+
+}  // namespace sph

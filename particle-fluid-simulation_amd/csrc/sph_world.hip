@@ -1,0 +1,451 @@
+// C-ABI of the single-scene SPH world (include/sph_world.h) and the host orchestration of
+// one substep: hash -> radix passes -> reorder/table -> density -> force/integrate, captured
+// once into a hipGraph and replayed for every substep.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/sph_world.h"
+#include "sph_host.hpp"
+#include "sph_kernels.hpp"
+
+namespace sph {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+    return fail(SPH_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+const char *last_error() { return g_last_error.c_str(); }
+
+// Constants derived from the parameter block, computed on the host in binary32 exactly as
+// the shaders derive them (calculateDensities.comp:110, applyPressure...comp:243,258,282).
+SceneConst make_scene_const(const sph_params &p, uint32_t table_cells, uint32_t table_base,
+                            uint32_t p_begin, uint32_t p_end) {
+    const float kPi = 3.14159265359f;
+    SceneConst c{};
+    const float R = p.smoothing_radius;
+    c.minx = p.min_xyz[0]; c.miny = p.min_xyz[1]; c.minz = p.min_xyz[2];
+    c.maxx = p.max_xyz[0]; c.maxy = p.max_xyz[1]; c.maxz = p.max_xyz[2];
+    c.R = R;
+    c.R2 = R * R;
+    // Smallest float above R^2 (1 + 2^-20): any d2 beyond it has a correctly rounded
+    // sqrt >= R, so skipping those candidates before the sqrt changes no result.
+    const double r2 = (double)R * (double)R * (1.0 + std::ldexp(1.0, -20));
+    c.R2_pre = std::nextafter((float)r2, INFINITY);
+    c.g = p.g;
+    c.dt = p.dt;
+    c.m = p.particle_mass;
+    c.rho0 = p.target_density;
+    c.k = p.pressure_multiplier;
+    c.kn = p.near_pressure_multiplier;
+    c.volume = p.volume;
+    c.slope_scale = 15.0f / (kPi * powf(R, 5.0f));
+    c.near_slope_scale = -45.0f / (kPi * powf(R, 6.0f));
+    c.near_vol_inv = 15.0f / (kPi * powf(R, 6.0f));
+    c.margin_predict = p.particle_radius;
+    c.margin_integrate = p.particle_radius + 0.5f;
+    c.mask = table_cells - 1u;
+    c.table_base = table_base;
+    c.p_begin = p_begin;
+    c.p_end = p_end;
+    return c;
+}
+
+int validate_params(const sph_params *p) {
+    if (!p) return fail(SPH_ERR_INVALID_ARG, "params is NULL");
+    const float vals[] = {p->particle_radius, p->smoothing_radius, p->g, p->dt,
+                          p->particle_mass, p->pressure_multiplier, p->target_density,
+                          p->volume, p->near_pressure_multiplier,
+                          p->min_xyz[0], p->min_xyz[1], p->min_xyz[2],
+                          p->max_xyz[0], p->max_xyz[1], p->max_xyz[2]};
+    for (float v : vals)
+        if (!std::isfinite(v)) return fail(SPH_ERR_INVALID_ARG, "non-finite parameter");
+    if (!(p->smoothing_radius > 0.0f)) return fail(SPH_ERR_INVALID_ARG, "smoothing_radius must be > 0");
+    if (!(p->volume > 0.0f)) return fail(SPH_ERR_INVALID_ARG, "volume must be > 0");
+    for (int a = 0; a < 3; ++a)
+        if (!(p->max_xyz[a] > p->min_xyz[a])) return fail(SPH_ERR_INVALID_ARG, "empty box");
+    // Cell coordinates must fit in int32 (floor((max-min)/R) is converted to int).
+    for (int a = 0; a < 3; ++a)
+        if ((double)(p->max_xyz[a] - p->min_xyz[a]) / p->smoothing_radius > 1.0e9)
+            return fail(SPH_ERR_INVALID_ARG, "box too large for 32-bit cell coordinates");
+    if (p->substeps == 0) return fail(SPH_ERR_INVALID_ARG, "substeps must be >= 1");
+    return SPH_OK;
+}
+
+int validate_state(const sph_params &p, uint32_t n, const float *pos4, const float *vel4) {
+    if (!pos4 || !vel4) return fail(SPH_ERR_INVALID_ARG, "state pointer is NULL");
+    for (uint32_t i = 0; i < n; ++i) {
+        for (int a = 0; a < 3; ++a) {
+            const float x = pos4[4 * (size_t)i + a], v = vel4[4 * (size_t)i + a];
+            if (!std::isfinite(x) || !std::isfinite(v))
+                return fail(SPH_ERR_INVALID_ARG, "particle %u: non-finite position or velocity", i);
+            if (x < p.min_xyz[a] || x > p.max_xyz[a])
+                return fail(SPH_ERR_INVALID_ARG, "particle %u: position outside the box", i);
+        }
+    }
+    return SPH_OK;
+}
+
+uint32_t key_bits_for(uint64_t table_total) {
+    uint32_t b = 0;
+    while ((1ull << b) < table_total) ++b;
+    return b;
+}
+
+// ---------------------------------------------------------------------------------------
+// Chain: buffers and launches of one substep, shared by the single world and the batch.
+
+int Chain::allocate(uint32_t n_, uint32_t table_total_) {
+    n = n_;
+    table_total = table_total_;
+    key_bits = key_bits_for(table_total);
+    passes = (key_bits + 7u) / 8u;
+    tiles = (n + radix_tile_size() - 1u) / radix_tile_size();
+    lookback_words = passes * tiles * 256u;
+    struct Req { void **p; size_t bytes; };
+    const size_t N = n;
+    Req reqs[] = {
+        {(void **)&b.pos, N * 16}, {(void **)&b.vel, N * 16}, {(void **)&b.keys, N * 4},
+        {(void **)&b.sort_k[0], N * 4}, {(void **)&b.sort_v[0], N * 4},
+        {(void **)&b.sort_k[1], N * 4}, {(void **)&b.sort_v[1], N * 4},
+        {(void **)&b.pred_pos, N * 16}, {(void **)&b.pos_sorted, N * 16},
+        {(void **)&b.vel_sorted, N * 16}, {(void **)&b.dens, N * 8},
+        {(void **)&b.table, (size_t)table_total * 8}, {(void **)&b.ghist, 4 * 256 * 4},
+        {(void **)&b.lookback, (size_t)(lookback_words ? lookback_words : 1) * 4},
+        {(void **)&b.tile_ctr, 4 * 4},
+    };
+    for (auto &r : reqs) {
+        hipError_t e = hipMalloc(r.p, r.bytes);
+        if (e != hipSuccess) {
+            release();
+            return fail(SPH_ERR_OUT_OF_MEMORY, "hipMalloc(%zu): %s", r.bytes, hipGetErrorString(e));
+        }
+    }
+    hipError_t e = hipMemset(b.ghist, 0, 4 * 256 * 4);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        release();
+        return hip_fail(e, "hipMemset");
+    }
+    return SPH_OK;
+}
+
+int Chain::set_debug(bool on) {
+    if (on && !b.dbg_pred_vel) {
+        hipError_t e = hipMalloc((void **)&b.dbg_pred_vel, (size_t)n * 16);
+        if (e == hipSuccess) e = hipMalloc((void **)&b.dbg_vel_force, (size_t)n * 16);
+        if (e != hipSuccess) return fail(SPH_ERR_OUT_OF_MEMORY, "debug buffers: %s", hipGetErrorString(e));
+    } else if (!on && b.dbg_pred_vel) {
+        hipFree(b.dbg_pred_vel);
+        hipFree(b.dbg_vel_force);
+        b.dbg_pred_vel = nullptr;
+        b.dbg_vel_force = nullptr;
+    }
+    return SPH_OK;
+}
+
+void Chain::release() {
+    void *ptrs[] = {b.pos, b.vel, b.keys, b.sort_k[0], b.sort_v[0], b.sort_k[1], b.sort_v[1],
+                    b.pred_pos, b.pos_sorted, b.vel_sorted, b.dens, b.table, b.ghist,
+                    b.lookback, b.tile_ctr, b.dbg_pred_vel, b.dbg_vel_force};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    b = ChainBuffers{};
+}
+
+const uint32_t *Chain::sorted_keys() const { return passes ? b.sort_k[(passes - 1u) & 1u] : b.keys; }
+const uint32_t *Chain::sorted_idx() const { return passes ? b.sort_v[(passes - 1u) & 1u] : nullptr; }
+
+template <class Scenes>
+hipError_t Chain::enqueue(hipStream_t s, const Scenes &sc, PassEvents *ev) {
+    hipError_t e;
+    if (ev) hipEventRecord(ev->ev[0], s);
+    if ((e = launch_hash(s, sc, b, n, table_total, passes, lookback_words)) != hipSuccess) return e;
+    if (ev) hipEventRecord(ev->ev[1], s);
+    for (uint32_t q = 0; q < passes; ++q) {
+        const uint32_t *kin = q == 0 ? b.keys : b.sort_k[(q - 1u) & 1u];
+        const uint32_t *vin = q == 0 ? nullptr : b.sort_v[(q - 1u) & 1u];
+        const uint32_t bits = key_bits - 8u * q < 8u ? key_bits - 8u * q : 8u;
+        e = radix_pass(s, kin, vin, b.sort_k[q & 1u], b.sort_v[q & 1u], n, 8u * q, bits,
+                       b.ghist + 256u * q, b.lookback + (size_t)q * tiles * 256u, b.tile_ctr + q);
+        if (e != hipSuccess) return e;
+    }
+    if (ev) hipEventRecord(ev->ev[2], s);
+    if ((e = launch_reorder(s, sc, b, sorted_keys(), sorted_idx(), n)) != hipSuccess) return e;
+    if (ev) hipEventRecord(ev->ev[3], s);
+    if ((e = launch_density(s, sc, b, n)) != hipSuccess) return e;
+    if (ev) hipEventRecord(ev->ev[4], s);
+    if ((e = launch_force(s, sc, b, n)) != hipSuccess) return e;
+    if (ev) hipEventRecord(ev->ev[5], s);
+    return hipSuccess;
+}
+
+template hipError_t Chain::enqueue<OneScene>(hipStream_t, const OneScene &, PassEvents *);
+template hipError_t Chain::enqueue<ManyScenes>(hipStream_t, const ManyScenes &, PassEvents *);
+
+int GraphCache::ensure_begin(hipStream_t s) {
+    drop();
+    hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+    return e == hipSuccess ? SPH_OK : hip_fail(e, "hipStreamBeginCapture");
+}
+
+int GraphCache::end(hipStream_t s, hipError_t launch_status) {
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (launch_status != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        return hip_fail(launch_status, "kernel launch during capture");
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        exec = nullptr;
+        return hip_fail(e, "hipGraphInstantiate");
+    }
+    return SPH_OK;
+}
+
+void GraphCache::drop() {
+    if (exec) hipGraphExecDestroy(exec);
+    exec = nullptr;
+}
+
+int PassEvents::create() {
+    for (auto &e : ev)
+        if (hipEventCreate(&e) != hipSuccess) return fail(SPH_ERR_DEVICE, "hipEventCreate");
+    return SPH_OK;
+}
+
+void PassEvents::destroy() {
+    for (auto &e : ev)
+        if (e) hipEventDestroy(e);
+}
+
+void speed_ramp(uint32_t n, const float *vel4, float *color4) {
+    static const float blue[3] = {0.3f, 0.3f, 1.0f}, cyan[3] = {0.0f, 1.0f, 1.0f},
+                       yellow[3] = {1.0f, 1.0f, 0.0f}, red[3] = {1.0f, 0.3f, 0.3f};
+    for (uint32_t i = 0; i < n; ++i) {
+        const float *v = &vel4[4 * (size_t)i];
+        const float speed = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+        const float t = std::fmin(std::fmax(speed / 60.0f, 0.0f), 1.0f);
+        const float *a, *b;
+        float w;
+        if (t < 0.33f) { a = blue; b = cyan; w = t / 0.33f; }
+        else if (t < 0.66f) { a = cyan; b = yellow; w = (t - 0.33f) / 0.33f; }
+        else { a = yellow; b = red; w = (t - 0.66f) / 0.34f; }
+        for (int q = 0; q < 3; ++q) color4[4 * (size_t)i + q] = a[q] * (1.0f - w) + b[q] * w;
+        color4[4 * (size_t)i + 3] = 1.0f;
+    }
+}
+
+}  // namespace sph
+
+using namespace sph;
+
+struct sph_world {
+    sph_params params{};
+    uint32_t n = 0;
+    uint32_t T = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Chain chain;
+    OneScene scene{};
+    GraphCache graph;
+    bool debug = false;
+    bool stepped = false;
+};
+
+extern "C" {
+
+void sph_params_derive(sph_params *p, double frame_dt, uint32_t substeps, double smoothing_radius) {
+    if (!p || substeps == 0) return;
+    p->substeps = substeps;
+    p->dt = (float)(frame_dt / (double)substeps);
+    p->volume = (float)(2.0 * M_PI * std::pow(smoothing_radius, 5) / 15.0);
+}
+
+uint64_t sph_world_footprint(uint32_t n, uint32_t table_cells) {
+    const uint64_t T = table_cells ? table_cells : n;
+    const uint64_t tiles = ((uint64_t)n + radix_tile_size() - 1) / radix_tile_size();
+    return (uint64_t)n * (16 * 2 + 4 + 16 + 16 * 3 + 8) + T * 8 + 4 * tiles * 256 * 4 + 4096;
+}
+
+int sph_world_create(const sph_params *params, uint32_t n, sph_world **out) {
+    if (!out) return fail(SPH_ERR_INVALID_ARG, "out is NULL");
+    *out = nullptr;
+    int rc = validate_params(params);
+    if (rc) return rc;
+    if (n == 0) return fail(SPH_ERR_INVALID_ARG, "particle count must be > 0");
+    if (n >= (1u << 30)) return fail(SPH_ERR_CAPACITY, "particle count must be < 2^30");
+    const uint32_t T = params->table_cells ? params->table_cells : n;
+    if (T >= (1u << 31)) return fail(SPH_ERR_CAPACITY, "table_cells must be < 2^31");
+    sph_world *w = new (std::nothrow) sph_world();
+    if (!w) return fail(SPH_ERR_OUT_OF_MEMORY, "host allocation");
+    w->params = *params;
+    w->params.table_cells = T;
+    w->n = n;
+    w->T = T;
+    hipError_t e = hipGetDevice(&w->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete w;
+        return hip_fail(e, "stream creation");
+    }
+    rc = w->chain.allocate(n, T);
+    if (rc) {
+        hipStreamDestroy(w->stream);
+        delete w;
+        return rc;
+    }
+    w->scene.c = make_scene_const(w->params, T, 0, 0, n);
+    *out = w;
+    return SPH_OK;
+}
+
+int sph_world_set_params(sph_world *w, const sph_params *params) {
+    if (!w) return fail(SPH_ERR_INVALID_ARG, "world is NULL");
+    int rc = validate_params(params);
+    if (rc) return rc;
+    const uint32_t T = params->table_cells ? params->table_cells : w->n;
+    if (T != w->T) return fail(SPH_ERR_INVALID_ARG, "table_cells is fixed at creation");
+    w->params = *params;
+    w->params.table_cells = T;
+    w->scene.c = make_scene_const(w->params, T, 0, 0, w->n);
+    w->graph.drop();
+    return SPH_OK;
+}
+
+int sph_world_upload_state(sph_world *w, const float *pos4, const float *vel4) {
+    if (!w) return fail(SPH_ERR_INVALID_ARG, "world is NULL");
+    int rc = validate_state(w->params, w->n, pos4, vel4);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(w->chain.b.pos, pos4, (size_t)w->n * 16, hipMemcpyHostToDevice, w->stream);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(w->chain.b.vel, vel4, (size_t)w->n * 16, hipMemcpyHostToDevice, w->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
+    if (e != hipSuccess) return hip_fail(e, "upload");
+    w->stepped = false;
+    return SPH_OK;
+}
+
+int sph_world_step(sph_world *w, uint32_t frames) {
+    if (!w) return fail(SPH_ERR_INVALID_ARG, "world is NULL");
+    const uint64_t substeps = (uint64_t)frames * w->params.substeps;
+    if (substeps == 0) return SPH_OK;
+    if (!w->graph.exec) {
+        int rc = w->graph.ensure_begin(w->stream);
+        if (rc) return rc;
+        hipError_t le = w->chain.enqueue(w->stream, w->scene, nullptr);
+        rc = w->graph.end(w->stream, le);
+        if (rc) return rc;
+    }
+    for (uint64_t s = 0; s < substeps; ++s) {
+        hipError_t e = hipGraphLaunch(w->graph.exec, w->stream);
+        if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+    }
+    w->stepped = true;
+    return SPH_OK;
+}
+
+int sph_world_sync(sph_world *w) {
+    if (!w) return fail(SPH_ERR_INVALID_ARG, "world is NULL");
+    hipError_t e = hipStreamSynchronize(w->stream);
+    return e == hipSuccess ? SPH_OK : hip_fail(e, "hipStreamSynchronize");
+}
+
+int sph_world_download_state(sph_world *w, float *pos4, float *vel4, float *color4) {
+    if (!w) return fail(SPH_ERR_INVALID_ARG, "world is NULL");
+    std::vector<float> tmp;
+    float *vdst = vel4;
+    if (color4 && !vel4) {
+        tmp.resize((size_t)w->n * 4);
+        vdst = tmp.data();
+    }
+    hipError_t e = hipSuccess;
+    if (pos4) e = hipMemcpyAsync(pos4, w->chain.b.pos, (size_t)w->n * 16, hipMemcpyDeviceToHost, w->stream);
+    if (e == hipSuccess && vdst)
+        e = hipMemcpyAsync(vdst, w->chain.b.vel, (size_t)w->n * 16, hipMemcpyDeviceToHost, w->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
+    if (e != hipSuccess) return hip_fail(e, "download");
+    if (color4) speed_ramp(w->n, vdst, color4);
+    return SPH_OK;
+}
+
+int sph_world_set_debug(sph_world *w, int enable) {
+    if (!w) return fail(SPH_ERR_INVALID_ARG, "world is NULL");
+    hipError_t e = hipStreamSynchronize(w->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    int rc = w->chain.set_debug(enable != 0);
+    if (rc) return rc;
+    w->debug = enable != 0;
+    w->graph.drop();
+    return SPH_OK;
+}
+
+int sph_world_debug_dump(sph_world *w, int stage, void *dst, size_t bytes) {
+    if (!w || !dst) return fail(SPH_ERR_INVALID_ARG, "NULL argument");
+    if (!w->stepped) return fail(SPH_ERR_STATE, "no substep has run since the last upload");
+    return dump_stage(w->chain, w->stream, stage, 0, w->n, 0, w->T, dst, bytes);
+}
+
+int sph_world_pass_times(sph_world *w, uint32_t substeps, sph_pass_times *out) {
+    if (!w || !out) return fail(SPH_ERR_INVALID_ARG, "NULL argument");
+    if (substeps == 0) return fail(SPH_ERR_INVALID_ARG, "substeps must be >= 1");
+    PassEvents ev;
+    int rc = ev.create();
+    if (rc) { ev.destroy(); return rc; }
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (uint32_t s = 0; s < substeps; ++s) {
+        hipError_t e = w->chain.enqueue(w->stream, w->scene, &ev);
+        if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
+        if (e != hipSuccess) { ev.destroy(); return hip_fail(e, "timed substep"); }
+        for (int q = 0; q < 5; ++q) {
+            float ms = 0.0f;
+            hipEventElapsedTime(&ms, ev.ev[q], ev.ev[q + 1]);
+            acc[q] += ms;
+        }
+    }
+    ev.destroy();
+    out->hash_ms = acc[0] / substeps;
+    out->sort_ms = acc[1] / substeps;
+    out->reorder_ms = acc[2] / substeps;
+    out->density_ms = acc[3] / substeps;
+    out->force_ms = acc[4] / substeps;
+    out->total_ms = out->hash_ms + out->sort_ms + out->reorder_ms + out->density_ms + out->force_ms;
+    w->stepped = true;
+    return SPH_OK;
+}
+
+void sph_world_destroy(sph_world *w) {
+    if (!w) return;
+    if (w->stream) hipStreamSynchronize(w->stream);
+    w->graph.drop();
+    w->chain.release();
+    if (w->stream) hipStreamDestroy(w->stream);
+    delete w;
+}
+
+const char *sph_last_error(void) { return sph::last_error(); }
+
+}  // extern "C"

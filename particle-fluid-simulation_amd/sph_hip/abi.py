@@ -1,0 +1,142 @@
+"""ctypes declarations of include/sph_world.h."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_LIB_PATH = os.path.join(_PKG, "lib", "libsph_hip.so")
+
+SPH_OK = 0
+STATUS_NAMES = {
+    -1: "SPH_ERR_INVALID_ARG", -2: "SPH_ERR_OUT_OF_MEMORY", -3: "SPH_ERR_DEVICE",
+    -4: "SPH_ERR_STATE", -5: "SPH_ERR_CAPACITY",
+}
+
+DUMP_STAGES = {
+    "cell_keys": 0, "sorted_keys": 1, "sorted_index": 2, "start": 3, "end": 4,
+    "pred_pos": 5, "pred_vel": 6, "density": 7, "near_density": 8, "vel_force": 9,
+}
+
+
+class SphError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class SphParams(C.Structure):
+    """sph_params: the reference UBO's physics fields (vk_types.h:52-65) + T + substeps."""
+    _fields_ = [
+        ("particle_radius", C.c_float),
+        ("smoothing_radius", C.c_float),
+        ("g", C.c_float),
+        ("min_xyz", C.c_float * 3),
+        ("max_xyz", C.c_float * 3),
+        ("dt", C.c_float),
+        ("particle_mass", C.c_float),
+        ("pressure_multiplier", C.c_float),
+        ("target_density", C.c_float),
+        ("viscosity_strength", C.c_float),
+        ("volume", C.c_float),
+        ("near_pressure_multiplier", C.c_float),
+        ("table_cells", C.c_uint32),
+        ("substeps", C.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            out[name] = list(v) if name.endswith("_xyz") else v
+        return out
+
+
+class PassTimes(C.Structure):
+    _fields_ = [(n, C.c_double) for n in
+                ("hash_ms", "sort_ms", "reorder_ms", "density_ms", "force_ms", "total_ms")]
+
+
+_lib = None
+
+
+def lib_path() -> str:
+    return _LIB_PATH
+
+
+def load_library() -> C.CDLL:
+    """Loads libsph_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise RuntimeError(
+            f"{_LIB_PATH} is missing: build it with `python particle-fluid-simulation_amd/build.py`"
+        )
+    lib = C.CDLL(_LIB_PATH)
+    P = C.POINTER
+    vp = C.c_void_p
+    sig = {
+        "sph_params_derive": (None, [P(SphParams), C.c_double, C.c_uint32, C.c_double]),
+        "sph_world_create": (C.c_int, [P(SphParams), C.c_uint32, P(vp)]),
+        "sph_world_set_params": (C.c_int, [vp, P(SphParams)]),
+        "sph_world_upload_state": (C.c_int, [vp, vp, vp]),
+        "sph_world_step": (C.c_int, [vp, C.c_uint32]),
+        "sph_world_sync": (C.c_int, [vp]),
+        "sph_world_download_state": (C.c_int, [vp, vp, vp, vp]),
+        "sph_world_set_debug": (C.c_int, [vp, C.c_int]),
+        "sph_world_debug_dump": (C.c_int, [vp, C.c_int, vp, C.c_size_t]),
+        "sph_world_pass_times": (C.c_int, [vp, C.c_uint32, P(PassTimes)]),
+        "sph_world_footprint": (C.c_uint64, [C.c_uint32, C.c_uint32]),
+        "sph_world_destroy": (None, [vp]),
+        "sph_last_error": (C.c_char_p, []),
+        "sph_scene_fill_block": (C.c_int, [C.c_uint32, C.c_uint64, P(C.c_float * 3),
+                                           P(C.c_float * 3), C.c_float, vp, vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(rc: int) -> None:
+    if rc != SPH_OK:
+        msg = load_library().sph_last_error()
+        raise SphError(rc, msg.decode() if msg else "")
+
+
+def params_for(*, particle_radius: float, smoothing_radius: float, g: float, box_min, box_max,
+               frame_dt: float, substeps: int, particle_mass: float = 1.0,
+               pressure_multiplier: float, target_density: float, viscosity_strength: float = 0.0,
+               near_pressure_multiplier: float = 0.0, table_cells: int = 0) -> SphParams:
+    """Builds sph_params; dt and volume are derived like main.c:57,349."""
+    p = SphParams()
+    p.particle_radius = particle_radius
+    p.smoothing_radius = smoothing_radius
+    p.g = g
+    for a in range(3):
+        p.min_xyz[a] = box_min[a]
+        p.max_xyz[a] = box_max[a]
+    p.particle_mass = particle_mass
+    p.pressure_multiplier = pressure_multiplier
+    p.target_density = target_density
+    p.viscosity_strength = viscosity_strength
+    p.near_pressure_multiplier = near_pressure_multiplier
+    p.table_cells = table_cells
+    load_library().sph_params_derive(C.byref(p), frame_dt, substeps, smoothing_radius)
+    return p
+
+
+def fill_block(n: int, seed: int, lo, hi, vel_sigma: float = 0.0):
+    """Seeded uniform block fill (PCG32); returns (pos, vel) as float32 [n, 4]."""
+    pos = np.zeros((n, 4), np.float32)
+    vel = np.zeros((n, 4), np.float32)
+    lo3 = (C.c_float * 3)(*lo)
+    hi3 = (C.c_float * 3)(*hi)
+    check(load_library().sph_scene_fill_block(n, seed, C.byref(lo3), C.byref(hi3), vel_sigma,
+                                              pos.ctypes.data, vel.ctypes.data))
+    return pos, vel

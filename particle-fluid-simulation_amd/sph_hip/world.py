@@ -1,0 +1,82 @@
+"""World: one SPH scene on one GPU, through the sph_world_* C-ABI."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from .abi import DUMP_STAGES, PassTimes, SphParams, check, load_library
+
+
+class World:
+    """Mirror of the reference's per-frame flow (main.c:142-199): set params
+    (updateUniforms), step (recordCommands' substep loop), read back."""
+
+    def __init__(self, params: SphParams, n: int):
+        self._lib = load_library()
+        self._h = C.c_void_p()
+        check(self._lib.sph_world_create(C.byref(params), n, C.byref(self._h)))
+        self.n = n
+        self.params = params
+        self.table_cells = params.table_cells or n
+
+    def close(self) -> None:
+        if self._h:
+            self._lib.sph_world_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params: SphParams) -> None:
+        check(self._lib.sph_world_set_params(self._h, C.byref(params)))
+        self.params = params
+
+    def upload(self, pos: np.ndarray, vel: np.ndarray) -> None:
+        pos = np.ascontiguousarray(pos, np.float32).reshape(self.n, 4)
+        vel = np.ascontiguousarray(vel, np.float32).reshape(self.n, 4)
+        check(self._lib.sph_world_upload_state(self._h, pos.ctypes.data, vel.ctypes.data))
+
+    def step(self, frames: int = 1) -> None:
+        check(self._lib.sph_world_step(self._h, frames))
+
+    def sync(self) -> None:
+        check(self._lib.sph_world_sync(self._h))
+
+    def download(self, color: bool = False):
+        pos = np.empty((self.n, 4), np.float32)
+        vel = np.empty((self.n, 4), np.float32)
+        col = np.empty((self.n, 4), np.float32) if color else None
+        check(self._lib.sph_world_download_state(
+            self._h, pos.ctypes.data, vel.ctypes.data, col.ctypes.data if color else None))
+        return (pos, vel, col) if color else (pos, vel)
+
+    def set_debug(self, on: bool) -> None:
+        check(self._lib.sph_world_set_debug(self._h, 1 if on else 0))
+
+    def dump(self, stage: str) -> np.ndarray:
+        sid = DUMP_STAGES[stage]
+        if stage in ("cell_keys", "sorted_keys", "sorted_index"):
+            out = np.empty(self.n, np.uint32)
+        elif stage in ("start", "end"):
+            out = np.empty(self.table_cells, np.int32)
+        elif stage in ("density", "near_density"):
+            out = np.empty(self.n, np.float32)
+        else:
+            out = np.empty((self.n, 4), np.float32)
+        check(self._lib.sph_world_debug_dump(self._h, sid, out.ctypes.data, out.nbytes))
+        return out
+
+    def pass_times(self, substeps: int) -> dict:
+        t = PassTimes()
+        check(self._lib.sph_world_pass_times(self._h, substeps, C.byref(t)))
+        return {name: getattr(t, name) for name, _ in PassTimes._fields_}

@@ -1,0 +1,148 @@
+"""GPU parity: libsph_hip.so (through the C-ABI) against the FP32 oracle on the same seeded
+inputs.  Integer stages (cell keys, stable sorted pairs, start/end table) must be bit-exact;
+because the HIP kernels evaluate the reference's float expressions in the same order with
+the same rounding (no contraction, IEEE div/sqrt), the float stages are compared bit-exactly
+too, and the FP64 oracle's error is reported beside them.  The stated floating-point
+tolerance of the path, 1e-5 relative, is asserted against the FP64 oracle."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+STAGES_INT = ("cell_keys", "sorted_keys", "sorted_index", "start", "end")
+STAGES_F32 = ("pred_pos", "pred_vel", "density", "near_density", "vel_force")
+REL_TOL = 1e-5  # single-step float tolerance of the path vs the FP64 oracle
+
+
+def _run_one_substep(sph_hip, params, pos, vel):
+    n = pos.shape[0]
+    with sph_hip.World(params, n) as w:
+        w.set_debug(True)
+        w.upload(pos, vel)
+        p1 = sph_hip.SphParams.from_buffer_copy(params)
+        p1.substeps = 1
+        w.set_params(p1)
+        w.step(1)
+        out = {s: w.dump(s) for s in STAGES_INT + STAGES_F32}
+        out["pos"], out["vel"] = w.download()
+    return out
+
+
+def _assert_bitexact(name, got, ref):
+    got = np.asarray(got)
+    ref = np.asarray(ref)
+    assert got.shape == ref.shape, name
+    if got.dtype.kind == "f":
+        bad = np.flatnonzero(got.view(np.uint32).ravel() != ref.view(np.uint32).ravel())
+    else:
+        bad = np.flatnonzero(got.ravel() != ref.ravel())
+    assert bad.size == 0, (f"{name}: {bad.size} mismatches, first at {bad[:5]}: "
+                           f"gpu {got.ravel()[bad[:5]]} ref {ref.ravel()[bad[:5]]}")
+
+
+def _check_substep(sph_hip, orc, params, pos, vel):
+    print(f"case n={pos.shape[0]} T={params.table_cells or pos.shape[0]}", flush=True)
+    got = _run_one_substep(sph_hip, params, pos, vel)
+    p1 = sph_hip.SphParams.from_buffer_copy(params)
+    p1.substeps = 1
+    rpos, rvel, ref = orc.substep(p1, pos, vel)
+    for s in STAGES_INT + STAGES_F32:
+        _assert_bitexact(s, got[s], ref[s])
+    _assert_bitexact("pos", got["pos"], rpos)
+    _assert_bitexact("vel", got["vel"], rvel)
+    return got, ref
+
+
+def _rel_err(x32, x64, scale):
+    return np.max(np.abs(x32.astype(np.float64) - x64) / scale)
+
+
+def test_dam_break_16k_one_substep(gpu, orc):
+    from sph_hip.workloads import DAM_BREAK_16K as W
+    params = W.params()
+    pos, vel = W.state()
+    got, ref = _check_substep(gpu, orc, params, pos, vel)
+    o64 = orc.substep64(params, pos, vel, ref)
+    e_rho = _rel_err(got["density"], o64["density"], np.abs(o64["density"]))
+    assert e_rho <= REL_TOL, e_rho
+    print(f"density max rel err vs fp64: {e_rho:.3e}")
+
+
+def test_uniform_1m_one_substep_table_n(gpu, orc):
+    from sph_hip.workloads import UNIFORM_1M as W
+    params = W.params(table_cells=0)
+    pos, vel = W.state()
+    _check_substep(gpu, orc, params, pos, vel)
+
+
+def test_uniform_1m_one_substep_table_2m(gpu, orc):
+    from sph_hip.workloads import UNIFORM_1M as W
+    params = W.params(table_cells=1 << 21)
+    pos, vel = W.state()
+    _check_substep(gpu, orc, params, pos, vel)
+
+
+def test_dam_break_multi_frame_bitexact(gpu, orc):
+    """10 frames x 3 substeps: the whole trajectory matches the oracle bit for bit."""
+    from sph_hip.workloads import DAM_BREAK_16K as W
+    params = W.params()
+    pos, vel = W.state()
+    with gpu.World(params, W.n) as w:
+        w.upload(pos, vel)
+        w.step(10)
+        gp, gv = w.download()
+    rp, rv = orc.step(params, pos, vel, 10)
+    _assert_bitexact("pos", gp, rp)
+    _assert_bitexact("vel", gv, rv)
+
+
+@pytest.mark.parametrize("n,T", [(1, 1), (2, 0), (3000, 0), (5000, 1 << 16), (4096, 1 << 24),
+                                 (777, 5)])
+def test_edge_sizes(gpu, orc, n, T):
+    """Non-power-of-two N (mask with holes, as the reference's default N=300000), T=1 (no
+    sort pass), a 16-bit table (exactly two 8-bit passes), a 24-bit table (three full
+    passes) and a 5-cell table (heavy collisions)."""
+    from sph_hip.workloads import DAM_BREAK_16K as W
+    params = W.params(table_cells=T)
+    pos, vel = W.state(n)
+    _check_substep(gpu, orc, params, pos, vel)
+
+
+def test_single_cell_crowd(gpu, orc):
+    """All particles in one cell: one key run of length n, every pair a neighbour."""
+    from sph_hip.workloads import DAM_BREAK_16K as W
+    params = W.params()
+    pos, vel = gpu.fill_block(700, 11, (5.0, 3.0, 3.0), (5.3, 3.3, 3.3), 0.5)
+    _check_substep(gpu, orc, params, pos, vel)
+
+
+def test_walls_and_coincident(gpu, orc):
+    """Particles exactly on the box faces and coincident pairs (distance 0, skipped by the
+    force's 1e-7 check)."""
+    from sph_hip.workloads import DAM_BREAK_16K as W
+    params = W.params()
+    pos, vel = W.state(2048)
+    pos[:64, 0] = 0.0
+    pos[64:128, 0] = 10.0
+    pos[128:192, 2] = 6.0
+    pos[192:256] = pos[256:320]
+    vel[192:256] = vel[256:320]
+    _check_substep(gpu, orc, params, pos, vel)
+
+
+def test_rejects_bad_state(gpu):
+    from sph_hip.workloads import DAM_BREAK_16K as W
+    params = W.params()
+    pos, vel = W.state(256)
+    with gpu.World(params, 256) as w:
+        bad = pos.copy()
+        bad[3, 1] = np.nan
+        with pytest.raises(gpu.SphError):
+            w.upload(bad, vel)
+        bad = pos.copy()
+        bad[5, 0] = 10.5
+        with pytest.raises(gpu.SphError):
+            w.upload(bad, vel)
+        w.upload(pos, vel)  # still usable
+        w.step(1)
+        w.sync()
