@@ -107,6 +107,18 @@ typedef enum sph_dump_stage {
     SPH_DUMP_VEL_FORCE = 9     /* f32[4n] velocities after the force pass (debug)          */
 } sph_dump_stage;
 
+/* Neighbour-pass kernels.  All variants give bit-identical results.
+ *   DIRECT: one lane per particle, runs read from global memory (the reference's scheme).
+ *   BRICK:  workgroups own 4x4x4-cell bricks drained from a device work queue and read the
+ *           6x6x6 halo's runs from LDS.
+ *   AUTO:   BRICK when the box's brick grid is small enough (<= 2^28 bricks). */
+typedef enum sph_kernel_variant {
+    SPH_KERNEL_AUTO = 0,
+    SPH_KERNEL_DIRECT = 1,
+    SPH_KERNEL_BRICK = 2
+} sph_kernel_variant;
+int sph_world_set_kernel(sph_world *w, int variant);
+
 int sph_world_set_debug(sph_world *w, int enable);
 int sph_world_debug_dump(sph_world *w, int stage, void *dst, size_t bytes);
 
@@ -129,6 +141,30 @@ uint64_t sph_world_footprint(uint32_t n, uint32_t table_cells);
 void sph_world_destroy(sph_world *w);
 
 const char *sph_last_error(void);
+
+/* ---------------------------------------------------------------------------------------
+ * Batched step: many independent scenes (own parameters, box, table size) stepped by one
+ * chain of launches and one hipGraph per substep.  No reference counterpart: the reference
+ * steps one scene per frame (commands.c:193-402); each scene here follows exactly those
+ * semantics and its results are bit-identical to a sph_world of that scene alone.
+ * Every scene of a batch must use the same substep count.  At most 65535 scenes. */
+typedef struct sph_batch sph_batch;
+int sph_batch_create(uint32_t scene_count, const sph_params *params, const uint32_t *counts,
+                     sph_batch **out);
+int sph_batch_set_kernel(sph_batch *b, int variant);  /* sph_kernel_variant; AUTO picks per
+                                                         scene from its particle density */
+int sph_batch_upload_scene(sph_batch *b, uint32_t scene, const float *pos_vec4,
+                           const float *vel_vec4);
+int sph_batch_step(sph_batch *b, uint32_t frames);
+int sph_batch_sync(sph_batch *b);
+int sph_batch_download_scene(sph_batch *b, uint32_t scene, float *pos_vec4, float *vel_vec4,
+                             float *color_vec4);
+int sph_batch_set_debug(sph_batch *b, int enable);
+int sph_batch_debug_dump(sph_batch *b, uint32_t scene, int stage, void *dst, size_t bytes);
+int sph_batch_scene_uses_bricks(sph_batch *b, uint32_t scene);  /* 1, 0, or < 0 on error */
+int sph_batch_pass_times(sph_batch *b, uint32_t substeps, sph_pass_times *out);
+uint32_t sph_batch_particles(const sph_batch *b);
+void sph_batch_destroy(sph_batch *b);
 
 /* Seeded scene initialisers (host only).  Counterparts of initializePositionsBuffer /
  * initializeVelocitiesBuffer (renderer.c:569-645) for synthetic workloads: PCG32 streams,

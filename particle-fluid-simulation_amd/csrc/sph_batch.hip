@@ -1,0 +1,297 @@
+// Batched step: many independent scenes of uneven size in one chain of launches.
+//
+// Scene s owns particle slots [p_begin_s, p_begin_s + n_s) and table cells
+// [t_begin_s, t_begin_s + T_s).  Its cell keys are t_begin_s + (hash & (T_s - 1)), i.e. the
+// scene id sits above the local key bits of one segmented key space, so a single stable
+// radix sort orders every scene's particles inside its own slot range exactly as a sort of
+// that scene alone would.  The hash, reorder and table kernels look a slot's scene up in a
+// 2-byte scene-id array; the density and force kernels drain one work queue that holds the
+// bricks of every dense scene followed by lane chunks of the sparse ones.  One hipGraph per
+// substep covers all scenes.
+//
+// Each particle's arithmetic sees only its own scene's constants and runs, in the same
+// order as in a single-scene world, so every scene's result is bit-identical alone and in
+// any mix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <new>
+#include <vector>
+
+#include "../../include/sph_world.h"
+#include "sph_host.hpp"
+#include "sph_kernels.hpp"
+
+using namespace sph;
+
+struct sph_batch {
+    uint32_t count = 0;
+    std::vector<sph_params> params;
+    std::vector<uint32_t> n, T, p_begin, t_begin;
+    std::vector<SceneConst> scenes;
+    std::vector<uint8_t> brick_choice;  // per scene: 0 auto, 1 direct, 2 brick
+    uint32_t n_total = 0, t_total = 0, b_total = 0, substeps = 1;
+    int variant = SPH_KERNEL_AUTO;
+    hipStream_t stream = nullptr;
+    Chain chain;
+    SceneConst *d_scenes = nullptr;
+    uint16_t *d_scene_of = nullptr;
+    GraphCache graph;
+    bool stepped = false;
+    ManyScenes policy() const { return ManyScenes{d_scenes, d_scene_of, count}; }
+};
+
+namespace {
+
+// A scene runs through the brick kernels when its particles are dense enough for a brick's
+// halo staging to be reused: at least 0.25 particles per cell of the box its particles
+// span (16 per 4x4x4 brick).
+constexpr double kBrickDensity = 0.25;
+
+uint32_t choose_bricks(const SceneConst &c, uint32_t n, const float *pos4, int variant) {
+    if (variant == SPH_KERNEL_DIRECT) return 0u;
+    if (variant == SPH_KERNEL_BRICK) return 1u;
+    if (!pos4) return 1u;
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (uint32_t i = 0; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], pos4[4 * (size_t)i + a]);
+            hi[a] = std::max(hi[a], pos4[4 * (size_t)i + a]);
+        }
+    double cells = 1.0;
+    for (int a = 0; a < 3; ++a) cells *= std::floor((hi[a] - lo[a]) / c.R) + 1.0;
+    return (double)n / cells >= kBrickDensity ? 1u : 0u;
+}
+
+int upload_scene_table(sph_batch *b) {
+    hipError_t e = hipMemcpyAsync(b->d_scenes, b->scenes.data(), b->scenes.size() * sizeof(SceneConst),
+                                  hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    return e == hipSuccess ? SPH_OK : hip_fail(e, "scene table upload");
+}
+
+}  // namespace
+
+extern "C" {
+
+int sph_batch_create(uint32_t count, const sph_params *params, const uint32_t *counts,
+                     sph_batch **out) {
+    if (!out || !params || !counts) return fail(SPH_ERR_INVALID_ARG, "NULL argument");
+    *out = nullptr;
+    if (count == 0 || count > 65535u) return fail(SPH_ERR_INVALID_ARG, "scene count must be in [1, 65535]");
+    uint64_t n_total = 0, t_total = 0, b_total = 0;
+    for (uint32_t s = 0; s < count; ++s) {
+        int rc = validate_params(&params[s]);
+        if (rc) return fail(rc, "scene %u: %s", s, last_error());
+        if (counts[s] == 0) return fail(SPH_ERR_INVALID_ARG, "scene %u has no particles", s);
+        if (params[s].substeps != params[0].substeps)
+            return fail(SPH_ERR_INVALID_ARG, "scene %u: every scene of a batch needs the same substep count", s);
+        n_total += counts[s];
+        t_total += params[s].table_cells ? params[s].table_cells : counts[s];
+    }
+    if (n_total >= (1ull << 30)) return fail(SPH_ERR_CAPACITY, "batch exceeds 2^30 particles");
+    if (t_total >= (1ull << 31)) return fail(SPH_ERR_CAPACITY, "batch exceeds 2^31 table cells");
+    sph_batch *b = new (std::nothrow) sph_batch();
+    if (!b) return fail(SPH_ERR_OUT_OF_MEMORY, "host allocation");
+    b->count = count;
+    b->params.assign(params, params + count);
+    b->substeps = params[0].substeps;
+    uint32_t pb = 0, tb = 0, bb = 0;
+    for (uint32_t s = 0; s < count; ++s) {
+        const uint32_t T = params[s].table_cells ? params[s].table_cells : counts[s];
+        b->params[s].table_cells = T;
+        b->n.push_back(counts[s]);
+        b->T.push_back(T);
+        b->p_begin.push_back(pb);
+        b->t_begin.push_back(tb);
+        SceneConst c = make_scene_const(b->params[s], T, tb, pb, pb + counts[s], bb);
+        const uint64_t nb = scene_bricks(c);
+        if (nb == 0 || b_total + nb >= (1ull << 31)) c.use_bricks = 0u;  // stays on lanes
+        else { bb += (uint32_t)nb; b_total += nb; }
+        b->scenes.push_back(c);
+        pb += counts[s];
+        tb += T;
+    }
+    b->n_total = (uint32_t)n_total;
+    b->t_total = (uint32_t)t_total;
+    b->b_total = (uint32_t)b_total;
+    b->brick_choice.assign(count, 0);
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete b;
+        return hip_fail(e, "stream creation");
+    }
+    int rc = b->chain.allocate(b->n_total, b->t_total, b->b_total ? b->b_total : 1u);
+    if (rc) {
+        hipStreamDestroy(b->stream);
+        delete b;
+        return rc;
+    }
+    b->chain.bricks = true;  // the queue kernels also carry the lane-per-particle scenes
+    std::vector<uint16_t> scene_of(b->n_total);
+    for (uint32_t s = 0; s < count; ++s)
+        std::fill(scene_of.begin() + b->p_begin[s], scene_of.begin() + b->p_begin[s] + b->n[s], (uint16_t)s);
+    e = hipMalloc((void **)&b->d_scenes, count * sizeof(SceneConst));
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_scene_of, (size_t)b->n_total * 2);
+    if (e == hipSuccess)
+        e = hipMemcpy(b->d_scene_of, scene_of.data(), (size_t)b->n_total * 2, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        sph_batch_destroy(b);
+        return fail(SPH_ERR_OUT_OF_MEMORY, "scene tables: %s", hipGetErrorString(e));
+    }
+    if ((rc = upload_scene_table(b))) {
+        sph_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return SPH_OK;
+}
+
+int sph_batch_set_kernel(sph_batch *b, int variant) {
+    if (!b) return fail(SPH_ERR_INVALID_ARG, "batch is NULL");
+    if (variant < SPH_KERNEL_AUTO || variant > SPH_KERNEL_BRICK)
+        return fail(SPH_ERR_INVALID_ARG, "unknown kernel variant %d", variant);
+    b->variant = variant;
+    for (uint32_t s = 0; s < b->count; ++s) {
+        if (variant == SPH_KERNEL_DIRECT) b->scenes[s].use_bricks = 0u;
+        else if (variant == SPH_KERNEL_BRICK) b->scenes[s].use_bricks = scene_bricks(b->scenes[s]) ? 1u : 0u;
+    }
+    hipError_t e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    b->graph.drop();
+    return upload_scene_table(b);
+}
+
+int sph_batch_upload_scene(sph_batch *b, uint32_t scene, const float *pos4, const float *vel4) {
+    if (!b) return fail(SPH_ERR_INVALID_ARG, "batch is NULL");
+    if (scene >= b->count) return fail(SPH_ERR_INVALID_ARG, "scene %u out of range", scene);
+    int rc = validate_state(b->params[scene], b->n[scene], pos4, vel4);
+    if (rc) return rc;
+    const size_t off = b->p_begin[scene], bytes = (size_t)b->n[scene] * 16;
+    hipError_t e = hipMemcpyAsync(b->chain.b.pos + off, pos4, bytes, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(b->chain.b.vel + off, vel4, bytes, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return hip_fail(e, "upload");
+    if (b->variant == SPH_KERNEL_AUTO && scene_bricks(b->scenes[scene])) {
+        const uint32_t ub = choose_bricks(b->scenes[scene], b->n[scene], pos4, b->variant);
+        if (ub != b->scenes[scene].use_bricks) {
+            b->scenes[scene].use_bricks = ub;
+            b->graph.drop();
+            if ((rc = upload_scene_table(b))) return rc;
+        }
+    }
+    b->stepped = false;
+    return SPH_OK;
+}
+
+int sph_batch_step(sph_batch *b, uint32_t frames) {
+    if (!b) return fail(SPH_ERR_INVALID_ARG, "batch is NULL");
+    const uint64_t substeps = (uint64_t)frames * b->substeps;
+    if (substeps == 0) return SPH_OK;
+    if (!b->graph.exec) {
+        int rc = b->graph.ensure_begin(b->stream);
+        if (rc) return rc;
+        hipError_t le = b->chain.enqueue(b->stream, b->policy(), nullptr);
+        rc = b->graph.end(b->stream, le);
+        if (rc) return rc;
+    }
+    for (uint64_t s = 0; s < substeps; ++s) {
+        hipError_t e = hipGraphLaunch(b->graph.exec, b->stream);
+        if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+    }
+    b->stepped = true;
+    return SPH_OK;
+}
+
+int sph_batch_sync(sph_batch *b) {
+    if (!b) return fail(SPH_ERR_INVALID_ARG, "batch is NULL");
+    hipError_t e = hipStreamSynchronize(b->stream);
+    return e == hipSuccess ? SPH_OK : hip_fail(e, "hipStreamSynchronize");
+}
+
+int sph_batch_download_scene(sph_batch *b, uint32_t scene, float *pos4, float *vel4, float *color4) {
+    if (!b) return fail(SPH_ERR_INVALID_ARG, "batch is NULL");
+    if (scene >= b->count) return fail(SPH_ERR_INVALID_ARG, "scene %u out of range", scene);
+    const size_t off = b->p_begin[scene], n = b->n[scene];
+    std::vector<float> tmp;
+    float *vdst = vel4;
+    if (color4 && !vel4) {
+        tmp.resize(n * 4);
+        vdst = tmp.data();
+    }
+    hipError_t e = hipSuccess;
+    if (pos4) e = hipMemcpyAsync(pos4, b->chain.b.pos + off, n * 16, hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess && vdst)
+        e = hipMemcpyAsync(vdst, b->chain.b.vel + off, n * 16, hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return hip_fail(e, "download");
+    if (color4) speed_ramp((uint32_t)n, vdst, color4);
+    return SPH_OK;
+}
+
+int sph_batch_set_debug(sph_batch *b, int enable) {
+    if (!b) return fail(SPH_ERR_INVALID_ARG, "batch is NULL");
+    hipError_t e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    int rc = b->chain.set_debug(enable != 0);
+    if (rc) return rc;
+    b->graph.drop();
+    return SPH_OK;
+}
+
+int sph_batch_debug_dump(sph_batch *b, uint32_t scene, int stage, void *dst, size_t bytes) {
+    if (!b || !dst) return fail(SPH_ERR_INVALID_ARG, "NULL argument");
+    if (scene >= b->count) return fail(SPH_ERR_INVALID_ARG, "scene %u out of range", scene);
+    if (!b->stepped) return fail(SPH_ERR_STATE, "no substep has run since the last upload");
+    return dump_stage(b->chain, b->stream, stage, b->p_begin[scene], b->p_begin[scene] + b->n[scene],
+                      b->t_begin[scene], b->t_begin[scene] + b->T[scene], dst, bytes);
+}
+
+int sph_batch_scene_uses_bricks(sph_batch *b, uint32_t scene) {
+    if (!b || scene >= b->count) return fail(SPH_ERR_INVALID_ARG, "bad batch or scene");
+    return (int)b->scenes[scene].use_bricks;
+}
+
+int sph_batch_pass_times(sph_batch *b, uint32_t substeps, sph_pass_times *out) {
+    if (!b || !out) return fail(SPH_ERR_INVALID_ARG, "NULL argument");
+    if (substeps == 0) return fail(SPH_ERR_INVALID_ARG, "substeps must be >= 1");
+    PassEvents ev;
+    int rc = ev.create();
+    if (rc) { ev.destroy(); return rc; }
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (uint32_t s = 0; s < substeps; ++s) {
+        hipError_t e = b->chain.enqueue(b->stream, b->policy(), &ev);
+        if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+        if (e != hipSuccess) { ev.destroy(); return hip_fail(e, "timed substep"); }
+        for (int q = 0; q < 5; ++q) {
+            float ms = 0.0f;
+            hipEventElapsedTime(&ms, ev.ev[q], ev.ev[q + 1]);
+            acc[q] += ms;
+        }
+    }
+    ev.destroy();
+    out->hash_ms = acc[0] / substeps;
+    out->sort_ms = acc[1] / substeps;
+    out->reorder_ms = acc[2] / substeps;
+    out->density_ms = acc[3] / substeps;
+    out->force_ms = acc[4] / substeps;
+    out->total_ms = out->hash_ms + out->sort_ms + out->reorder_ms + out->density_ms + out->force_ms;
+    b->stepped = true;
+    return SPH_OK;
+}
+
+uint32_t sph_batch_particles(const sph_batch *b) { return b ? b->n_total : 0u; }
+
+void sph_batch_destroy(sph_batch *b) {
+    if (!b) return;
+    if (b->stream) hipStreamSynchronize(b->stream);
+    b->graph.drop();
+    b->chain.release();
+    if (b->d_scenes) hipFree(b->d_scenes);
+    if (b->d_scene_of) hipFree(b->d_scene_of);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+}
+
+}  // extern "C"

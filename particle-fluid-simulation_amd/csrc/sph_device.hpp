@@ -30,7 +30,10 @@ struct SceneConst {
     uint32_t table_base;     // first cell of this scene in the table
     uint32_t p_begin;        // first particle of this scene
     uint32_t p_end;          // one past the last particle
-    uint32_t pad;
+    int32_t gx, gy, gz;      // cells per axis inside the box: floor((max-min)/R) + 1
+    uint32_t nbx, nby, nbz;  // bricks per axis (brick = kBrick^3 cells)
+    uint32_t brick_base;     // first global brick id of this scene
+    uint32_t use_bricks;     // 1: brick kernels; 0: one lane per particle (sparse scenes)
 };
 
 // cellCoordToHash (updateCellEntries.comp:36-52): u32 arithmetic, wraps mod 2^32.
@@ -86,12 +89,51 @@ __device__ __forceinline__ void kick_drift(const SceneConst &c, float3 &p, float
 struct OneScene {
     SceneConst c;
     __device__ __forceinline__ const SceneConst &scene(uint32_t) const { return c; }
+    __device__ __forceinline__ const SceneConst &scene_of_brick(uint32_t) const { return c; }
 };
 
 struct ManyScenes {
     const SceneConst *__restrict__ table;
     const uint16_t *__restrict__ scene_of;  // scene id per particle slot
+    uint32_t count;                         // number of scenes
     __device__ __forceinline__ SceneConst scene(uint32_t i) const { return table[scene_of[i]]; }
+    // Scene whose brick range [brick_base, next brick_base) holds global brick id `b`.
+    __device__ __forceinline__ SceneConst scene_of_brick(uint32_t b) const {
+        uint32_t lo = 0, hi = count;
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (table[mid].brick_base <= b) lo = mid; else hi = mid;
+        }
+        return table[lo];
+    }
 };
+
+// Edge of a brick in cells.  A workgroup stages the (kBrick+2)^3 halo cells of its brick
+// in LDS and evaluates every particle whose current cell lies in the brick.
+constexpr int kBrick = 4;
+
+// The neighbour walk of the density pass (calculateDensities.comp:114-131) for sample
+// point s.  Per axis only three sample coordinates exist (s + R*{-1,0,1}), so their cell
+// coordinates and in-box flags are computed once; the 27 cells are then visited in the
+// reference's (dz, dy, dx) order.  `visit(cx, cy, cz)` is called for each non-skipped cell.
+struct AxisCells {
+    int32_t c0, c1, c2;  // cell of s - R, s, s + R
+    bool in0, in1, in2;  // that sample coordinate lies inside [lo, hi]
+    // Selected with compares rather than array indexing so nothing spills to scratch.
+    __device__ __forceinline__ int32_t c(int d) const { return d == 0 ? c0 : (d == 1 ? c1 : c2); }
+    __device__ __forceinline__ bool in(int d) const { return d == 0 ? in0 : (d == 1 ? in1 : in2); }
+};
+
+__device__ __forceinline__ AxisCells sample_axis(float s, float lo, float hi, float R) {
+    AxisCells a;
+    const float q0 = s + R * -1.0f, q1 = s + R * 0.0f, q2 = s + R * 1.0f;
+    a.in0 = !(q0 < lo) && !(q0 > hi);
+    a.in1 = !(q1 < lo) && !(q1 > hi);
+    a.in2 = !(q2 < lo) && !(q2 > hi);
+    a.c0 = cell_of(q0, lo, R);
+    a.c1 = cell_of(q1, lo, R);
+    a.c2 = cell_of(q2, lo, R);
+    return a;
+}
 
 }  // namespace sph

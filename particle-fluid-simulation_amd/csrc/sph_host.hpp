@@ -14,7 +14,9 @@ int hip_fail(hipError_t e, const char *what);
 const char *last_error();
 
 SceneConst make_scene_const(const sph_params &p, uint32_t table_cells, uint32_t table_base,
-                            uint32_t p_begin, uint32_t p_end);
+                            uint32_t p_begin, uint32_t p_end, uint32_t brick_base);
+// Bricks of one scene's cell grid (0 when the grid is too large to tile).
+uint64_t scene_bricks(const SceneConst &c);
 int validate_params(const sph_params *p);
 int validate_state(const sph_params &p, uint32_t n, const float *pos4, const float *vel4);
 uint32_t key_bits_for(uint64_t table_total);
@@ -31,8 +33,10 @@ struct PassEvents {
 struct Chain {
     ChainBuffers b{};
     uint32_t n = 0, table_total = 0, key_bits = 0, passes = 0, tiles = 0, lookback_words = 0;
+    uint32_t brick_total = 0;  // bricks over all scenes; 0 disables the brick kernels
+    bool bricks = false;       // density/force through the brick kernels (else per lane)
 
-    int allocate(uint32_t n, uint32_t table_total);
+    int allocate(uint32_t n, uint32_t table_total, uint32_t brick_total);
     int set_debug(bool on);
     void release();
     const uint32_t *sorted_keys() const;

@@ -6,6 +6,7 @@
 // does not depend on launch geometry or on which other scenes share the launch.
 #include "sph_device.hpp"
 #include "sph_kernels.hpp"
+#include "sph_pairs.hpp"
 
 namespace sph {
 
@@ -22,7 +23,7 @@ __global__ __launch_bounds__(kBlock) void k_hash(
     Scenes sc, const float4 *__restrict__ pos, uint32_t *__restrict__ keys, uint32_t n,
     uint32_t passes, uint32_t *__restrict__ ghist, uint32_t hash_blocks,
     int2 *__restrict__ table, uint32_t table_total, uint32_t *__restrict__ lookback,
-    uint32_t lookback_words, uint32_t *__restrict__ tile_ctr) {
+    uint32_t lookback_words, uint32_t *__restrict__ tile_ctr, uint32_t *__restrict__ counters) {
     __shared__ uint32_t s_hist[4][256];
     const uint32_t tid = threadIdx.x;
     if (blockIdx.x < hash_blocks) {
@@ -50,6 +51,7 @@ __global__ __launch_bounds__(kBlock) void k_hash(
         const uint32_t b = blockIdx.x - hash_blocks;
         const uint32_t nb = gridDim.x - hash_blocks;
         if (b == 0 && tid < 4) tile_ctr[tid] = 0u;
+        if (b == 0 && tid < 2 && counters) counters[tid] = 0u;
         for (uint32_t i = b * kBlock + tid; i < table_total; i += nb * kBlock) table[i] = make_int2(-1, -1);
         for (uint32_t i = b * kBlock + tid; i < lookback_words; i += nb * kBlock) lookback[i] = 0u;
     }
@@ -65,38 +67,69 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
     uint32_t n, const float4 *__restrict__ pos, const float4 *__restrict__ vel,
     float4 *__restrict__ pos_sorted, float4 *__restrict__ vel_sorted,
     float4 *__restrict__ pred_pos, float4 *__restrict__ dbg_pred_vel, int2 *__restrict__ table,
-    uint32_t *__restrict__ ghist) {
+    uint32_t *__restrict__ ghist, uint32_t *__restrict__ hfull, uint32_t *__restrict__ brick_flag,
+    uint32_t *__restrict__ stray, uint32_t *__restrict__ counters) {
     const uint32_t tid = threadIdx.x;
     if (blockIdx.x == 0)
         for (uint32_t i = tid; i < 4u * 256u; i += kBlock) ghist[i] = 0u;
     const uint32_t j = blockIdx.x * kBlock + tid;
-    if (j >= n) return;
-    const uint32_t key = skeys[j];
-    const uint32_t idx = sidx ? sidx[j] : j;
-    const float4 P = pos[idx];
-    const float4 V = vel[idx];
-    const SceneConst c = sc.scene(j);
-    float3 p = make_float3(P.x, P.y, P.z), v = make_float3(V.x, V.y, V.z);
-    kick_drift<false>(c, p, v);
-    pos_sorted[j] = P;
-    vel_sorted[j] = V;
-    pred_pos[j] = make_float4(p.x, p.y, p.z, 1.0f);
-    if (dbg_pred_vel) dbg_pred_vel[j] = make_float4(v.x, v.y, v.z, 1.0f);
-    int *tbl = reinterpret_cast<int *>(table);
-    if (j == 0) {
-        tbl[2u * key] = 0;
-    } else {
-        const uint32_t prev = skeys[j - 1u];
-        if (key != prev) {
-            tbl[2u * key] = (int)j;
-            tbl[2u * prev + 1u] = (int)j;
+    bool to_stray = false;
+    if (j < n) {
+        const uint32_t key = skeys[j];
+        const uint32_t idx = sidx ? sidx[j] : j;
+        const float4 P = pos[idx];
+        const float4 V = vel[idx];
+        const SceneConst c = sc.scene(j);
+        float3 p = make_float3(P.x, P.y, P.z), v = make_float3(V.x, V.y, V.z);
+        kick_drift<false>(c, p, v);
+        pos_sorted[j] = P;
+        vel_sorted[j] = V;
+        pred_pos[j] = make_float4(p.x, p.y, p.z, 1.0f);
+        if (dbg_pred_vel) dbg_pred_vel[j] = make_float4(v.x, v.y, v.z, 1.0f);
+        int *tbl = reinterpret_cast<int *>(table);
+        if (j == 0) {
+            tbl[2u * key] = 0;
+        } else {
+            const uint32_t prev = skeys[j - 1u];
+            if (key != prev) {
+                tbl[2u * key] = (int)j;
+                tbl[2u * prev + 1u] = (int)j;
+            }
+        }
+        if (j == n - 1u) tbl[2u * key + 1u] = (int)n;
+        if (brick_flag) {
+            // Work for the brick kernels: the full hash of the particle's current cell
+            // tells its own cell's run apart from colliding cells; its brick is marked
+            // non-empty.  Particles of one-lane-per-particle scenes and particles outside
+            // the brick grid go to the stray list instead.
+            const int32_t cx = cell_of(P.x, c.minx, c.R), cy = cell_of(P.y, c.miny, c.R),
+                          cz = cell_of(P.z, c.minz, c.R);
+            hfull[j] = cell_hash((uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
+            if (c.use_bricks && cx >= 0 && cy >= 0 && cz >= 0 && cx < c.gx && cy < c.gy && cz < c.gz) {
+                const uint32_t b = c.brick_base + (uint32_t)(cx / kBrick) +
+                                   c.nbx * ((uint32_t)(cy / kBrick) + c.nby * (uint32_t)(cz / kBrick));
+                brick_flag[b] = 1u;
+            } else {
+                to_stray = true;
+            }
         }
     }
-    if (j == n - 1u) tbl[2u * key + 1u] = (int)n;
+    if (brick_flag) {
+        // One atomic per wave keeps a wave's slots consecutive in the stray list.
+        const uint64_t m = __ballot(to_stray);
+        if (m) {
+            const int lane = tid & 63;
+            const int leader = __ffsll((long long)m) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&counters[1], (uint32_t)__popcll(m));
+            base = __shfl(base, leader, 64);
+            if (to_stray) stray[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = j;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------------------
-// Density: calculateDensities.comp:96-155.
+// Density: calculateDensities.comp:96-155, one lane per particle, runs read from global.
 template <class Scenes>
 __global__ __launch_bounds__(kBlock) void k_density(
     Scenes sc, const float4 *__restrict__ pred_pos, const int2 *__restrict__ table,
@@ -105,36 +138,19 @@ __global__ __launch_bounds__(kBlock) void k_density(
     if (j >= n) return;
     const SceneConst c = sc.scene(j);
     const float4 s = pred_pos[j];
+    const AxisCells ax = sample_axis(s.x, c.minx, c.maxx, c.R);
+    const AxisCells ay = sample_axis(s.y, c.miny, c.maxy, c.R);
+    const AxisCells az = sample_axis(s.z, c.minz, c.maxz, c.R);
     float M = 0.0f, Mn = 0.0f;
-    for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                // samplePoint + smoothingRadius * vec3(dx, dy, dz), then samplePointToKey
-                const float qx = s.x + c.R * (float)dx;
-                const float qy = s.y + c.R * (float)dy;
-                const float qz = s.z + c.R * (float)dz;
-                if (qx < c.minx || qy < c.miny || qz < c.minz) continue;
-                if (qx > c.maxx || qy > c.maxy || qz > c.maxz) continue;
-                const uint32_t key = c.table_base +
-                    (cell_hash((uint32_t)cell_of(qx, c.minx, c.R), (uint32_t)cell_of(qy, c.miny, c.R),
-                               (uint32_t)cell_of(qz, c.minz, c.R)) & c.mask);
-                const int2 se = table[key];
+    for (int dz = 0; dz < 3; ++dz)
+        for (int dy = 0; dy < 3; ++dy)
+            for (int dx = 0; dx < 3; ++dx) {
+                if (!(ax.in(dx) && ay.in(dy) && az.in(dz))) continue;
+                const int2 se = table[key_of_cell(c, ax.c(dx), ay.c(dy), az.c(dz))];
                 if (se.x == -1) continue;
-                for (int i = se.x; i < se.y; ++i) {
-                    const float4 p = pred_pos[i];
-                    const float ex = p.x - s.x, ey = p.y - s.y, ez = p.z - s.z;
-                    const float d2 = ex * ex + ey * ey + ez * ez;
-                    if (d2 > c.R2_pre) continue;  // sqrt(d2) >= R: the kernels are 0
-                    const float dist = sqrtf(d2);
-                    if (dist >= c.R) continue;
-                    const float t = c.R - dist;
-                    M = M + c.m * (t * t);
-                    Mn = Mn + c.m * (t * t * t);
-                }
+                density_run(c, pred_pos + se.x, se.y - se.x, s, M, Mn);
             }
-    const float density = M / c.volume;
-    const float near_density = Mn * c.near_vol_inv;
-    dens[j] = make_float2(fmaxf(0.01f, density), fmaxf(0.01f, near_density));
+    dens[j] = density_finish(c, M, Mn);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -153,64 +169,23 @@ __global__ __launch_bounds__(kBlock) void k_force(
     const float4 s = pred_pos[j];
     const float4 P = pos_sorted[j];
     const float4 V = vel_sorted[j];
-    float3 ps = make_float3(P.x, P.y, P.z), u = make_float3(V.x, V.y, V.z);
-    kick_drift<false>(c, ps, u);  // u = predictedVelocities[j]
     const float2 myd = dens[j];
-    const float my_density = myd.x;
-    const float near_pressure = myd.y * c.kn;
-    const float my_pressure = (my_density - c.rho0) * c.k;
-    const bool outside = s.x < c.minx || s.y < c.miny || s.z < c.minz || s.x > c.maxx ||
-                         s.y > c.maxy || s.z > c.maxz;
-    const int32_t cx = outside ? -1 : cell_of(s.x, c.minx, c.R);
-    const int32_t cy = outside ? -1 : cell_of(s.y, c.miny, c.R);
-    const int32_t cz = outside ? -1 : cell_of(s.z, c.minz, c.R);
+    const ForceSelf me = force_self(c, j, s, P, V, myd);
+    const int3 c0 = force_cell(c, s);
     float3 Fp = make_float3(0.0f, 0.0f, 0.0f), Fv = make_float3(0.0f, 0.0f, 0.0f);
     for (int dz = -1; dz <= 1; ++dz)
         for (int dy = -1; dy <= 1; ++dy)
             for (int dx = -1; dx <= 1; ++dx) {
-                const uint32_t key = c.table_base +
-                    (cell_hash((uint32_t)(cx + dx), (uint32_t)(cy + dy), (uint32_t)(cz + dz)) & c.mask);
-                const int2 se = table[key];
+                const int2 se = table[key_of_cell(c, c0.x + dx, c0.y + dy, c0.z + dz)];
                 if (se.x == -1) continue;
-                for (int i = se.x; i < se.y; ++i) {
-                    if ((uint32_t)i == j) continue;
-                    const float4 pi = pred_pos[i];
-                    const float ex = pi.x - s.x, ey = pi.y - s.y, ez = pi.z - s.z;
-                    const float d2 = ex * ex + ey * ey + ez * ez;
-                    if (d2 > c.R2) continue;
-                    const float dist = sqrtf(d2);
-                    if (fabsf(dist) < 1e-7f) continue;
-                    const float dirx = ex / dist, diry = ey / dist, dirz = ez / dist;
-                    const float2 di = dens[i];
-                    const float slope = dist >= c.R ? 0.0f : (dist - c.R) * c.slope_scale;
-                    const float shared = ((di.x - c.rho0) * c.k + my_pressure) / 2.0f;
-                    const float a = shared * slope * c.m / di.x;
-                    Fp.x = Fp.x + a * dirx;
-                    Fp.y = Fp.y + a * diry;
-                    Fp.z = Fp.z + a * dirz;
-                    const float infl = dist >= c.R ? 0.0f : (c.R - dist) * (c.R - dist);
-                    const float b = c.m * infl / di.x;
-                    const float4 vi = vel_sorted[i];
-                    Fv.x = Fv.x + b * (vi.x - u.x);
-                    Fv.y = Fv.y + b * (vi.y - u.y);
-                    Fv.z = Fv.z + b * (vi.z - u.z);
-                    const float shared_near = (near_pressure + di.y * c.kn) / 2.0f;
-                    const float x = c.R - dist;
-                    const float nslope = dist >= c.R ? 0.0f : x * x * c.near_slope_scale;
-                    const float cn = shared_near * nslope * c.m / di.y;
-                    Fp.x = Fp.x + cn * dirx;
-                    Fp.y = Fp.y + cn * diry;
-                    Fp.z = Fp.z + cn * dirz;
-                }
+                force_run(c, pred_pos + se.x, vel_sorted + se.x, dens + se.x, se.y - se.x,
+                          (uint32_t)se.x, me, Fp, Fv);
             }
-    float3 v = make_float3(V.x + (Fp.x + Fv.x + 0.0f) / my_density * c.dt,
-                           V.y + (Fp.y + Fv.y + 0.0f) / my_density * c.dt,
-                           V.z + (Fp.z + Fv.z + 0.0f) / my_density * c.dt);
-    if (dbg_vel_force) dbg_vel_force[j] = make_float4(v.x, v.y, v.z, V.w + 1.0f);
-    float3 p = make_float3(P.x, P.y, P.z);
-    kick_drift<true>(c, p, v);
-    pos_out[j] = make_float4(p.x, p.y, p.z, 1.0f);
-    vel_out[j] = make_float4(v.x, v.y, v.z, 1.0f);
+    float4 po, vo, vf;
+    force_finish(c, P, V, myd.x, Fp, Fv, po, vo, vf);
+    if (dbg_vel_force) dbg_vel_force[j] = vf;
+    pos_out[j] = po;
+    vel_out[j] = vo;
 }
 
 inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1u) / kBlock; }
@@ -229,16 +204,18 @@ hipError_t launch_hash(hipStream_t stream, const Scenes &sc, const ChainBuffers 
     if (clear_blocks > 2048u) clear_blocks = 2048u;
     hipLaunchKernelGGL(k_hash<Scenes>, dim3(hash_blocks + clear_blocks), dim3(kBlock), 0, stream,
                        sc, b.pos, b.keys, n, passes, b.ghist, hash_blocks, b.table, table_total,
-                       b.lookback, lookback_words, b.tile_ctr);
+                       b.lookback, lookback_words, b.tile_ctr, b.counters);
     return hipGetLastError();
 }
 
 template <class Scenes>
 hipError_t launch_reorder(hipStream_t stream, const Scenes &sc, const ChainBuffers &b,
-                          const uint32_t *sorted_keys, const uint32_t *sorted_idx, uint32_t n) {
+                          const uint32_t *sorted_keys, const uint32_t *sorted_idx, uint32_t n,
+                          bool bricks) {
     hipLaunchKernelGGL(k_reorder<Scenes>, dim3(blocks_for(n)), dim3(kBlock), 0, stream, sc,
                        sorted_keys, sorted_idx, n, b.pos, b.vel, b.pos_sorted, b.vel_sorted,
-                       b.pred_pos, b.dbg_pred_vel, b.table, b.ghist);
+                       b.pred_pos, b.dbg_pred_vel, b.table, b.ghist, b.hfull,
+                       bricks ? b.brick_flag : nullptr, b.stray, b.counters);
     return hipGetLastError();
 }
 
@@ -261,7 +238,7 @@ hipError_t launch_force(hipStream_t stream, const Scenes &sc, const ChainBuffers
     template hipError_t launch_hash<S>(hipStream_t, const S &, const ChainBuffers &, uint32_t,   \
                                        uint32_t, uint32_t, uint32_t);                            \
     template hipError_t launch_reorder<S>(hipStream_t, const S &, const ChainBuffers &,          \
-                                          const uint32_t *, const uint32_t *, uint32_t);         \
+                                          const uint32_t *, const uint32_t *, uint32_t, bool);   \
     template hipError_t launch_density<S>(hipStream_t, const S &, const ChainBuffers &, uint32_t); \
     template hipError_t launch_force<S>(hipStream_t, const S &, const ChainBuffers &, uint32_t);
 

@@ -27,6 +27,12 @@ struct ChainBuffers {
     uint32_t *tile_ctr;  // [4] tile counters
     float4 *dbg_pred_vel;   // optional (debug capture): predicted velocities, sorted
     float4 *dbg_vel_force;  // optional (debug capture): velocity after force, sorted
+    // Brick work queue (see sph_brick.hip).
+    uint32_t *hfull;        // full cell hash of each slot's current position, sorted order
+    uint32_t *brick_flag;   // [brick_total] non-empty marks
+    uint32_t *queue;        // [brick_total] non-empty bricks of this substep
+    uint32_t *stray;        // [n] slots outside every brick grid
+    uint32_t *counters;     // [0] queue length [1] strays [2] density drain [3] force drain
 };
 
 uint32_t radix_tile_size();
@@ -44,9 +50,11 @@ hipError_t launch_hash(hipStream_t stream, const Scenes &sc, const ChainBuffers 
                        uint32_t lookback_words);
 
 // Gather into sorted order, predicted positions, cell start/end table; clears histograms.
+// With `bricks`, also marks non-empty bricks and lists stray particles.
 template <class Scenes>
 hipError_t launch_reorder(hipStream_t stream, const Scenes &sc, const ChainBuffers &b,
-                          const uint32_t *sorted_keys, const uint32_t *sorted_idx, uint32_t n);
+                          const uint32_t *sorted_keys, const uint32_t *sorted_idx, uint32_t n,
+                          bool bricks);
 
 template <class Scenes>
 hipError_t launch_density(hipStream_t stream, const Scenes &sc, const ChainBuffers &b,
@@ -56,5 +64,16 @@ hipError_t launch_density(hipStream_t stream, const Scenes &sc, const ChainBuffe
 template <class Scenes>
 hipError_t launch_force(hipStream_t stream, const Scenes &sc, const ChainBuffers &b,
                         uint32_t n);
+
+// Brick path (sph_brick.hip): compacts the non-empty bricks into the work queue, then
+// persistent workgroups drain it for the density and the force + integrate passes.
+void brick_prepare();  // caches occupancy-derived grid sizes; call outside graph capture
+hipError_t launch_queue(hipStream_t stream, const ChainBuffers &b, uint32_t brick_total);
+template <class Scenes>
+hipError_t launch_density_brick(hipStream_t stream, const Scenes &sc, const ChainBuffers &b,
+                                uint32_t n, uint32_t brick_total);
+template <class Scenes>
+hipError_t launch_force_brick(hipStream_t stream, const Scenes &sc, const ChainBuffers &b,
+                              uint32_t n, uint32_t brick_total);
 
 }  // namespace sph

@@ -40,7 +40,7 @@ const char *last_error() { return g_last_error.c_str(); }
 // Constants derived from the parameter block, computed on the host in binary32 exactly as
 // the shaders derive them (calculateDensities.comp:110, applyPressure...comp:243,258,282).
 SceneConst make_scene_const(const sph_params &p, uint32_t table_cells, uint32_t table_base,
-                            uint32_t p_begin, uint32_t p_end) {
+                            uint32_t p_begin, uint32_t p_end, uint32_t brick_base) {
     const float kPi = 3.14159265359f;
     SceneConst c{};
     const float R = p.smoothing_radius;
@@ -68,7 +68,22 @@ SceneConst make_scene_const(const sph_params &p, uint32_t table_cells, uint32_t 
     c.table_base = table_base;
     c.p_begin = p_begin;
     c.p_end = p_end;
+    // Cells a position inside the box can fall in: floor((max - min) / R) + 1 per axis,
+    // evaluated in float exactly as the kernels evaluate a particle's cell.
+    c.gx = (int32_t)floorf((c.maxx - c.minx) / R) + 1;
+    c.gy = (int32_t)floorf((c.maxy - c.miny) / R) + 1;
+    c.gz = (int32_t)floorf((c.maxz - c.minz) / R) + 1;
+    c.nbx = (uint32_t)((c.gx + kBrick - 1) / kBrick);
+    c.nby = (uint32_t)((c.gy + kBrick - 1) / kBrick);
+    c.nbz = (uint32_t)((c.gz + kBrick - 1) / kBrick);
+    c.brick_base = brick_base;
+    c.use_bricks = 1u;
     return c;
+}
+
+uint64_t scene_bricks(const SceneConst &c) {
+    const uint64_t b = (uint64_t)c.nbx * c.nby * c.nbz;
+    return b > (1ull << 28) ? 0 : b;
 }
 
 int validate_params(const sph_params *p) {
@@ -115,9 +130,11 @@ uint32_t key_bits_for(uint64_t table_total) {
 // ---------------------------------------------------------------------------------------
 // Chain: buffers and launches of one substep, shared by the single world and the batch.
 
-int Chain::allocate(uint32_t n_, uint32_t table_total_) {
+int Chain::allocate(uint32_t n_, uint32_t table_total_, uint32_t brick_total_) {
     n = n_;
     table_total = table_total_;
+    brick_total = brick_total_;
+    bricks = brick_total > 0;
     key_bits = key_bits_for(table_total);
     passes = (key_bits + 7u) / 8u;
     tiles = (n + radix_tile_size() - 1u) / radix_tile_size();
@@ -132,7 +149,10 @@ int Chain::allocate(uint32_t n_, uint32_t table_total_) {
         {(void **)&b.vel_sorted, N * 16}, {(void **)&b.dens, N * 8},
         {(void **)&b.table, (size_t)table_total * 8}, {(void **)&b.ghist, 4 * 256 * 4},
         {(void **)&b.lookback, (size_t)(lookback_words ? lookback_words : 1) * 4},
-        {(void **)&b.tile_ctr, 4 * 4},
+        {(void **)&b.tile_ctr, 4 * 4}, {(void **)&b.hfull, N * 4}, {(void **)&b.stray, N * 4},
+        {(void **)&b.counters, 8 * 4},
+        {(void **)&b.brick_flag, (size_t)(brick_total ? brick_total : 1) * 4},
+        {(void **)&b.queue, (size_t)(brick_total ? brick_total : 1) * 4},
     };
     for (auto &r : reqs) {
         hipError_t e = hipMalloc(r.p, r.bytes);
@@ -142,7 +162,10 @@ int Chain::allocate(uint32_t n_, uint32_t table_total_) {
         }
     }
     hipError_t e = hipMemset(b.ghist, 0, 4 * 256 * 4);
+    if (e == hipSuccess) e = hipMemset(b.counters, 0, 8 * 4);
+    if (e == hipSuccess) e = hipMemset(b.brick_flag, 0, (size_t)(brick_total ? brick_total : 1) * 4);
     if (e == hipSuccess) e = hipDeviceSynchronize();
+    brick_prepare();
     if (e != hipSuccess) {
         release();
         return hip_fail(e, "hipMemset");
@@ -167,7 +190,8 @@ int Chain::set_debug(bool on) {
 void Chain::release() {
     void *ptrs[] = {b.pos, b.vel, b.keys, b.sort_k[0], b.sort_v[0], b.sort_k[1], b.sort_v[1],
                     b.pred_pos, b.pos_sorted, b.vel_sorted, b.dens, b.table, b.ghist,
-                    b.lookback, b.tile_ctr, b.dbg_pred_vel, b.dbg_vel_force};
+                    b.lookback, b.tile_ctr, b.dbg_pred_vel, b.dbg_vel_force, b.hfull,
+                    b.brick_flag, b.queue, b.stray, b.counters};
     for (void *p : ptrs)
         if (p) hipFree(p);
     b = ChainBuffers{};
@@ -191,11 +215,14 @@ hipError_t Chain::enqueue(hipStream_t s, const Scenes &sc, PassEvents *ev) {
         if (e != hipSuccess) return e;
     }
     if (ev) hipEventRecord(ev->ev[2], s);
-    if ((e = launch_reorder(s, sc, b, sorted_keys(), sorted_idx(), n)) != hipSuccess) return e;
+    if ((e = launch_reorder(s, sc, b, sorted_keys(), sorted_idx(), n, bricks)) != hipSuccess) return e;
+    if (bricks && (e = launch_queue(s, b, brick_total)) != hipSuccess) return e;
     if (ev) hipEventRecord(ev->ev[3], s);
-    if ((e = launch_density(s, sc, b, n)) != hipSuccess) return e;
+    e = bricks ? launch_density_brick(s, sc, b, n, brick_total) : launch_density(s, sc, b, n);
+    if (e != hipSuccess) return e;
     if (ev) hipEventRecord(ev->ev[4], s);
-    if ((e = launch_force(s, sc, b, n)) != hipSuccess) return e;
+    e = bricks ? launch_force_brick(s, sc, b, n, brick_total) : launch_force(s, sc, b, n);
+    if (e != hipSuccess) return e;
     if (ev) hipEventRecord(ev->ev[5], s);
     return hipSuccess;
 }
@@ -312,13 +339,13 @@ int sph_world_create(const sph_params *params, uint32_t n, sph_world **out) {
         delete w;
         return hip_fail(e, "stream creation");
     }
-    rc = w->chain.allocate(n, T);
+    w->scene.c = make_scene_const(w->params, T, 0, 0, n, 0);
+    rc = w->chain.allocate(n, T, (uint32_t)scene_bricks(w->scene.c));
     if (rc) {
         hipStreamDestroy(w->stream);
         delete w;
         return rc;
     }
-    w->scene.c = make_scene_const(w->params, T, 0, 0, n);
     *out = w;
     return SPH_OK;
 }
@@ -329,9 +356,12 @@ int sph_world_set_params(sph_world *w, const sph_params *params) {
     if (rc) return rc;
     const uint32_t T = params->table_cells ? params->table_cells : w->n;
     if (T != w->T) return fail(SPH_ERR_INVALID_ARG, "table_cells is fixed at creation");
+    SceneConst nc = make_scene_const(*params, T, 0, 0, w->n, 0);
+    if (scene_bricks(nc) != w->chain.brick_total)
+        return fail(SPH_ERR_INVALID_ARG, "the box and smoothing radius fix the brick grid at creation");
     w->params = *params;
     w->params.table_cells = T;
-    w->scene.c = make_scene_const(w->params, T, 0, 0, w->n);
+    w->scene.c = nc;
     w->graph.drop();
     return SPH_OK;
 }
@@ -389,6 +419,19 @@ int sph_world_download_state(sph_world *w, float *pos4, float *vel4, float *colo
     if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
     if (e != hipSuccess) return hip_fail(e, "download");
     if (color4) speed_ramp(w->n, vdst, color4);
+    return SPH_OK;
+}
+
+int sph_world_set_kernel(sph_world *w, int variant) {
+    if (!w) return fail(SPH_ERR_INVALID_ARG, "world is NULL");
+    if (variant < SPH_KERNEL_AUTO || variant > SPH_KERNEL_BRICK)
+        return fail(SPH_ERR_INVALID_ARG, "unknown kernel variant %d", variant);
+    if (variant == SPH_KERNEL_BRICK && w->chain.brick_total == 0)
+        return fail(SPH_ERR_INVALID_ARG, "brick kernels need a grid of at most 2^28 bricks");
+    hipError_t e = hipStreamSynchronize(w->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    w->chain.bricks = variant == SPH_KERNEL_DIRECT ? false : w->chain.brick_total > 0;
+    w->graph.drop();
     return SPH_OK;
 }
 

@@ -4,7 +4,8 @@ The binding is ctypes over the C-ABI only: no torch types, no CPU fallback.  Loa
 loudly when the HIP library is missing.
 """
 from .abi import (  # noqa: F401
-    SphError, SphParams, PassTimes, load_library, lib_path, DUMP_STAGES,
+    SphError, SphParams, PassTimes, load_library, lib_path, DUMP_STAGES, KERNELS,
     params_for, fill_block,
 )
 from .world import World  # noqa: F401
+from .batch import Batch  # noqa: F401

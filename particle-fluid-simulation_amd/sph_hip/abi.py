@@ -15,6 +15,8 @@ STATUS_NAMES = {
     -4: "SPH_ERR_STATE", -5: "SPH_ERR_CAPACITY",
 }
 
+KERNELS = {"auto": 0, "direct": 1, "brick": 2}
+
 DUMP_STAGES = {
     "cell_keys": 0, "sorted_keys": 1, "sorted_index": 2, "start": 3, "end": 4,
     "pred_pos": 5, "pred_vel": 6, "density": 7, "near_density": 8, "vel_force": 9,
@@ -87,11 +89,24 @@ def load_library() -> C.CDLL:
         "sph_world_sync": (C.c_int, [vp]),
         "sph_world_download_state": (C.c_int, [vp, vp, vp, vp]),
         "sph_world_set_debug": (C.c_int, [vp, C.c_int]),
+        "sph_world_set_kernel": (C.c_int, [vp, C.c_int]),
         "sph_world_debug_dump": (C.c_int, [vp, C.c_int, vp, C.c_size_t]),
         "sph_world_pass_times": (C.c_int, [vp, C.c_uint32, P(PassTimes)]),
         "sph_world_footprint": (C.c_uint64, [C.c_uint32, C.c_uint32]),
         "sph_world_destroy": (None, [vp]),
         "sph_last_error": (C.c_char_p, []),
+        "sph_batch_create": (C.c_int, [C.c_uint32, P(SphParams), P(C.c_uint32), P(vp)]),
+        "sph_batch_set_kernel": (C.c_int, [vp, C.c_int]),
+        "sph_batch_upload_scene": (C.c_int, [vp, C.c_uint32, vp, vp]),
+        "sph_batch_step": (C.c_int, [vp, C.c_uint32]),
+        "sph_batch_sync": (C.c_int, [vp]),
+        "sph_batch_download_scene": (C.c_int, [vp, C.c_uint32, vp, vp, vp]),
+        "sph_batch_set_debug": (C.c_int, [vp, C.c_int]),
+        "sph_batch_debug_dump": (C.c_int, [vp, C.c_uint32, C.c_int, vp, C.c_size_t]),
+        "sph_batch_scene_uses_bricks": (C.c_int, [vp, C.c_uint32]),
+        "sph_batch_pass_times": (C.c_int, [vp, C.c_uint32, P(PassTimes)]),
+        "sph_batch_particles": (C.c_uint32, [vp]),
+        "sph_batch_destroy": (None, [vp]),
         "sph_scene_fill_block": (C.c_int, [C.c_uint32, C.c_uint64, P(C.c_float * 3),
                                            P(C.c_float * 3), C.c_float, vp, vp]),
     }

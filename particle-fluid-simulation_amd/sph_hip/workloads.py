@@ -52,3 +52,42 @@ DAM_BREAK_16K = Workload("dam_break_16k", 16384, (10.0, 6.0, 6.0), (0.0, 0.0, 0.
                          (4.0, 6.0, 4.0), 0.0, 1)
 UNIFORM_1M = Workload("uniform_1m", 1 << 20, (40.0, 20.0, 20.0), (0.0, 0.0, 0.0),
                       (40.0, 20.0, 20.0), 0.1, 2)
+
+
+@dataclass(frozen=True)
+class SceneSpec:
+    n: int
+    box_max: tuple
+    target_density: float
+    seed: int
+
+    def params(self, base: Workload = DAM_BREAK_16K) -> SphParams:
+        return params_for(
+            particle_radius=base.particle_radius, smoothing_radius=base.smoothing_radius,
+            g=base.g, box_min=(0.0, 0.0, 0.0), box_max=self.box_max, frame_dt=base.frame_dt,
+            substeps=base.substeps, particle_mass=base.particle_mass,
+            pressure_multiplier=base.pressure_multiplier, target_density=self.target_density,
+            viscosity_strength=base.viscosity_strength,
+            near_pressure_multiplier=base.near_pressure_multiplier, table_cells=0)
+
+    def state(self):
+        """Uniform fill of the lower half of the box, zero velocity."""
+        bx, by, bz = self.box_max
+        return fill_block(self.n, self.seed, (0.0, 0.0, 0.0), (bx, by, bz * 0.5), 0.0)
+
+
+def mixed_scenes(count: int = 256, seed: int = 3, n_min: int = 2048, n_max: int = 65536,
+                 extent=(4.0, 16.0), rho0=(40.0, 70.0)) -> list:
+    """The mixed-size batch: scene s draws from numpy's PCG64 seeded with seed + s a
+    particle count log-uniform in [n_min, n_max], box extents uniform in `extent` per axis
+    and a target density uniform in `rho0`; its particles fill the lower half of its box.
+    Other parameters are those of dam_break_16k (R 0.35, k 500, g 9.8, 1/120 s x 3)."""
+    import numpy as np
+    out = []
+    for s in range(count):
+        rng = np.random.Generator(np.random.PCG64(seed + s))
+        n = int(round(float(np.exp(rng.uniform(np.log(n_min), np.log(n_max))))))
+        box = tuple(float(np.float32(x)) for x in rng.uniform(extent[0], extent[1], 3))
+        r0 = float(np.float32(rng.uniform(rho0[0], rho0[1])))
+        out.append(SceneSpec(n, box, r0, seed + s))
+    return out

@@ -5,7 +5,7 @@ import ctypes as C
 
 import numpy as np
 
-from .abi import DUMP_STAGES, PassTimes, SphParams, check, load_library
+from .abi import DUMP_STAGES, KERNELS, PassTimes, SphParams, check, load_library
 
 
 class World:
@@ -59,6 +59,10 @@ class World:
         check(self._lib.sph_world_download_state(
             self._h, pos.ctypes.data, vel.ctypes.data, col.ctypes.data if color else None))
         return (pos, vel, col) if color else (pos, vel)
+
+    def set_kernel(self, variant: str) -> None:
+        """Neighbour-pass kernels: "auto", "direct" (one lane per particle) or "brick"."""
+        check(self._lib.sph_world_set_kernel(self._h, KERNELS[variant]))
 
     def set_debug(self, on: bool) -> None:
         check(self._lib.sph_world_set_debug(self._h, 1 if on else 0))

@@ -14,9 +14,10 @@ STAGES_F32 = ("pred_pos", "pred_vel", "density", "near_density", "vel_force")
 REL_TOL = 1e-5  # single-step float tolerance of the path vs the FP64 oracle
 
 
-def _run_one_substep(sph_hip, params, pos, vel):
+def _run_one_substep(sph_hip, params, pos, vel, kernel="auto"):
     n = pos.shape[0]
     with sph_hip.World(params, n) as w:
+        w.set_kernel(kernel)
         w.set_debug(True)
         w.upload(pos, vel)
         p1 = sph_hip.SphParams.from_buffer_copy(params)
@@ -40,14 +41,20 @@ def _assert_bitexact(name, got, ref):
                            f"gpu {got.ravel()[bad[:5]]} ref {ref.ravel()[bad[:5]]}")
 
 
-def _check_substep(sph_hip, orc, params, pos, vel):
-    print(f"case n={pos.shape[0]} T={params.table_cells or pos.shape[0]}", flush=True)
-    got = _run_one_substep(sph_hip, params, pos, vel)
+def _check_substep(sph_hip, orc, params, pos, vel, kernels=("direct", "brick")):
+    for kernel in kernels:
+        got, ref = _check_substep_one(sph_hip, orc, params, pos, vel, kernel)
+    return got, ref
+
+
+def _check_substep_one(sph_hip, orc, params, pos, vel, kernel):
+    print(f"case n={pos.shape[0]} T={params.table_cells or pos.shape[0]} kernel={kernel}", flush=True)
+    got = _run_one_substep(sph_hip, params, pos, vel, kernel)
     p1 = sph_hip.SphParams.from_buffer_copy(params)
     p1.substeps = 1
     rpos, rvel, ref = orc.substep(p1, pos, vel)
     for s in STAGES_INT + STAGES_F32:
-        _assert_bitexact(s, got[s], ref[s])
+        _assert_bitexact(f"{kernel}:{s}", got[s], ref[s])
     _assert_bitexact("pos", got["pos"], rpos)
     _assert_bitexact("vel", got["vel"], rvel)
     return got, ref
@@ -82,18 +89,43 @@ def test_uniform_1m_one_substep_table_2m(gpu, orc):
     _check_substep(gpu, orc, params, pos, vel)
 
 
-def test_dam_break_multi_frame_bitexact(gpu, orc):
+@pytest.mark.parametrize("kernel", ["direct", "brick"])
+def test_dam_break_multi_frame_bitexact(gpu, orc, kernel):
     """10 frames x 3 substeps: the whole trajectory matches the oracle bit for bit."""
     from sph_hip.workloads import DAM_BREAK_16K as W
     params = W.params()
     pos, vel = W.state()
     with gpu.World(params, W.n) as w:
+        w.set_kernel(kernel)
         w.upload(pos, vel)
         w.step(10)
         gp, gv = w.download()
     rp, rv = orc.step(params, pos, vel, 10)
     _assert_bitexact("pos", gp, rp)
     _assert_bitexact("vel", gv, rv)
+
+
+def test_tiny_box_strays(gpu, orc):
+    """A box narrower than the integrate margin (min + r + 0.5 > max) pushes particles
+    outside the cell grid on the first substep; the second substep must still process
+    them (through the brick kernels' stray list) exactly like the oracle."""
+    import sph_hip
+    params = sph_hip.params_for(particle_radius=0.3, smoothing_radius=0.35, g=9.8,
+                                box_min=(0, 0, 0), box_max=(10, 0.5, 6), frame_dt=1 / 120,
+                                substeps=1, pressure_multiplier=500, target_density=55)
+    pos, vel = gpu.fill_block(3000, 5, (0, 0, 0), (10, 0.5, 6), 1.0)
+    rp1, _ = orc.step(params, pos, vel, 1)
+    # every particle left the cell grid (y clamped to 0.8 or to -0.3) after one substep
+    assert ((rp1[:, 1] > 0.5) | (rp1[:, 1] < 0.0)).all()
+    rp, rv = orc.step(params, pos, vel, 2)
+    for kernel in ("direct", "brick"):
+        with gpu.World(params, 3000) as w:
+            w.set_kernel(kernel)
+            w.upload(pos, vel)
+            w.step(2)
+            gp, gv = w.download()
+        _assert_bitexact(f"{kernel}:pos", gp, rp)
+        _assert_bitexact(f"{kernel}:vel", gv, rv)
 
 
 @pytest.mark.parametrize("n,T", [(1, 1), (2, 0), (3000, 0), (5000, 1 << 16), (4096, 1 << 24),
