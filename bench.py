@@ -69,6 +69,60 @@ def cpu_baseline(workload, table_cells: int, budget_s: float = 12.0) -> dict:
     }
 
 
+def mixed_batch(steps: int, warmup: int, count: int = 256) -> dict:
+    """Batched vs plain on the mixed workload: `count` scenes (log-uniform 2,048-65,536
+    particles, boxes U[4,16]^3, target density U[40,70], seeds 3+s) stepped together in one
+    batch, then the same scenes as one world each, stepped one after another."""
+    import torch
+    import sph_hip
+    from sph_hip.workloads import mixed_scenes
+
+    specs = mixed_scenes(count=count, seed=3)
+    params = [sp.params() for sp in specs]
+    counts = [sp.n for sp in specs]
+    states = [sp.state() for sp in specs]
+    total = sum(counts)
+
+    b = sph_hip.Batch(params, counts)
+    for i, (p, v) in enumerate(states):
+        b.upload(i, p, v)
+    b.step(warmup)
+    b.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.sync()
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    bricks = sum(b.uses_bricks(i) for i in range(count))
+    b.close()
+
+    worlds = []
+    for i, (p, v) in enumerate(states):
+        w = sph_hip.World(params[i], counts[i])
+        w.upload(p, v)
+        w.step(warmup)
+        worlds.append(w)
+    for w in worlds:
+        w.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for w in worlds:
+        w.step(steps)
+    for w in worlds:
+        w.sync()
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    for w in worlds:
+        w.close()
+    batched = total * steps / t_batch
+    plain = total * steps / t_plain
+    return {"scenes": count, "particles": total, "steps": steps,
+            "batched_particle_steps_per_s": batched, "plain_particle_steps_per_s": plain,
+            "batched_over_plain": batched / plain, "batched_ms_per_step": t_batch * 1e3 / steps,
+            "plain_ms_per_step": t_plain * 1e3 / steps, "scenes_on_brick_kernels": bricks}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +132,8 @@ def main() -> int:
     ap.add_argument("--table-cells", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--batch-steps", type=int, default=100,
+                    help="steps of the mixed 256-scene batched-vs-plain comparison (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (scenes shard "
@@ -97,6 +153,8 @@ def main() -> int:
     world = sph_hip.World(params, W.n)
     world.upload(pos, vel)
     print(f"[bench] {W.name}: N={W.n} T={T} substeps/step={W.substeps}", file=sys.stderr, flush=True)
+    chosen, tuned_ms = world.autotune(5)
+    print(f"[bench] autotune: {chosen} kernels, {tuned_ms:.3f} ms/substep", file=sys.stderr, flush=True)
 
     world.step(args.warmup)
     world.sync()
@@ -112,6 +170,10 @@ def main() -> int:
 
     # Dominant kernel, timed with HIP events on the library's stream.
     pt = world.pass_times(30)
+    variants = {}
+    for v in ("direct", "brick"):
+        world.set_kernel(v)
+        variants[v] = world.pass_times(10)
     force_s = pt["force_ms"] * 1e-3
     force_gbs = W.n * FORCE_BYTES_PER_PARTICLE / force_s / 1e9
     b_alg = alg_bytes_per_particle_substep(W.n, T)
@@ -123,6 +185,12 @@ def main() -> int:
     import numpy as np
     ok = bool(np.isfinite(p_out).all() and np.isfinite(v_out).all())
     world.close()
+
+    mixed = None
+    if args.batch_steps > 0:
+        print("[bench] mixed 256-scene batch vs plain", file=sys.stderr, flush=True)
+        mixed = mixed_batch(args.batch_steps, 3)
+        print(f"[bench] batched/plain = {mixed['batched_over_plain']:.2f}", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -143,7 +211,10 @@ def main() -> int:
                      "alg_bytes_per_launch": W.n * FORCE_BYTES_PER_PARTICLE},
         "chain_roofline": {"alg_bytes_per_particle_substep": b_alg, "achieved_gbs": chain_gbs,
                            "frac": chain_gbs / HBM_PEAK_GBS},
+        "kernels": chosen,
         "pass_ms": pt,
+        "pass_ms_by_kernel": variants,
+        "mixed_batch": mixed,
         "state_finite": ok,
         "cpu_baseline": cpu,
     }

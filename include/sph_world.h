@@ -111,13 +111,22 @@ typedef enum sph_dump_stage {
  *   DIRECT: one lane per particle, runs read from global memory (the reference's scheme).
  *   BRICK:  workgroups own 4x4x4-cell bricks drained from a device work queue and read the
  *           6x6x6 halo's runs from LDS.
- *   AUTO:   BRICK when the box's brick grid is small enough (<= 2^28 bricks). */
+ *   AUTO:   the autotuner's choice (the default): before the first step of a size bucket
+ *           (particle count, table size, brick count rounded to powers of two) each
+ *           variant is timed on a saved copy of the world's state, and the fastest is
+ *           cached for the process.  SPH_AUTOTUNE=0 in the environment disables the
+ *           measurement (DIRECT is used). */
 typedef enum sph_kernel_variant {
     SPH_KERNEL_AUTO = 0,
     SPH_KERNEL_DIRECT = 1,
     SPH_KERNEL_BRICK = 2
 } sph_kernel_variant;
 int sph_world_set_kernel(sph_world *w, int variant);
+/* Runs the measurement now (`substeps` timed substeps per variant, state restored) and
+ * caches the result; *chosen receives the variant, *ms_per_substep (may be NULL) its time. */
+int sph_world_autotune(sph_world *w, uint32_t substeps, int *chosen, double *ms_per_substep);
+/* The variant the next step will use (DIRECT or BRICK), or < 0 on error. */
+int sph_world_kernel(const sph_world *w);
 
 int sph_world_set_debug(sph_world *w, int enable);
 int sph_world_debug_dump(sph_world *w, int stage, void *dst, size_t bytes);

@@ -31,13 +31,12 @@ namespace sph {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kH1 = kBrick + 2;
-constexpr int kHalo = kH1 * kH1 * kH1;
 constexpr int kCells = kBrick * kBrick * kBrick;
-constexpr int kCapD = 2048;    // staged particles, density pass: 32 KiB of float4
-constexpr int kCapF = 1280;    // staged particles, force pass: 40 B each, 50 KiB
-constexpr int kOwnCap = 1024;  // particles evaluated per round of a brick
+constexpr int kCapD = 1536;    // staged particles, density pass: 16 B each, 24 KiB
+constexpr int kCapF = 1024;    // staged particles, force pass: 32 B each, 32 KiB
+constexpr int kOwnCap = 512;   // particles evaluated per round of a brick
 static_assert(kHalo <= kBlock && kCells <= kBlock, "one thread per halo cell");
+static_assert(kHalo < 256, "halo cell ids are stored in 8 bits");
 
 // Exclusive prefix sum over the 256 threads; also returns the block total.
 __device__ __forceinline__ uint32_t scan256(uint32_t v, uint32_t *s_tmp, uint32_t &total) {
@@ -62,10 +61,10 @@ __device__ __forceinline__ uint32_t scan256(uint32_t v, uint32_t *s_tmp, uint32_
     return base + x - v;
 }
 
-struct Halo {
-    int off[kHalo];  // LDS offset of the staged run; -1 = run read from global memory
-    int cnt[kHalo];  // run length (0 = empty cell)
-    int gst[kHalo];  // first slot of the run
+// Owned particles of the current round: slot and LDS index (0xFFFF = not staged).
+struct Owned {
+    uint32_t slot[kOwnCap];
+    uint16_t lds[kOwnCap];
 };
 
 struct BrickCtx {
@@ -78,75 +77,79 @@ __device__ __forceinline__ BrickCtx brick_origin(const SceneConst &c, uint32_t b
     return BrickCtx{(int)bx * kBrick - 1, (int)by * kBrick - 1, (int)bz * kBrick - 1};
 }
 
-// Looks up the halo runs and assigns LDS space in halo order until `cap` is used up.
+__device__ __forceinline__ bool interior(int h) {
+    const int hx = h % kH1, hy = (h / kH1) % kH1, hz = h / (kH1 * kH1);
+    return hx >= 1 && hx <= kBrick && hy >= 1 && hy <= kBrick && hz >= 1 && hz <= kBrick;
+}
+
+// Looks up the halo runs (one thread per halo cell) and assigns LDS space in halo order
+// while it lasts; writes the owning halo cell of every staged entry into s_cellof.
 __device__ __forceinline__ void halo_lookup(const SceneConst &c, const int2 *__restrict__ table,
-                                            const BrickCtx &bc, int cap, Halo &H, uint32_t *s_tmp) {
+                                            const BrickCtx &bc, int cap, Halo &H,
+                                            uint8_t *s_cellof, uint32_t *s_tmp) {
     const int tid = threadIdx.x;
     uint32_t cnt = 0;
     int gst = -1;
     if (tid < kHalo) {
         const int hx = tid % kH1, hy = (tid / kH1) % kH1, hz = tid / (kH1 * kH1);
-        const int2 se = table[key_of_cell(c, bc.ox + hx, bc.oy + hy, bc.oz + hz)];
+        const uint32_t x = (uint32_t)(bc.ox + hx), y = (uint32_t)(bc.oy + hy), z = (uint32_t)(bc.oz + hz);
+        const uint32_t h = cell_hash(x, y, z);
+        const int2 se = table[c.table_base + (h & c.mask)];
         if (se.x != -1) {
             cnt = (uint32_t)(se.y - se.x);
             gst = se.x;
         }
+        H.hash[tid] = h;
     }
     uint32_t total;
     const uint32_t off = scan256(cnt, s_tmp, total);
+    if (tid == 0) H.staged = 0;
+    __syncthreads();
     if (tid < kHalo) {
-        H.cnt[tid] = (int)cnt;
+        const bool fits = off + cnt <= (uint32_t)cap;
+        H.oc[tid] = make_int2(fits ? (int)off : -1, (int)cnt);
         H.gst[tid] = gst;
-        H.off[tid] = off + cnt <= (uint32_t)cap ? (int)off : -1;
+        if (fits) {
+            for (uint32_t k = 0; k < cnt; ++k) s_cellof[off + k] = (uint8_t)tid;
+            if (cnt) atomicMax(&H.staged, off + cnt);
+        }
     }
     __syncthreads();
 }
 
-// Number of slots of brick cell `t` (thread t < kCells) whose current cell is that cell.
-struct OwnedCell {
-    int gst, cnt;
-    uint32_t hash;
-};
-
-__device__ __forceinline__ OwnedCell owned_cell(const BrickCtx &bc, const Halo &H) {
-    const int t = threadIdx.x;
-    const int lx = t % kBrick + 1, ly = (t / kBrick) % kBrick + 1, lz = t / (kBrick * kBrick) + 1;
-    const int h = lx + kH1 * (ly + kH1 * lz);
-    OwnedCell oc;
-    oc.gst = H.gst[h];
-    oc.cnt = H.cnt[h];
-    oc.hash = cell_hash((uint32_t)(bc.ox + lx), (uint32_t)(bc.oy + ly), (uint32_t)(bc.oz + lz));
-    return oc;
-}
-
-// Runs `eval(slot)` for every particle owned by the brick, kOwnCap at a time.
-template <class Eval>
-__device__ __forceinline__ void for_owned(const BrickCtx &bc, const Halo &H,
-                                          const uint32_t *__restrict__ hfull, uint32_t *s_own,
-                                          uint32_t *s_tmp, Eval eval) {
+// Builds the owned list of entries [chunk, chunk + kOwnCap) of the brick's particles in
+// interior-cell order.  Ownership of staged entries is read from s_own_flag, of unstaged
+// interior runs from the slots' full hashes.  Returns the total owned count.
+__device__ __forceinline__ uint32_t owned_round(const Halo &H, const uint8_t *s_own_flag,
+                                                const uint32_t *__restrict__ hfull, uint32_t chunk,
+                                                Owned &O, uint32_t *s_tmp) {
     const int tid = threadIdx.x;
-    OwnedCell oc{0, 0, 0u};
+    int h = 0, off = -1, cnt = 0, gst = 0;
     uint32_t own = 0;
     if (tid < kCells) {
-        oc = owned_cell(bc, H);
-        for (int k = 0; k < oc.cnt; ++k) own += hfull[oc.gst + k] == oc.hash ? 1u : 0u;
+        const int lx = tid % kBrick + 1, ly = (tid / kBrick) % kBrick + 1, lz = tid / (kBrick * kBrick) + 1;
+        h = lx + kH1 * (ly + kH1 * lz);
+        off = H.oc[h].x;
+        cnt = H.oc[h].y;
+        gst = H.gst[h];
+        for (int k = 0; k < cnt; ++k)
+            own += (off >= 0 ? s_own_flag[off + k] != 0 : hfull[gst + k] == H.hash[h]) ? 1u : 0u;
     }
     uint32_t total;
-    const uint32_t base = scan256(own, s_tmp, total);
-    for (uint32_t chunk = 0; chunk < total; chunk += kOwnCap) {
-        if (tid < kCells && own) {
-            uint32_t idx = base;
-            for (int k = 0; k < oc.cnt; ++k) {
-                if (hfull[oc.gst + k] != oc.hash) continue;
-                if (idx >= chunk && idx < chunk + kOwnCap) s_own[idx - chunk] = (uint32_t)(oc.gst + k);
-                ++idx;
+    uint32_t idx = scan256(own, s_tmp, total);
+    if (own && idx < chunk + kOwnCap && idx + own > chunk) {
+        for (int k = 0; k < cnt; ++k) {
+            const bool mine = off >= 0 ? s_own_flag[off + k] != 0 : hfull[gst + k] == H.hash[h];
+            if (!mine) continue;
+            if (idx >= chunk && idx < chunk + kOwnCap) {
+                O.slot[idx - chunk] = (uint32_t)(gst + k);
+                O.lds[idx - chunk] = off >= 0 ? (uint16_t)(off + k) : (uint16_t)0xFFFFu;
             }
+            ++idx;
         }
-        __syncthreads();
-        const uint32_t m = min((uint32_t)kOwnCap, total - chunk);
-        for (uint32_t o = tid; o < m; o += kBlock) eval(s_own[o]);
-        __syncthreads();
     }
+    __syncthreads();
+    return total;
 }
 
 // Pulls the next work item of a persistent workgroup.
@@ -177,7 +180,6 @@ __global__ __launch_bounds__(kBlock) void k_queue(uint32_t *__restrict__ flag, u
     if (blockIdx.x == 0 && threadIdx.x < 2) counters[2 + threadIdx.x] = 0u;
 }
 
-// ---------------------------------------------------------------------------------------
 template <class Scenes>
 __global__ __launch_bounds__(kBlock) void k_density_brick(
     Scenes sc, const uint32_t *__restrict__ queue, uint32_t *__restrict__ counters,
@@ -185,8 +187,10 @@ __global__ __launch_bounds__(kBlock) void k_density_brick(
     const uint32_t *__restrict__ hfull, const uint32_t *__restrict__ stray,
     float2 *__restrict__ dens) {
     __shared__ Halo H;
+    __shared__ Owned O;
     __shared__ float4 s_pos[kCapD];
-    __shared__ uint32_t s_own[kOwnCap];
+    __shared__ uint8_t s_cellof[kCapD];
+    __shared__ uint8_t s_flag[kCapD];
     __shared__ uint32_t s_tmp[kBlock / 64];
     __shared__ uint32_t s_item;
     const uint32_t qn = counters[0], ns = counters[1];
@@ -200,86 +204,69 @@ __global__ __launch_bounds__(kBlock) void k_density_brick(
                 const uint32_t j = stray[idx];
                 const SceneConst c = sc.scene(j);
                 const float4 s = pred_pos[j];
-                const AxisCells ax = sample_axis(s.x, c.minx, c.maxx, c.R);
-                const AxisCells ay = sample_axis(s.y, c.miny, c.maxy, c.R);
-                const AxisCells az = sample_axis(s.z, c.minz, c.maxz, c.R);
-                float M = 0.0f, Mn = 0.0f;
-                for (int dz = 0; dz < 3; ++dz)
-                    for (int dy = 0; dy < 3; ++dy)
-                        for (int dx = 0; dx < 3; ++dx) {
-                            if (!(ax.in(dx) && ay.in(dy) && az.in(dz))) continue;
-                            const int2 se = table[key_of_cell(c, ax.c(dx), ay.c(dy), az.c(dz))];
-                            if (se.x == -1) continue;
-                            density_run(c, pred_pos + se.x, se.y - se.x, s, M, Mn);
-                        }
-                dens[j] = density_finish(c, M, Mn);
+                const DensityCells<false> cells{CellRuns<false>{c.table_base, c.mask, table, nullptr, 0, 0, 0},
+                                                sample_axis(s.x, c.minx, c.maxx, c.R),
+                                                sample_axis(s.y, c.miny, c.maxy, c.R),
+                                                sample_axis(s.z, c.minz, c.maxz, c.R)};
+                dens[j] = density_walk<false>(c, s, pred_pos, nullptr, cells);
             }
             continue;
         }
-        const SceneConst c = sc.scene_of_brick(queue[item]);
-        const BrickCtx bc = brick_origin(c, queue[item]);
-        halo_lookup(c, table, bc, kCapD, H, s_tmp);
-        if (threadIdx.x < kHalo && H.off[threadIdx.x] >= 0) {
-            const int o = H.off[threadIdx.x], g = H.gst[threadIdx.x], n = H.cnt[threadIdx.x];
-            for (int k = 0; k < n; ++k) s_pos[o + k] = pred_pos[g + k];
+        const uint32_t brick = queue[item];
+        const SceneConst c = sc.scene_of_brick(brick);
+        const BrickCtx bc = brick_origin(c, brick);
+        halo_lookup(c, table, bc, kCapD, H, s_cellof, s_tmp);
+        // Stage all runs with one load per entry; mark the entries owned by the brick.
+        for (uint32_t i = threadIdx.x; i < H.staged; i += kBlock) {
+            const int h = s_cellof[i];
+            const uint32_t slot = (uint32_t)(H.gst[h] + ((int)i - H.oc[h].x));
+            s_pos[i] = pred_pos[slot];
+            s_flag[i] = interior(h) && hfull[slot] == H.hash[h] ? 1 : 0;
         }
         __syncthreads();
-        for_owned(bc, H, hfull, s_own, s_tmp, [&](uint32_t j) {
-            const float4 s = pred_pos[j];
-            const AxisCells ax = sample_axis(s.x, c.minx, c.maxx, c.R);
-            const AxisCells ay = sample_axis(s.y, c.miny, c.maxy, c.R);
-            const AxisCells az = sample_axis(s.z, c.minz, c.maxz, c.R);
-            float M = 0.0f, Mn = 0.0f;
-            for (int dz = 0; dz < 3; ++dz)
-                for (int dy = 0; dy < 3; ++dy)
-                    for (int dx = 0; dx < 3; ++dx) {
-                        if (!(ax.in(dx) && ay.in(dy) && az.in(dz))) continue;
-                        const int lx = ax.c(dx) - bc.ox, ly = ay.c(dy) - bc.oy, lz = az.c(dz) - bc.oz;
-                        if ((unsigned)lx < (unsigned)kH1 && (unsigned)ly < (unsigned)kH1 &&
-                            (unsigned)lz < (unsigned)kH1) {
-                            const int h = lx + kH1 * (ly + kH1 * lz);
-                            const int n = H.cnt[h];
-                            if (n == 0) continue;
-                            const int o = H.off[h];
-                            if (o >= 0) density_run(c, s_pos + o, n, s, M, Mn);
-                            else density_run(c, pred_pos + H.gst[h], n, s, M, Mn);
-                        } else {
-                            const int2 se = table[key_of_cell(c, ax.c(dx), ay.c(dy), az.c(dz))];
-                            if (se.x == -1) continue;
-                            density_run(c, pred_pos + se.x, se.y - se.x, s, M, Mn);
-                        }
-                    }
-            dens[j] = density_finish(c, M, Mn);
-        });
+        for (uint32_t chunk = 0;; chunk += kOwnCap) {
+            const uint32_t total = owned_round(H, s_flag, hfull, chunk, O, s_tmp);
+            if (chunk >= total) break;
+            const uint32_t m = min((uint32_t)kOwnCap, total - chunk);
+            for (uint32_t o = threadIdx.x; o < m; o += kBlock) {
+                const uint32_t j = O.slot[o];
+                const uint16_t l = O.lds[o];
+                float4 s;
+                if (l != 0xFFFFu) s = s_pos[l];
+                else s = pred_pos[j];
+                const DensityCells<true> cells{CellRuns<true>{c.table_base, c.mask, table, &H, bc.ox, bc.oy, bc.oz},
+                                               sample_axis(s.x, c.minx, c.maxx, c.R),
+                                               sample_axis(s.y, c.miny, c.maxy, c.R),
+                                               sample_axis(s.z, c.minz, c.maxz, c.R)};
+                dens[j] = density_walk<true>(c, s, pred_pos, s_pos, cells);
+            }
+            __syncthreads();
+        }
     }
 }
 
 // ---------------------------------------------------------------------------------------
-template <class Scenes>
-__device__ __forceinline__ void force_one_global(const SceneConst &c, uint32_t j,
-                                                 const float4 *__restrict__ pred_pos,
-                                                 const float4 *__restrict__ pos_sorted,
-                                                 const float4 *__restrict__ vel_sorted,
-                                                 const float2 *__restrict__ dens,
-                                                 const int2 *__restrict__ table,
-                                                 float4 *__restrict__ pos_out,
-                                                 float4 *__restrict__ vel_out,
-                                                 float4 *__restrict__ dbg_vel_force) {
+// Force + integrate of slot j (applyPressure...comp:328-337, updateVelocities.comp:71-119).
+template <bool kHaloLds>
+__device__ __forceinline__ void force_of(const SceneConst &c, uint32_t j,
+                                         const float4 *__restrict__ pred_pos,
+                                         const float4 *__restrict__ pos_sorted,
+                                         const float4 *__restrict__ vel_sorted,
+                                         const float2 *__restrict__ dens,
+                                         const int2 *__restrict__ table, const Halo *H,
+                                         const BrickCtx &bc, const float4 *s_a, const float4 *s_b,
+                                         uint32_t *q, float4 *__restrict__ pos_out,
+                                         float4 *__restrict__ vel_out,
+                                         float4 *__restrict__ dbg_vel_force) {
     const float4 s = pred_pos[j];
     const float4 P = pos_sorted[j];
     const float4 V = vel_sorted[j];
     const float2 myd = dens[j];
     const ForceSelf me = force_self(c, j, s, P, V, myd);
-    const int3 c0 = force_cell(c, s);
+    const ForceCells<kHaloLds> cells{CellRuns<kHaloLds>{c.table_base, c.mask, table, H, bc.ox, bc.oy, bc.oz},
+                                     force_cell(c, s)};
     float3 Fp = make_float3(0.0f, 0.0f, 0.0f), Fv = make_float3(0.0f, 0.0f, 0.0f);
-    for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int2 se = table[key_of_cell(c, c0.x + dx, c0.y + dy, c0.z + dz)];
-                if (se.x == -1) continue;
-                force_run(c, pred_pos + se.x, vel_sorted + se.x, dens + se.x, se.y - se.x,
-                          (uint32_t)se.x, me, Fp, Fv);
-            }
+    force_walk<kHaloLds>(c, me, pred_pos, vel_sorted, dens, s_a, s_b, cells, q, kBlock, Fp, Fv);
     float4 po, vo, vf;
     force_finish(c, P, V, myd.x, Fp, Fv, po, vo, vf);
     if (dbg_vel_force) dbg_vel_force[j] = vf;
@@ -296,10 +283,12 @@ __global__ __launch_bounds__(kBlock) void k_force_brick(
     const uint32_t *__restrict__ stray, float4 *__restrict__ pos_out,
     float4 *__restrict__ vel_out, float4 *__restrict__ dbg_vel_force) {
     __shared__ Halo H;
-    __shared__ float4 s_pos[kCapF];
-    __shared__ float4 s_vel[kCapF];
-    __shared__ float2 s_dd[kCapF];
-    __shared__ uint32_t s_own[kOwnCap];
+    __shared__ Owned O;
+    __shared__ float4 s_a[kCapF];
+    __shared__ float4 s_b[kCapF];
+    __shared__ uint8_t s_cellof[kCapF];
+    __shared__ uint8_t s_flag[kCapF];
+    __shared__ uint32_t s_queue[kQueue * kBlock];
     __shared__ uint32_t s_tmp[kBlock / 64];
     __shared__ uint32_t s_item;
     const uint32_t qn = counters[0], ns = counters[1];
@@ -311,58 +300,36 @@ __global__ __launch_bounds__(kBlock) void k_force_brick(
             const uint32_t idx = (item - qn) * kBlock + threadIdx.x;
             if (idx < ns) {
                 const uint32_t j = stray[idx];
-                force_one_global<Scenes>(sc.scene(j), j, pred_pos, pos_sorted, vel_sorted, dens,
-                                         table, pos_out, vel_out, dbg_vel_force);
+                force_of<false>(sc.scene(j), j, pred_pos, pos_sorted, vel_sorted, dens, table,
+                                nullptr, BrickCtx{0, 0, 0}, nullptr, nullptr, s_queue + threadIdx.x,
+                                pos_out, vel_out, dbg_vel_force);
             }
             continue;
         }
-        const SceneConst c = sc.scene_of_brick(queue[item]);
-        const BrickCtx bc = brick_origin(c, queue[item]);
-        halo_lookup(c, table, bc, kCapF, H, s_tmp);
-        if (threadIdx.x < kHalo && H.off[threadIdx.x] >= 0) {
-            const int o = H.off[threadIdx.x], g = H.gst[threadIdx.x], n = H.cnt[threadIdx.x];
-            for (int k = 0; k < n; ++k) {
-                s_pos[o + k] = pred_pos[g + k];
-                s_vel[o + k] = vel_sorted[g + k];
-                s_dd[o + k] = dens[g + k];
-            }
+        const uint32_t brick = queue[item];
+        const SceneConst c = sc.scene_of_brick(brick);
+        const BrickCtx bc = brick_origin(c, brick);
+        halo_lookup(c, table, bc, kCapF, H, s_cellof, s_tmp);
+        for (uint32_t i = threadIdx.x; i < H.staged; i += kBlock) {
+            const int h = s_cellof[i];
+            const uint32_t slot = (uint32_t)(H.gst[h] + ((int)i - H.oc[h].x));
+            const float4 p = pred_pos[slot];
+            const float4 v = vel_sorted[slot];
+            const float2 d = dens[slot];
+            s_a[i] = make_float4(p.x, p.y, p.z, d.y);
+            s_b[i] = make_float4(v.x, v.y, v.z, d.x);
+            s_flag[i] = interior(h) && hfull[slot] == H.hash[h] ? 1 : 0;
         }
         __syncthreads();
-        for_owned(bc, H, hfull, s_own, s_tmp, [&](uint32_t j) {
-            const float4 s = pred_pos[j];
-            const float4 P = pos_sorted[j];
-            const float4 V = vel_sorted[j];
-            const float2 myd = dens[j];
-            const ForceSelf me = force_self(c, j, s, P, V, myd);
-            const int3 c0 = force_cell(c, s);
-            float3 Fp = make_float3(0.0f, 0.0f, 0.0f), Fv = make_float3(0.0f, 0.0f, 0.0f);
-            for (int dz = -1; dz <= 1; ++dz)
-                for (int dy = -1; dy <= 1; ++dy)
-                    for (int dx = -1; dx <= 1; ++dx) {
-                        const int cx = c0.x + dx, cy = c0.y + dy, cz = c0.z + dz;
-                        const int lx = cx - bc.ox, ly = cy - bc.oy, lz = cz - bc.oz;
-                        if ((unsigned)lx < (unsigned)kH1 && (unsigned)ly < (unsigned)kH1 &&
-                            (unsigned)lz < (unsigned)kH1) {
-                            const int h = lx + kH1 * (ly + kH1 * lz);
-                            const int n = H.cnt[h];
-                            if (n == 0) continue;
-                            const int o = H.off[h];
-                            const uint32_t g = (uint32_t)H.gst[h];
-                            if (o >= 0) force_run(c, s_pos + o, s_vel + o, s_dd + o, n, g, me, Fp, Fv);
-                            else force_run(c, pred_pos + g, vel_sorted + g, dens + g, n, g, me, Fp, Fv);
-                        } else {
-                            const int2 se = table[key_of_cell(c, cx, cy, cz)];
-                            if (se.x == -1) continue;
-                            force_run(c, pred_pos + se.x, vel_sorted + se.x, dens + se.x,
-                                      se.y - se.x, (uint32_t)se.x, me, Fp, Fv);
-                        }
-                    }
-            float4 po, vo, vf;
-            force_finish(c, P, V, myd.x, Fp, Fv, po, vo, vf);
-            if (dbg_vel_force) dbg_vel_force[j] = vf;
-            pos_out[j] = po;
-            vel_out[j] = vo;
-        });
+        for (uint32_t chunk = 0;; chunk += kOwnCap) {
+            const uint32_t total = owned_round(H, s_flag, hfull, chunk, O, s_tmp);
+            if (chunk >= total) break;
+            const uint32_t m = min((uint32_t)kOwnCap, total - chunk);
+            for (uint32_t o = threadIdx.x; o < m; o += kBlock)
+                force_of<true>(c, O.slot[o], pred_pos, pos_sorted, vel_sorted, dens, table, &H, bc,
+                               s_a, s_b, s_queue + threadIdx.x, pos_out, vel_out, dbg_vel_force);
+            __syncthreads();
+        }
     }
 }
 
@@ -387,7 +354,9 @@ template <class Scenes> uint32_t BrickGrids<Scenes>::density = 0;
 template <class Scenes> uint32_t BrickGrids<Scenes>::force = 0;
 
 inline uint32_t clamp_grid(uint32_t resident, uint64_t max_items) {
-    uint64_t g = resident ? resident : 256u;
+    // At least 8 workgroups per CU are launched; the ones that cannot be resident start
+    // as others retire and find the queue drained or not, which costs nothing either way.
+    uint64_t g = resident < 2048u ? 2048u : resident;
     if (g > max_items) g = max_items;
     return g ? (uint32_t)g : 1u;
 }

@@ -117,22 +117,26 @@ constexpr int kBrick = 4;
 // coordinates and in-box flags are computed once; the 27 cells are then visited in the
 // reference's (dz, dy, dx) order.  `visit(cx, cy, cz)` is called for each non-skipped cell.
 struct AxisCells {
-    int32_t c0, c1, c2;  // cell of s - R, s, s + R
-    bool in0, in1, in2;  // that sample coordinate lies inside [lo, hi]
-    // Selected with compares rather than array indexing so nothing spills to scratch.
-    __device__ __forceinline__ int32_t c(int d) const { return d == 0 ? c0 : (d == 1 ? c1 : c2); }
-    __device__ __forceinline__ bool in(int d) const { return d == 0 ? in0 : (d == 1 ? in1 : in2); }
+    uint32_t mid;     // cell of s
+    uint32_t lo_off;  // cell(s - R) - mid (mod 2^32)
+    uint32_t hi_off;  // cell(s + R) - mid (mod 2^32)
+    uint32_t inmask;  // bit d set: sample coordinate d lies inside [lo, hi]
+    // Arithmetic selection (no per-field selects) keeps the struct in registers when d is
+    // not a compile-time constant.
+    __device__ __forceinline__ int32_t c(int d) const {
+        return (int32_t)(mid + (uint32_t)(d == 0) * lo_off + (uint32_t)(d == 2) * hi_off);
+    }
+    __device__ __forceinline__ bool in(int d) const { return (inmask >> d) & 1u; }
 };
 
 __device__ __forceinline__ AxisCells sample_axis(float s, float lo, float hi, float R) {
     AxisCells a;
     const float q0 = s + R * -1.0f, q1 = s + R * 0.0f, q2 = s + R * 1.0f;
-    a.in0 = !(q0 < lo) && !(q0 > hi);
-    a.in1 = !(q1 < lo) && !(q1 > hi);
-    a.in2 = !(q2 < lo) && !(q2 > hi);
-    a.c0 = cell_of(q0, lo, R);
-    a.c1 = cell_of(q1, lo, R);
-    a.c2 = cell_of(q2, lo, R);
+    a.inmask = ((!(q0 < lo) && !(q0 > hi)) ? 1u : 0u) | ((!(q1 < lo) && !(q1 > hi)) ? 2u : 0u) |
+               ((!(q2 < lo) && !(q2 > hi)) ? 4u : 0u);
+    a.mid = (uint32_t)cell_of(q1, lo, R);
+    a.lo_off = (uint32_t)cell_of(q0, lo, R) - a.mid;
+    a.hi_off = (uint32_t)cell_of(q2, lo, R) - a.mid;
     return a;
 }
 

@@ -138,19 +138,11 @@ __global__ __launch_bounds__(kBlock) void k_density(
     if (j >= n) return;
     const SceneConst c = sc.scene(j);
     const float4 s = pred_pos[j];
-    const AxisCells ax = sample_axis(s.x, c.minx, c.maxx, c.R);
-    const AxisCells ay = sample_axis(s.y, c.miny, c.maxy, c.R);
-    const AxisCells az = sample_axis(s.z, c.minz, c.maxz, c.R);
-    float M = 0.0f, Mn = 0.0f;
-    for (int dz = 0; dz < 3; ++dz)
-        for (int dy = 0; dy < 3; ++dy)
-            for (int dx = 0; dx < 3; ++dx) {
-                if (!(ax.in(dx) && ay.in(dy) && az.in(dz))) continue;
-                const int2 se = table[key_of_cell(c, ax.c(dx), ay.c(dy), az.c(dz))];
-                if (se.x == -1) continue;
-                density_run(c, pred_pos + se.x, se.y - se.x, s, M, Mn);
-            }
-    dens[j] = density_finish(c, M, Mn);
+    const DensityCells<false> cells{CellRuns<false>{c.table_base, c.mask, table, nullptr, 0, 0, 0},
+                                    sample_axis(s.x, c.minx, c.maxx, c.R),
+                                    sample_axis(s.y, c.miny, c.maxy, c.R),
+                                    sample_axis(s.z, c.minz, c.maxz, c.R)};
+    dens[j] = density_walk<false>(c, s, pred_pos, nullptr, cells);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -163,6 +155,7 @@ __global__ __launch_bounds__(kBlock) void k_force(
     const float4 *__restrict__ vel_sorted, const float2 *__restrict__ dens,
     const int2 *__restrict__ table, float4 *__restrict__ pos_out, float4 *__restrict__ vel_out,
     float4 *__restrict__ dbg_vel_force, uint32_t n) {
+    __shared__ uint32_t s_queue[kQueue * kBlock];
     const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
     if (j >= n) return;
     const SceneConst c = sc.scene(j);
@@ -171,16 +164,10 @@ __global__ __launch_bounds__(kBlock) void k_force(
     const float4 V = vel_sorted[j];
     const float2 myd = dens[j];
     const ForceSelf me = force_self(c, j, s, P, V, myd);
-    const int3 c0 = force_cell(c, s);
+    const ForceCells<false> cells{CellRuns<false>{c.table_base, c.mask, table, nullptr, 0, 0, 0}, force_cell(c, s)};
     float3 Fp = make_float3(0.0f, 0.0f, 0.0f), Fv = make_float3(0.0f, 0.0f, 0.0f);
-    for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int2 se = table[key_of_cell(c, c0.x + dx, c0.y + dy, c0.z + dz)];
-                if (se.x == -1) continue;
-                force_run(c, pred_pos + se.x, vel_sorted + se.x, dens + se.x, se.y - se.x,
-                          (uint32_t)se.x, me, Fp, Fv);
-            }
+    force_walk<false>(c, me, pred_pos, vel_sorted, dens, nullptr, nullptr, cells, s_queue + threadIdx.x,
+               kBlock, Fp, Fv);
     float4 po, vo, vf;
     force_finish(c, P, V, myd.x, Fp, Fv, po, vo, vf);
     if (dbg_vel_force) dbg_vel_force[j] = vf;

@@ -6,23 +6,10 @@
 
 namespace sph {
 
-// calculateDensities.comp:122-128 over one cell run: pp[0..cnt) are the run's predicted
-// positions.  smoothingKernel / nearDensitySmoothingKernel are 0 for dist >= R, and
-// M + m*0 == M for M >= +0, so those candidates are skipped; d2 > R2_pre implies
-// sqrt(d2) >= R, which lets the sqrt be skipped for them too.
-__device__ __forceinline__ void density_run(const SceneConst &c, const float4 *__restrict__ pp,
-                                            int cnt, const float4 &s, float &M, float &Mn) {
-    for (int k = 0; k < cnt; ++k) {
-        const float4 p = pp[k];
-        const float ex = p.x - s.x, ey = p.y - s.y, ez = p.z - s.z;
-        const float d2 = ex * ex + ey * ey + ez * ez;
-        if (d2 > c.R2_pre) continue;
-        const float dist = sqrtf(d2);
-        if (dist >= c.R) continue;
-        const float t = c.R - dist;
-        M = M + c.m * (t * t);
-        Mn = Mn + c.m * (t * t * t);
-    }
+// Table cell of cell (x, y, z): cellCoordToKey (updateCellEntries.comp:59-61) offset by
+// the scene's first table cell.  Negative coordinates wrap through uint32.
+__device__ __forceinline__ uint32_t key_of_cell(const SceneConst &c, int32_t x, int32_t y, int32_t z) {
+    return c.table_base + (cell_hash((uint32_t)x, (uint32_t)y, (uint32_t)z) & c.mask);
 }
 
 // calculateDensities.comp:133-152
@@ -40,44 +27,209 @@ struct ForceSelf {
     uint32_t j;          // slot of the particle
 };
 
-// applyPressureAndViscosityForce.comp:297-322 over one cell run whose first slot is
-// `first`: pp/vv/dd hold the run's predicted positions, velocities and
-// (density, near density).
-__device__ __forceinline__ void force_run(const SceneConst &c, const float4 *__restrict__ pp,
-                                          const float4 *__restrict__ vv,
-                                          const float2 *__restrict__ dd, int cnt,
-                                          uint32_t first, const ForceSelf &me, float3 &Fp,
-                                          float3 &Fv) {
-    for (int k = 0; k < cnt; ++k) {
-        if (first + (uint32_t)k == me.j) continue;
-        const float4 pi = pp[k];
-        const float ex = pi.x - me.s.x, ey = pi.y - me.s.y, ez = pi.z - me.s.z;
-        const float d2 = ex * ex + ey * ey + ez * ez;
-        if (d2 > c.R2) continue;
-        const float dist = sqrtf(d2);
-        if (fabsf(dist) < 1e-7f) continue;
-        const float dirx = ex / dist, diry = ey / dist, dirz = ez / dist;
-        const float2 di = dd[k];
-        const float slope = dist >= c.R ? 0.0f : (dist - c.R) * c.slope_scale;
-        const float shared = ((di.x - c.rho0) * c.k + me.my_pressure) / 2.0f;
-        const float a = shared * slope * c.m / di.x;
-        Fp.x = Fp.x + a * dirx;
-        Fp.y = Fp.y + a * diry;
-        Fp.z = Fp.z + a * dirz;
-        const float infl = dist >= c.R ? 0.0f : (c.R - dist) * (c.R - dist);
-        const float b = c.m * infl / di.x;
-        const float4 vi = vv[k];
-        Fv.x = Fv.x + b * (vi.x - me.u.x);
-        Fv.y = Fv.y + b * (vi.y - me.u.y);
-        Fv.z = Fv.z + b * (vi.z - me.u.z);
-        const float shared_near = (me.near_pressure + di.y * c.kn) / 2.0f;
-        const float x = c.R - dist;
-        const float nslope = dist >= c.R ? 0.0f : x * x * c.near_slope_scale;
-        const float cn = shared_near * nslope * c.m / di.y;
-        Fp.x = Fp.x + cn * dirx;
-        Fp.y = Fp.y + cn * diry;
-        Fp.z = Fp.z + cn * dirz;
+// ---------------------------------------------------------------------------------------
+// Neighbour walks.
+//
+// One lane walks one particle's 27 cells in the reference's (dz, dy, dx) order and each
+// run in ascending slot order.  The cell loop is the same for every lane of a wave, so
+// the 27 run lookups are issued together; the runs themselves differ in length per lane.
+// For the force pass, a loop that did all of a candidate's work would make every lane
+// wait for the longest run and execute the expensive in-range arithmetic whenever any lane
+// needs it, so each run is only scanned with the distance test, and candidates that pass
+// are appended to a per-lane queue in LDS.  Queues are drained, in the order they were filled, when a lane's
+// queue is full, after a cell once some lane holds kFlushAt entries, and at the end: every
+// particle's sums keep the reference's order exactly, while the expensive arithmetic runs
+// with most lanes of the wave active.
+
+constexpr int kQueue = 12;
+constexpr int kFlushAt = 6;
+constexpr uint32_t kGlobalBit = 0x80000000u;  // queue entry names a global slot
+
+// Edge of a brick in cells and the halo staged around it.
+constexpr int kH1 = kBrick + 2;
+constexpr int kHalo = kH1 * kH1 * kH1;
+
+// Per halo cell of a brick: (LDS offset of the staged run or -1 when it is read from
+// global memory, run length), the run's first slot and the cell's full hash.
+struct Halo {
+    int2 oc[kHalo];
+    int gst[kHalo];
+    uint32_t hash[kHalo];
+    uint32_t staged;  // staged entries [0, staged)
+};
+
+// A run: entries [base, base + cnt) of the LDS copy or of the global arrays.
+struct RunRef {
+    int base;
+    int cnt;
+    int first;    // slot of the run's first particle
+    bool global;
+};
+
+// Finds cell runs: through the brick's halo when the cell lies in it (kHaloLds), else
+// through the global start/end table.
+template <bool kHaloLds>
+struct CellRuns {
+    uint32_t table_base, mask;  // the scene's first table cell and key mask
+    const int2 *__restrict__ table;
+    const Halo *H;
+    int ox, oy, oz;
+    __device__ __forceinline__ RunRef get(int cx, int cy, int cz) const {
+        if (kHaloLds) {
+            const int lx = cx - ox, ly = cy - oy, lz = cz - oz;
+            if ((unsigned)lx < (unsigned)kH1 && (unsigned)ly < (unsigned)kH1 &&
+                (unsigned)lz < (unsigned)kH1) {
+                const int h = lx + kH1 * (ly + kH1 * lz);
+                const int2 oc = H->oc[h];
+                const int g = H->gst[h];
+                return RunRef{oc.x >= 0 ? oc.x : g, oc.y, g, oc.x < 0};
+            }
+        }
+        const int2 se = table[table_base + (cell_hash((uint32_t)cx, (uint32_t)cy, (uint32_t)cz) & mask)];
+        if (se.x == -1) return RunRef{0, 0, 0, true};
+        return RunRef{se.x, se.y - se.x, se.x, true};
     }
+};
+
+// The density pass's 27 cells (calculateDensities.comp:114-118): sample points
+// s + R*(dx, dy, dz); a sample point outside the box skips its cell.
+template <bool kHaloLds>
+struct DensityCells {
+    CellRuns<kHaloLds> runs;
+    AxisCells ax, ay, az;
+    __device__ __forceinline__ RunRef operator()(int dz, int dy, int dx) const {
+        if (!(ax.in(dx) && ay.in(dy) && az.in(dz))) return RunRef{0, 0, 0, true};
+        return runs.get(ax.c(dx), ay.c(dy), az.c(dz));
+    }
+};
+
+// The force pass's 27 cells (applyPressure...comp:283-292): the sample point's cell plus
+// integer offsets, no bounds skip.
+template <bool kHaloLds>
+struct ForceCells {
+    CellRuns<kHaloLds> runs;
+    int3 c0;
+    __device__ __forceinline__ RunRef operator()(int dz, int dy, int dx) const {
+        return runs.get(c0.x + dx - 1, c0.y + dy - 1, c0.z + dz - 1);
+    }
+};
+
+// calculateDensities.comp:114-134 for sample point s.  With kLds false every run is read
+// from global memory and s_pos is unused.  The density's in-range arithmetic is short, so
+// it is done in the scan itself (no queue).
+template <bool kLds, class Cells>
+__device__ __forceinline__ float2 density_walk(const SceneConst &c, const float4 &s,
+                                               const float4 *__restrict__ pred_pos,
+                                               const float4 *s_pos, const Cells &cells) {
+    float M = 0.0f, Mn = 0.0f;
+    for (int dz = 0; dz < 3; ++dz)
+        for (int dy = 0; dy < 3; ++dy)
+            for (int dx = 0; dx < 3; ++dx) {
+                const RunRef r = cells(dz, dy, dx);
+                const bool global = !kLds || r.global;
+                for (int k = 0; k < r.cnt; ++k) {
+                    const int idx = r.base + k;
+                    const float4 p = global ? pred_pos[idx] : s_pos[idx];
+                    const float ex = p.x - s.x, ey = p.y - s.y, ez = p.z - s.z;
+                    const float d2 = ex * ex + ey * ey + ez * ez;
+                    // d2 > R2_pre implies sqrt(d2) >= R: both kernels are 0 and
+                    // M + m*0 == M for M >= +0, so the candidate changes nothing.
+                    if (d2 > c.R2_pre) continue;
+                    const float dist = sqrtf(d2);
+                    if (dist >= c.R) continue;
+                    const float t = c.R - dist;
+                    M = M + c.m * (t * t);
+                    Mn = Mn + c.m * (t * t * t);
+                }
+            }
+    return density_finish(c, M, Mn);
+}
+
+// Full arithmetic of one force candidate (applyPressureAndViscosityForce.comp:303-321).
+// Staged neighbours are packed as s_a = (predicted position xyz, near density) and
+// s_b = (velocity xyz, density).
+template <bool kLds>
+__device__ __forceinline__ void force_term(const SceneConst &c, const ForceSelf &me, uint32_t e,
+                                           const float4 *__restrict__ pred_pos,
+                                           const float4 *__restrict__ vel_sorted,
+                                           const float2 *__restrict__ dens, const float4 *s_a,
+                                           const float4 *s_b, float3 &Fp, float3 &Fv) {
+    float3 pi, vi;
+    float2 di;
+    if (!kLds || (e & kGlobalBit)) {
+        const uint32_t g = e & ~kGlobalBit;
+        const float4 a = pred_pos[g], b = vel_sorted[g];
+        pi = make_float3(a.x, a.y, a.z);
+        vi = make_float3(b.x, b.y, b.z);
+        di = dens[g];
+    } else {
+        const float4 a = s_a[e], b = s_b[e];
+        pi = make_float3(a.x, a.y, a.z);
+        vi = make_float3(b.x, b.y, b.z);
+        di = make_float2(b.w, a.w);
+    }
+    const float ex = pi.x - me.s.x, ey = pi.y - me.s.y, ez = pi.z - me.s.z;
+    const float d2 = ex * ex + ey * ey + ez * ez;
+    const float dist = sqrtf(d2);
+    if (fabsf(dist) < 1e-7f) return;
+    const float dirx = ex / dist, diry = ey / dist, dirz = ez / dist;
+    const float slope = dist >= c.R ? 0.0f : (dist - c.R) * c.slope_scale;
+    const float shared = ((di.x - c.rho0) * c.k + me.my_pressure) / 2.0f;
+    const float a = shared * slope * c.m / di.x;
+    Fp.x = Fp.x + a * dirx;
+    Fp.y = Fp.y + a * diry;
+    Fp.z = Fp.z + a * dirz;
+    const float infl = dist >= c.R ? 0.0f : (c.R - dist) * (c.R - dist);
+    const float b = c.m * infl / di.x;
+    Fv.x = Fv.x + b * (vi.x - me.u.x);
+    Fv.y = Fv.y + b * (vi.y - me.u.y);
+    Fv.z = Fv.z + b * (vi.z - me.u.z);
+    const float shared_near = (me.near_pressure + di.y * c.kn) / 2.0f;
+    const float x = c.R - dist;
+    const float nslope = dist >= c.R ? 0.0f : x * x * c.near_slope_scale;
+    const float cn = shared_near * nslope * c.m / di.y;
+    Fp.x = Fp.x + cn * dirx;
+    Fp.y = Fp.y + cn * diry;
+    Fp.z = Fp.z + cn * dirz;
+}
+
+// applyPressureAndViscosityForce.comp:288-325: accumulates Fp and Fv for `me`.
+template <bool kLds, class Cells>
+__device__ __forceinline__ void force_walk(const SceneConst &c, const ForceSelf &me,
+                                           const float4 *__restrict__ pred_pos,
+                                           const float4 *__restrict__ vel_sorted,
+                                           const float2 *__restrict__ dens,
+                                           const float4 *s_a, const float4 *s_b,
+                                           const Cells &cells, uint32_t *q, int stride,
+                                           float3 &Fp, float3 &Fv) {
+    int qc = 0;
+    for (int dz = 0; dz < 3; ++dz)
+        for (int dy = 0; dy < 3; ++dy)
+            for (int dx = 0; dx < 3; ++dx) {
+                const RunRef r = cells(dz, dy, dx);
+                const bool global = !kLds || r.global;
+                for (int k = 0; k < r.cnt; ++k) {
+                    if ((uint32_t)(r.first + k) == me.j) continue;
+                    const int idx = r.base + k;
+                    const float4 p = global ? pred_pos[idx] : s_a[idx];
+                    const float ex = p.x - me.s.x, ey = p.y - me.s.y, ez = p.z - me.s.z;
+                    const float d2 = ex * ex + ey * ey + ez * ez;
+                    if (d2 > c.R2) continue;
+                    q[qc * stride] = global ? (kGlobalBit | (uint32_t)idx) : (uint32_t)idx;
+                    if (++qc == kQueue) {
+                        for (int i = 0; i < qc; ++i)
+                            force_term<kLds>(c, me, q[i * stride], pred_pos, vel_sorted, dens, s_a, s_b, Fp, Fv);
+                        qc = 0;
+                    }
+                }
+                if (__ballot(qc >= kFlushAt)) {
+                    for (int i = 0; i < qc; ++i)
+                        force_term<kLds>(c, me, q[i * stride], pred_pos, vel_sorted, dens, s_a, s_b, Fp, Fv);
+                    qc = 0;
+                }
+            }
+    for (int i = 0; i < qc; ++i)
+        force_term<kLds>(c, me, q[i * stride], pred_pos, vel_sorted, dens, s_a, s_b, Fp, Fv);
 }
 
 // Self terms of the force pass for slot j (applyPressureAndViscosityForce.comp:276-286).
@@ -118,8 +270,5 @@ __device__ __forceinline__ void force_finish(const SceneConst &c, const float4 &
     vel_out = make_float4(v.x, v.y, v.z, 1.0f);
 }
 
-__device__ __forceinline__ uint32_t key_of_cell(const SceneConst &c, int32_t x, int32_t y, int32_t z) {
-    return c.table_base + (cell_hash((uint32_t)x, (uint32_t)y, (uint32_t)z) & c.mask);
-}
 
 }  // namespace sph

@@ -6,7 +6,10 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -301,7 +304,109 @@ struct sph_world {
     GraphCache graph;
     bool debug = false;
     bool stepped = false;
+    int variant = SPH_KERNEL_AUTO;  // requested neighbour-pass kernels
+    bool tuned = false;             // AUTO resolved (by the tuning cache or a measurement)
 };
+
+namespace {
+
+// Autotuning: every kernel variant gives bit-identical results, so the choice only
+// affects speed.  A variant is chosen per size bucket (particle count and table size,
+// rounded to powers of two) by timing a few substeps of each candidate on the world's own
+// state (saved and restored around the measurement) and cached for the process.
+std::mutex g_tune_mu;
+std::map<uint64_t, int> g_tune_cache;
+
+uint64_t tune_bucket(uint32_t n, uint32_t T, uint64_t bricks) {
+    return ((uint64_t)key_bits_for(n) << 48) | ((uint64_t)key_bits_for(T) << 32) |
+           (uint64_t)key_bits_for(bricks ? bricks : 1);
+}
+
+int world_apply_variant(sph_world *w, int v) {
+    w->chain.bricks = v == SPH_KERNEL_BRICK && w->chain.brick_total > 0;
+    w->graph.drop();
+    return SPH_OK;
+}
+
+// Times `substeps` eager substeps of each variant; restores the state afterwards.
+int world_autotune(sph_world *w, uint32_t substeps, int *chosen, double *ms_out) {
+    const size_t bytes = (size_t)w->n * 16;
+    float4 *save_p = nullptr, *save_v = nullptr;
+    hipError_t e = hipMalloc((void **)&save_p, bytes);
+    if (e == hipSuccess) e = hipMalloc((void **)&save_v, bytes);
+    if (e == hipSuccess) e = hipMemcpyAsync(save_p, w->chain.b.pos, bytes, hipMemcpyDeviceToDevice, w->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(save_v, w->chain.b.vel, bytes, hipMemcpyDeviceToDevice, w->stream);
+    if (e != hipSuccess) {
+        if (save_p) hipFree(save_p);
+        if (save_v) hipFree(save_v);
+        return hip_fail(e, "autotune state copy");
+    }
+    const int cands[2] = {SPH_KERNEL_DIRECT, SPH_KERNEL_BRICK};
+    int best = SPH_KERNEL_DIRECT;
+    double best_ms = 1e30;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    hipEventCreate(&t0);
+    hipEventCreate(&t1);
+    for (int v : cands) {
+        if (v == SPH_KERNEL_BRICK && w->chain.brick_total == 0) continue;
+        world_apply_variant(w, v);
+        // one untimed substep, then the timed ones, each from the saved state
+        e = hipMemcpyAsync(w->chain.b.pos, save_p, bytes, hipMemcpyDeviceToDevice, w->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(w->chain.b.vel, save_v, bytes, hipMemcpyDeviceToDevice, w->stream);
+        if (e == hipSuccess) e = w->chain.enqueue(w->stream, w->scene, nullptr);
+        if (e == hipSuccess) e = hipEventRecord(t0, w->stream);
+        for (uint32_t s = 0; s < substeps && e == hipSuccess; ++s) e = w->chain.enqueue(w->stream, w->scene, nullptr);
+        if (e == hipSuccess) e = hipEventRecord(t1, w->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(t1);
+        if (e != hipSuccess) break;
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, t0, t1);
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = v;
+        }
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(w->chain.b.pos, save_p, bytes, hipMemcpyDeviceToDevice, w->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(w->chain.b.vel, save_v, bytes, hipMemcpyDeviceToDevice, w->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
+    hipEventDestroy(t0);
+    hipEventDestroy(t1);
+    hipFree(save_p);
+    hipFree(save_v);
+    if (e != hipSuccess) return hip_fail(e, "autotune");
+    world_apply_variant(w, best);
+    *chosen = best;
+    if (ms_out) *ms_out = best_ms / substeps;
+    return SPH_OK;
+}
+
+// Resolves AUTO before the first step of a configuration.
+int world_resolve_variant(sph_world *w) {
+    if (w->tuned || w->variant != SPH_KERNEL_AUTO) return SPH_OK;
+    const uint64_t key = tune_bucket(w->n, w->T, w->chain.brick_total);
+    int v = -1;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) v = it->second;
+    }
+    if (v < 0) {
+        const char *off = std::getenv("SPH_AUTOTUNE");
+        if (off && std::strcmp(off, "0") == 0) {
+            v = SPH_KERNEL_DIRECT;
+        } else {
+            int rc = world_autotune(w, 3, &v, nullptr);
+            if (rc) return rc;
+        }
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_cache[key] = v;
+    }
+    world_apply_variant(w, v);
+    w->tuned = true;
+    return SPH_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -346,6 +451,7 @@ int sph_world_create(const sph_params *params, uint32_t n, sph_world **out) {
         delete w;
         return rc;
     }
+    world_apply_variant(w, SPH_KERNEL_DIRECT);
     *out = w;
     return SPH_OK;
 }
@@ -383,6 +489,8 @@ int sph_world_step(sph_world *w, uint32_t frames) {
     if (!w) return fail(SPH_ERR_INVALID_ARG, "world is NULL");
     const uint64_t substeps = (uint64_t)frames * w->params.substeps;
     if (substeps == 0) return SPH_OK;
+    int rc0 = world_resolve_variant(w);
+    if (rc0) return rc0;
     if (!w->graph.exec) {
         int rc = w->graph.ensure_begin(w->stream);
         if (rc) return rc;
@@ -430,9 +538,29 @@ int sph_world_set_kernel(sph_world *w, int variant) {
         return fail(SPH_ERR_INVALID_ARG, "brick kernels need a grid of at most 2^28 bricks");
     hipError_t e = hipStreamSynchronize(w->stream);
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
-    w->chain.bricks = variant == SPH_KERNEL_DIRECT ? false : w->chain.brick_total > 0;
-    w->graph.drop();
+    w->variant = variant;
+    w->tuned = variant != SPH_KERNEL_AUTO;
+    world_apply_variant(w, variant == SPH_KERNEL_AUTO ? SPH_KERNEL_DIRECT : variant);
     return SPH_OK;
+}
+
+int sph_world_autotune(sph_world *w, uint32_t substeps, int *chosen, double *ms_per_substep) {
+    if (!w || !chosen) return fail(SPH_ERR_INVALID_ARG, "NULL argument");
+    if (substeps == 0) return fail(SPH_ERR_INVALID_ARG, "substeps must be >= 1");
+    hipError_t e = hipStreamSynchronize(w->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    int rc = world_autotune(w, substeps, chosen, ms_per_substep);
+    if (rc) return rc;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[tune_bucket(w->n, w->T, w->chain.brick_total)] = *chosen;
+    w->variant = SPH_KERNEL_AUTO;
+    w->tuned = true;
+    return SPH_OK;
+}
+
+int sph_world_kernel(const sph_world *w) {
+    if (!w) return fail(SPH_ERR_INVALID_ARG, "world is NULL");
+    return w->chain.bricks ? SPH_KERNEL_BRICK : SPH_KERNEL_DIRECT;
 }
 
 int sph_world_set_debug(sph_world *w, int enable) {
@@ -455,8 +583,10 @@ int sph_world_debug_dump(sph_world *w, int stage, void *dst, size_t bytes) {
 int sph_world_pass_times(sph_world *w, uint32_t substeps, sph_pass_times *out) {
     if (!w || !out) return fail(SPH_ERR_INVALID_ARG, "NULL argument");
     if (substeps == 0) return fail(SPH_ERR_INVALID_ARG, "substeps must be >= 1");
+    int rc = world_resolve_variant(w);
+    if (rc) return rc;
     PassEvents ev;
-    int rc = ev.create();
+    rc = ev.create();
     if (rc) { ev.destroy(); return rc; }
     double acc[5] = {0, 0, 0, 0, 0};
     for (uint32_t s = 0; s < substeps; ++s) {

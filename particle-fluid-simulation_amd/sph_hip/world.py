@@ -64,6 +64,21 @@ class World:
         """Neighbour-pass kernels: "auto", "direct" (one lane per particle) or "brick"."""
         check(self._lib.sph_world_set_kernel(self._h, KERNELS[variant]))
 
+    def autotune(self, substeps: int = 3):
+        """Times every kernel variant on this world's state (restored afterwards) and keeps
+        the fastest; returns (variant name, ms per substep)."""
+        v = C.c_int()
+        ms = C.c_double()
+        check(self._lib.sph_world_autotune(self._h, substeps, C.byref(v), C.byref(ms)))
+        names = {b: a for a, b in KERNELS.items()}
+        return names[v.value], ms.value
+
+    @property
+    def kernel(self) -> str:
+        r = self._lib.sph_world_kernel(self._h)
+        check(min(r, 0))
+        return {1: "direct", 2: "brick"}[r]
+
     def set_debug(self, on: bool) -> None:
         check(self._lib.sph_world_set_debug(self._h, 1 if on else 0))
 

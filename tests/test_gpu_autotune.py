@@ -1,0 +1,39 @@
+"""Autotuner: the measurement leaves the state untouched and whichever variant it picks
+stays bit-exact with the oracle."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_autotune_keeps_state_and_parity(gpu, orc):
+    from sph_hip.workloads import DAM_BREAK_16K as W
+    params = W.params()
+    pos, vel = W.state()
+    with gpu.World(params, W.n) as w:
+        w.upload(pos, vel)
+        before = w.download()
+        chosen, ms = w.autotune(2)
+        after = w.download()
+        assert chosen in ("direct", "brick") and ms > 0
+        assert w.kernel == chosen
+        assert np.array_equal(before[0].view(np.uint32), after[0].view(np.uint32))
+        assert np.array_equal(before[1].view(np.uint32), after[1].view(np.uint32))
+        w.step(2)
+        gp, gv = w.download()
+    rp, rv = orc.step(params, pos, vel, 2)
+    assert np.array_equal(gp.view(np.uint32), rp.view(np.uint32))
+    assert np.array_equal(gv.view(np.uint32), rv.view(np.uint32))
+
+
+def test_auto_world_steps_like_oracle(gpu, orc):
+    from sph_hip.workloads import DAM_BREAK_16K as W
+    params = W.params()
+    pos, vel = W.state(6000)
+    with gpu.World(params, 6000) as w:  # AUTO: tunes on the first step
+        w.upload(pos, vel)
+        w.step(3)
+        gp, gv = w.download()
+    rp, rv = orc.step(params, pos, vel, 3)
+    assert np.array_equal(gp.view(np.uint32), rp.view(np.uint32))
+    assert np.array_equal(gv.view(np.uint32), rv.view(np.uint32))
