@@ -68,7 +68,15 @@ class SceneSpec:
             substeps=base.substeps, particle_mass=base.particle_mass,
             pressure_multiplier=base.pressure_multiplier, target_density=self.target_density,
             viscosity_strength=base.viscosity_strength,
-            near_pressure_multiplier=base.near_pressure_multiplier, table_cells=0)
+            near_pressure_multiplier=base.near_pressure_multiplier,
+            table_cells=self.table_cells)
+
+    @property
+    def table_cells(self) -> int:
+        """Next power of two >= n: the key mask T-1 must be all ones below its top bit
+        (updateCellEntries.comp:55 requires a power-of-two count), otherwise a scene of
+        e.g. 2077 particles would have only 32 distinct keys."""
+        return 1 << max(0, (self.n - 1).bit_length())
 
     def state(self):
         """Uniform fill of the lower half of the box, zero velocity."""
