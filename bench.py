@@ -123,6 +123,81 @@ def mixed_batch(steps: int, warmup: int, count: int = 256) -> dict:
             "plain_ms_per_step": t_plain * 1e3 / steps, "scenes_on_brick_kernels": bricks}
 
 
+def request_stream(count: int, seed: int, rate_hz: float, n_range=(4096, 262144),
+                   steps_range=(50, 500)):
+    """Serving workload: `count` requests with Poisson arrivals at `rate_hz`; each draws
+    (numpy PCG64, seed + i) a particle count log-uniform in n_range, a step count uniform in
+    steps_range and a dam-break (particles in the lower-x half of the box) or uniform-fill
+    start.  Boxes are 2a x a x a with a = (n / 130)^(1/3), i.e. 65 particles per unit^3
+    over the whole box; kernel parameters are those of dam_break_16k, T = next pow2 >= n."""
+    import numpy as np
+    import sph_hip
+    from sph_hip.workloads import DAM_BREAK_16K as base
+    rng = np.random.Generator(np.random.PCG64(seed))
+    t = 0.0
+    out = []
+    for i in range(count):
+        t += float(rng.exponential(1.0 / rate_hz))
+        r = np.random.Generator(np.random.PCG64(seed + 1 + i))
+        n = int(round(float(np.exp(r.uniform(np.log(n_range[0]), np.log(n_range[1]))))))
+        steps = int(r.integers(steps_range[0], steps_range[1] + 1))
+        dam = bool(r.integers(0, 2))
+        a = float(np.float32((n / 130.0) ** (1.0 / 3.0)))
+        box = (2 * a, a, a)
+        params = sph_hip.params_for(
+            particle_radius=base.particle_radius, smoothing_radius=base.smoothing_radius,
+            g=base.g, box_min=(0, 0, 0), box_max=box, frame_dt=base.frame_dt,
+            substeps=base.substeps, pressure_multiplier=base.pressure_multiplier,
+            target_density=base.target_density, viscosity_strength=base.viscosity_strength,
+            table_cells=1 << max(0, (n - 1).bit_length()))
+        hi = (a if dam else 2 * a, a, a)
+        pos, vel = sph_hip.fill_block(n, seed + 1 + i, (0, 0, 0), hi, 0.0)
+        out.append((t, params, pos, vel, steps))
+    return out
+
+
+def serving(count: int, rate_hz: float, seed: int = 4, resident: int = 0,
+            budget_gb: float = 8.0) -> dict:
+    """Submits the request stream at its arrival times to a Server and waits for all; with
+    `resident` > 0 a scene of that many particles (uniform fill of a 160x40x80 box) is
+    submitted first with enough steps to stay resident for the whole stream."""
+    import sph_hip
+    from sph_hip.workloads import DAM_BREAK_16K as base
+    reqs = request_stream(count, seed, rate_hz)
+    max_n = sum(r[2].shape[0] for r in reqs) + resident
+    max_t = sum(r[1].table_cells for r in reqs) + (1 << max(0, (resident - 1).bit_length()) if resident else 0)
+    srv = sph_hip.Server(hbm_budget_bytes=int(budget_gb * (1 << 30)), max_scenes=min(65535, count + 1),
+                         max_particles=max_n, max_table_cells=max_t, substeps=base.substeps)
+    big = None
+    if resident:
+        box = (160.0, 40.0, 80.0)
+        params = sph_hip.params_for(
+            particle_radius=base.particle_radius, smoothing_radius=base.smoothing_radius,
+            g=base.g, box_min=(0, 0, 0), box_max=box, frame_dt=base.frame_dt, substeps=base.substeps,
+            pressure_multiplier=base.pressure_multiplier, target_density=base.target_density,
+            table_cells=1 << max(0, (resident - 1).bit_length()))
+        pos, vel = sph_hip.fill_block(resident, seed, (0, 0, 0), box, 0.1)
+        big = srv.submit(params, pos, vel, 1 << 30)
+    t0 = time.perf_counter()
+    ids = []
+    for t, params, pos, vel, steps in reqs:
+        delay = t - (time.perf_counter() - t0)
+        if delay > 0:
+            time.sleep(delay)
+        ids.append(srv.submit(params, pos, vel, steps))
+    for rid in ids:
+        srv.wait(rid, 600.0)
+        srv.result(rid)
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    st.update({"requests": count, "rate_hz": rate_hz, "wall_s": wall,
+               "request_particles": sum(r[2].shape[0] for r in reqs),
+               "request_particle_steps": float(sum(r[2].shape[0] * r[4] for r in reqs)),
+               "resident_scene_particles": resident})
+    return st
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -134,6 +209,13 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--batch-steps", type=int, default=100,
                     help="steps of the mixed 256-scene batched-vs-plain comparison (0 = skip)")
+    ap.add_argument("--serve-requests", type=int, default=60,
+                    help="requests of the Poisson serving run (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=10.0, help="request arrivals per second")
+    ap.add_argument("--serve-resident", type=int, default=0,
+                    help="particles of a scene kept resident beside the request stream "
+                         "(33554432 = the mixed-serving-at-scale case)")
+    ap.add_argument("--serve-budget-gb", type=float, default=8.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (scenes shard "
@@ -192,6 +274,15 @@ def main() -> int:
         mixed = mixed_batch(args.batch_steps, 3)
         print(f"[bench] batched/plain = {mixed['batched_over_plain']:.2f}", file=sys.stderr, flush=True)
 
+    serve = None
+    if args.serve_requests > 0:
+        print(f"[bench] serving {args.serve_requests} requests at {args.serve_rate}/s", file=sys.stderr,
+              flush=True)
+        serve = serving(args.serve_requests, args.serve_rate, resident=args.serve_resident,
+                        budget_gb=args.serve_budget_gb)
+        print(f"[bench] serving p50/p99 scene {serve['scene_p50_ms']:.1f}/{serve['scene_p99_ms']:.1f} ms",
+              file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(W, T, args.cpu_budget)
@@ -215,6 +306,7 @@ def main() -> int:
         "pass_ms": pt,
         "pass_ms_by_kernel": variants,
         "mixed_batch": mixed,
+        "serving": serve,
         "state_finite": ok,
         "cpu_baseline": cpu,
     }

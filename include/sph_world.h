@@ -175,6 +175,50 @@ int sph_batch_pass_times(sph_batch *b, uint32_t substeps, sph_pass_times *out);
 uint32_t sph_batch_particles(const sph_batch *b);
 void sph_batch_destroy(sph_batch *b);
 
+/* ---------------------------------------------------------------------------------------
+ * Serving: a request loop over the batched step.  No reference counterpart (the reference
+ * runs one interactive scene).  Requests carry their parameters, initial state and step
+ * count; a worker thread admits them in arrival order while the sum of resident footprints
+ * (128 B per particle + 8 B per table cell) stays within hbm_budget_bytes and the
+ * capacities hold, keeps their state resident in device memory, steps every resident scene
+ * together one frame at a time, and retires finished scenes with their state copied back.
+ * Requests arriving while others run are admitted at the next frame boundary (continuous
+ * batching); each result is bit-identical to stepping that scene alone.  Every request
+ * must use the server's substep count.  Calls are thread-safe. */
+typedef struct sph_server sph_server;
+typedef struct sph_server_config {
+    uint64_t hbm_budget_bytes;  /* admission budget for resident scenes                   */
+    uint32_t max_scenes;        /* resident scenes at once (<= 65535)                     */
+    uint32_t max_particles;     /* resident particles at once (buffers sized for it)      */
+    uint32_t max_table_cells;   /* resident table cells at once                           */
+    uint32_t max_bricks;        /* resident bricks at once; 0 = 2^22                      */
+    uint32_t substeps;          /* substeps per frame, shared by all requests             */
+} sph_server_config;
+typedef enum sph_request_state {
+    SPH_REQ_QUEUED = 0, SPH_REQ_RUNNING = 1, SPH_REQ_DONE = 2, SPH_REQ_FAILED = 3
+} sph_request_state;
+typedef struct sph_server_stats {
+    uint64_t submitted, completed, failed, frames;
+    uint32_t resident_scenes, queued;
+    uint64_t resident_particles, resident_bytes;
+    double step_p50_ms, step_p99_ms;    /* wall time of one batched frame               */
+    double scene_p50_ms, scene_p99_ms;  /* submit -> result ready                       */
+    double wait_p50_ms, wait_p99_ms;    /* submit -> admitted                           */
+    double particle_steps_per_s;        /* resident particles x frames / stepping time  */
+} sph_server_stats;
+int sph_server_create(const sph_server_config *cfg, sph_server **out);
+/* Validates and copies the request; invalid requests (NaN or out-of-box particles, n = 0,
+ * bad parameters) are refused here and never reach the batch. */
+int sph_server_submit(sph_server *s, const sph_params *params, uint32_t n,
+                      const float *pos_vec4, const float *vel_vec4, uint32_t steps,
+                      uint64_t *request_id);
+int sph_server_poll(sph_server *s, uint64_t request_id);  /* sph_request_state or < 0 */
+int sph_server_wait(sph_server *s, uint64_t request_id, double timeout_s);  /* < 0: forever */
+/* Copies a finished request's state (vec4 AoS, sorted order) and forgets the request. */
+int sph_server_result(sph_server *s, uint64_t request_id, float *pos_vec4, float *vel_vec4);
+int sph_server_get_stats(sph_server *s, sph_server_stats *out);
+void sph_server_destroy(sph_server *s);
+
 /* Seeded scene initialisers (host only).  Counterparts of initializePositionsBuffer /
  * initializeVelocitiesBuffer (renderer.c:569-645) for synthetic workloads: PCG32 streams,
  * w = 0 like the reference's initial buffers.

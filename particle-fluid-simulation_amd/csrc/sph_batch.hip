@@ -42,6 +42,32 @@ struct sph_batch {
     ManyScenes policy() const { return ManyScenes{d_scenes, d_scene_of, count}; }
 };
 
+namespace sph {
+
+// scene_of[j] = the scene whose slot range [p_begin, p_end) holds slot j.
+__global__ __launch_bounds__(256) void k_fill_scene_of(const SceneConst *__restrict__ table,
+                                                       uint32_t count, uint16_t *__restrict__ scene_of,
+                                                       uint32_t n) {
+    const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+    if (j >= n) return;
+    uint32_t lo = 0, hi = count;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (table[mid].p_begin <= j) lo = mid; else hi = mid;
+    }
+    scene_of[j] = (uint16_t)lo;
+}
+
+hipError_t launch_fill_scene_of(hipStream_t stream, const SceneConst *table, uint32_t count,
+                                uint16_t *scene_of, uint32_t n) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_scene_of, dim3((n + 255u) / 256u), dim3(256), 0, stream, table, count,
+                       scene_of, n);
+    return hipGetLastError();
+}
+
+}  // namespace sph
+
 namespace {
 
 // A scene runs through the brick kernels when its particles are dense enough for a brick's

@@ -35,8 +35,12 @@ struct Chain {
     uint32_t n = 0, table_total = 0, key_bits = 0, passes = 0, tiles = 0, lookback_words = 0;
     uint32_t brick_total = 0;  // bricks over all scenes; 0 disables the brick kernels
     bool bricks = false;       // density/force through the brick kernels (else per lane)
+    uint32_t n_cap = 0, t_cap = 0, b_cap = 0;  // allocated capacity
 
+    // Allocates for the given capacity and configures the chain to use all of it.
     int allocate(uint32_t n, uint32_t table_total, uint32_t brick_total);
+    // Uses the first n slots, table_total cells and brick_total bricks (within capacity).
+    void configure(uint32_t n, uint32_t table_total, uint32_t brick_total);
     int set_debug(bool on);
     void release();
     const uint32_t *sorted_keys() const;

@@ -67,6 +67,10 @@ hipError_t launch_force(hipStream_t stream, const Scenes &sc, const ChainBuffers
 
 // Brick path (sph_brick.hip): compacts the non-empty bricks into the work queue, then
 // persistent workgroups drain it for the density and the force + integrate passes.
+// scene_of[j] for slots [0, n) from the scenes' slot ranges (table sorted by p_begin).
+hipError_t launch_fill_scene_of(hipStream_t stream, const SceneConst *table, uint32_t count,
+                                uint16_t *scene_of, uint32_t n);
+
 void brick_prepare();  // caches occupancy-derived grid sizes; call outside graph capture
 hipError_t launch_queue(hipStream_t stream, const ChainBuffers &b, uint32_t brick_total);
 template <class Scenes>

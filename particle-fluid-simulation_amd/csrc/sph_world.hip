@@ -133,15 +133,22 @@ uint32_t key_bits_for(uint64_t table_total) {
 // ---------------------------------------------------------------------------------------
 // Chain: buffers and launches of one substep, shared by the single world and the batch.
 
-int Chain::allocate(uint32_t n_, uint32_t table_total_, uint32_t brick_total_) {
+void Chain::configure(uint32_t n_, uint32_t table_total_, uint32_t brick_total_) {
     n = n_;
     table_total = table_total_;
     brick_total = brick_total_;
-    bricks = brick_total > 0;
     key_bits = key_bits_for(table_total);
     passes = (key_bits + 7u) / 8u;
     tiles = (n + radix_tile_size() - 1u) / radix_tile_size();
     lookback_words = passes * tiles * 256u;
+}
+
+int Chain::allocate(uint32_t n_, uint32_t table_total_, uint32_t brick_total_) {
+    n_cap = n_;
+    t_cap = table_total_;
+    b_cap = brick_total_;
+    configure(n_, table_total_, brick_total_);
+    bricks = brick_total > 0;
     struct Req { void **p; size_t bytes; };
     const size_t N = n;
     Req reqs[] = {
@@ -337,16 +344,16 @@ int world_autotune(sph_world *w, uint32_t substeps, int *chosen, double *ms_out)
     if (e == hipSuccess) e = hipMemcpyAsync(save_p, w->chain.b.pos, bytes, hipMemcpyDeviceToDevice, w->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(save_v, w->chain.b.vel, bytes, hipMemcpyDeviceToDevice, w->stream);
     if (e != hipSuccess) {
-        if (save_p) hipFree(save_p);
-        if (save_v) hipFree(save_v);
+        if (save_p) (void)hipFree(save_p);
+        if (save_v) (void)hipFree(save_v);
         return hip_fail(e, "autotune state copy");
     }
     const int cands[2] = {SPH_KERNEL_DIRECT, SPH_KERNEL_BRICK};
     int best = SPH_KERNEL_DIRECT;
     double best_ms = 1e30;
     hipEvent_t t0 = nullptr, t1 = nullptr;
-    hipEventCreate(&t0);
-    hipEventCreate(&t1);
+    (void)hipEventCreate(&t0);
+    (void)hipEventCreate(&t1);
     for (int v : cands) {
         if (v == SPH_KERNEL_BRICK && w->chain.brick_total == 0) continue;
         world_apply_variant(w, v);
@@ -360,7 +367,7 @@ int world_autotune(sph_world *w, uint32_t substeps, int *chosen, double *ms_out)
         if (e == hipSuccess) e = hipEventSynchronize(t1);
         if (e != hipSuccess) break;
         float ms = 0.0f;
-        hipEventElapsedTime(&ms, t0, t1);
+        (void)hipEventElapsedTime(&ms, t0, t1);
         if (ms < best_ms) {
             best_ms = ms;
             best = v;
@@ -369,10 +376,10 @@ int world_autotune(sph_world *w, uint32_t substeps, int *chosen, double *ms_out)
     if (e == hipSuccess) e = hipMemcpyAsync(w->chain.b.pos, save_p, bytes, hipMemcpyDeviceToDevice, w->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(w->chain.b.vel, save_v, bytes, hipMemcpyDeviceToDevice, w->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
-    hipEventDestroy(t0);
-    hipEventDestroy(t1);
-    hipFree(save_p);
-    hipFree(save_v);
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    (void)hipFree(save_p);
+    (void)hipFree(save_v);
     if (e != hipSuccess) return hip_fail(e, "autotune");
     world_apply_variant(w, best);
     *chosen = best;

@@ -9,3 +9,4 @@ from .abi import (  # noqa: F401
 )
 from .world import World  # noqa: F401
 from .batch import Batch  # noqa: F401
+from .server import Server  # noqa: F401

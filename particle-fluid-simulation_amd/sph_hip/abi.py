@@ -109,6 +109,14 @@ def load_library() -> C.CDLL:
         "sph_batch_pass_times": (C.c_int, [vp, C.c_uint32, P(PassTimes)]),
         "sph_batch_particles": (C.c_uint32, [vp]),
         "sph_batch_destroy": (None, [vp]),
+        "sph_server_create": (C.c_int, [vp, P(vp)]),
+        "sph_server_submit": (C.c_int, [vp, P(SphParams), C.c_uint32, vp, vp, C.c_uint32,
+                                        P(C.c_uint64)]),
+        "sph_server_poll": (C.c_int, [vp, C.c_uint64]),
+        "sph_server_wait": (C.c_int, [vp, C.c_uint64, C.c_double]),
+        "sph_server_result": (C.c_int, [vp, C.c_uint64, vp, vp]),
+        "sph_server_get_stats": (C.c_int, [vp, vp]),
+        "sph_server_destroy": (None, [vp]),
         "sph_scene_fill_block": (C.c_int, [C.c_uint32, C.c_uint64, P(C.c_float * 3),
                                            P(C.c_float * 3), C.c_float, vp, vp]),
     }

@@ -90,3 +90,16 @@ def test_footprint_is_monotone():
     f2 = lib.sph_world_footprint(1 << 20, 1 << 21)
     assert f2 - f1 == 8 * (1 << 20)
     assert lib.sph_world_footprint(1 << 20, 0) == f1
+
+
+def test_server_rejects_bad_config_without_a_device():
+    import sph_hip
+    from sph_hip.server import ServerConfig
+    lib = sph_hip.load_library()
+    h = C.c_void_p()
+    cfg = ServerConfig(1 << 30, 0, 1024, 1024, 0, 3)
+    assert lib.sph_server_create(C.byref(cfg), C.byref(h)) == -1
+    cfg = ServerConfig(1 << 30, 4, 0, 1024, 0, 3)
+    assert lib.sph_server_create(C.byref(cfg), C.byref(h)) == -1
+    cfg = ServerConfig(1 << 30, 4, 1024, 1024, 0, 0)
+    assert lib.sph_server_create(C.byref(cfg), C.byref(h)) == -1
