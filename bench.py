@@ -128,8 +128,10 @@ def request_stream(count: int, seed: int, rate_hz: float, n_range=(4096, 262144)
     """Serving workload: `count` requests with Poisson arrivals at `rate_hz`; each draws
     (numpy PCG64, seed + i) a particle count log-uniform in n_range, a step count uniform in
     steps_range and a dam-break (particles in the lower-x half of the box) or uniform-fill
-    start.  Boxes are 2a x a x a with a = (n / 130)^(1/3), i.e. 65 particles per unit^3
-    over the whole box; kernel parameters are those of dam_break_16k, T = next pow2 >= n."""
+    start.  Boxes are 2a x a x a sized so the start is at the target density 55: uniform
+    fill of the whole box (a = (n / 110)^(1/3)) or a block filling its lower-x half
+    (a = (n / 55)^(1/3)); kernel parameters are those of dam_break_16k, T = next
+    pow2 >= n."""
     import numpy as np
     import sph_hip
     from sph_hip.workloads import DAM_BREAK_16K as base
@@ -142,7 +144,7 @@ def request_stream(count: int, seed: int, rate_hz: float, n_range=(4096, 262144)
         n = int(round(float(np.exp(r.uniform(np.log(n_range[0]), np.log(n_range[1]))))))
         steps = int(r.integers(steps_range[0], steps_range[1] + 1))
         dam = bool(r.integers(0, 2))
-        a = float(np.float32((n / 130.0) ** (1.0 / 3.0)))
+        a = float(np.float32((n / (55.0 if dam else 110.0)) ** (1.0 / 3.0)))
         box = (2 * a, a, a)
         params = sph_hip.params_for(
             particle_radius=base.particle_radius, smoothing_radius=base.smoothing_radius,

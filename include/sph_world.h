@@ -184,7 +184,9 @@ void sph_batch_destroy(sph_batch *b);
  * together one frame at a time, and retires finished scenes with their state copied back.
  * Requests arriving while others run are admitted at the next frame boundary (continuous
  * batching); each result is bit-identical to stepping that scene alone.  Every request
- * must use the server's substep count.  Calls are thread-safe. */
+ * must use the server's substep count.  Calls are thread-safe.  Resident scenes run on the
+ * one-lane-per-particle kernels; SPH_SERVER_BRICKS=1 in the environment selects the brick
+ * kernels instead. */
 typedef struct sph_server sph_server;
 typedef struct sph_server_config {
     uint64_t hbm_budget_bytes;  /* admission budget for resident scenes                   */

@@ -33,6 +33,7 @@ struct sph_batch {
     std::vector<uint8_t> brick_choice;  // per scene: 0 auto, 1 direct, 2 brick
     uint32_t n_total = 0, t_total = 0, b_total = 0, substeps = 1;
     int variant = SPH_KERNEL_AUTO;
+    int tuned_variant = -1;  // AUTO's measured choice; -1 = measure before the next step
     hipStream_t stream = nullptr;
     Chain chain;
     SceneConst *d_scenes = nullptr;
@@ -154,7 +155,7 @@ int sph_batch_create(uint32_t count, const sph_params *params, const uint32_t *c
         delete b;
         return rc;
     }
-    b->chain.bricks = true;  // the queue kernels also carry the lane-per-particle scenes
+    b->chain.bricks = false;  // AUTO starts on the per-lane kernels until tuned
     std::vector<uint16_t> scene_of(b->n_total);
     for (uint32_t s = 0; s < count; ++s)
         std::fill(scene_of.begin() + b->p_begin[s], scene_of.begin() + b->p_begin[s] + b->n[s], (uint16_t)s);
@@ -174,19 +175,73 @@ int sph_batch_create(uint32_t count, const sph_params *params, const uint32_t *c
     return SPH_OK;
 }
 
+// DIRECT: every scene on the per-lane kernels.  BRICK: the queue kernels, with bricks for
+// the scenes whose brick grid exists and whose density (AUTO's per-scene choice, or all of
+// them when `all` is set) makes staging pay; the others ride along as lane chunks.
+static int batch_apply(sph_batch *b, int v, bool all) {
+    for (uint32_t s = 0; s < b->count; ++s) {
+        uint32_t ub = 0u;
+        if (v == SPH_KERNEL_BRICK && b->b_total && scene_bricks(b->scenes[s]))
+            ub = all ? 1u : b->brick_choice[s];
+        b->scenes[s].use_bricks = ub;
+    }
+    b->chain.bricks = v == SPH_KERNEL_BRICK && b->b_total > 0;
+    b->graph.drop();
+    return upload_scene_table(b);
+}
+
+// Times a few substeps of each configuration on a saved copy of the state and keeps the
+// fastest (results are identical either way).
+static int batch_autotune(sph_batch *b, uint32_t substeps) {
+    const size_t bytes = (size_t)b->n_total * 16;
+    float4 *sp = nullptr, *sv = nullptr;
+    hipError_t e = hipMalloc((void **)&sp, bytes);
+    if (e == hipSuccess) e = hipMalloc((void **)&sv, bytes);
+    if (e == hipSuccess) e = hipMemcpyAsync(sp, b->chain.b.pos, bytes, hipMemcpyDeviceToDevice, b->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(sv, b->chain.b.vel, bytes, hipMemcpyDeviceToDevice, b->stream);
+    int best = SPH_KERNEL_DIRECT;
+    float best_ms = 3.0e38f;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&t0);
+    if (e == hipSuccess) e = hipEventCreate(&t1);
+    const int cands[2] = {SPH_KERNEL_DIRECT, SPH_KERNEL_BRICK};
+    for (int v : cands) {
+        if (e != hipSuccess) break;
+        if (v == SPH_KERNEL_BRICK && !b->b_total) continue;
+        int rc = batch_apply(b, v, false);
+        if (rc) { e = hipErrorUnknown; break; }
+        e = hipMemcpyAsync(b->chain.b.pos, sp, bytes, hipMemcpyDeviceToDevice, b->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(b->chain.b.vel, sv, bytes, hipMemcpyDeviceToDevice, b->stream);
+        if (e == hipSuccess) e = b->chain.enqueue(b->stream, b->policy(), nullptr);
+        if (e == hipSuccess) e = hipEventRecord(t0, b->stream);
+        for (uint32_t k = 0; k < substeps && e == hipSuccess; ++k) e = b->chain.enqueue(b->stream, b->policy(), nullptr);
+        if (e == hipSuccess) e = hipEventRecord(t1, b->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(t1);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
+        if (e == hipSuccess && ms < best_ms) { best_ms = ms; best = v; }
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(b->chain.b.pos, sp, bytes, hipMemcpyDeviceToDevice, b->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(b->chain.b.vel, sv, bytes, hipMemcpyDeviceToDevice, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    if (sp) (void)hipFree(sp);
+    if (sv) (void)hipFree(sv);
+    if (e != hipSuccess) return hip_fail(e, "batch autotune");
+    b->tuned_variant = best;
+    return batch_apply(b, best, false);
+}
+
 int sph_batch_set_kernel(sph_batch *b, int variant) {
     if (!b) return fail(SPH_ERR_INVALID_ARG, "batch is NULL");
     if (variant < SPH_KERNEL_AUTO || variant > SPH_KERNEL_BRICK)
         return fail(SPH_ERR_INVALID_ARG, "unknown kernel variant %d", variant);
-    b->variant = variant;
-    for (uint32_t s = 0; s < b->count; ++s) {
-        if (variant == SPH_KERNEL_DIRECT) b->scenes[s].use_bricks = 0u;
-        else if (variant == SPH_KERNEL_BRICK) b->scenes[s].use_bricks = scene_bricks(b->scenes[s]) ? 1u : 0u;
-    }
     hipError_t e = hipStreamSynchronize(b->stream);
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
-    b->graph.drop();
-    return upload_scene_table(b);
+    b->variant = variant;
+    b->tuned_variant = variant == SPH_KERNEL_AUTO ? -1 : variant;
+    return batch_apply(b, variant == SPH_KERNEL_AUTO ? SPH_KERNEL_DIRECT : variant, variant == SPH_KERNEL_BRICK);
 }
 
 int sph_batch_upload_scene(sph_batch *b, uint32_t scene, const float *pos4, const float *vel4) {
@@ -199,14 +254,8 @@ int sph_batch_upload_scene(sph_batch *b, uint32_t scene, const float *pos4, cons
     if (e == hipSuccess) e = hipMemcpyAsync(b->chain.b.vel + off, vel4, bytes, hipMemcpyHostToDevice, b->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     if (e != hipSuccess) return hip_fail(e, "upload");
-    if (b->variant == SPH_KERNEL_AUTO && scene_bricks(b->scenes[scene])) {
-        const uint32_t ub = choose_bricks(b->scenes[scene], b->n[scene], pos4, b->variant);
-        if (ub != b->scenes[scene].use_bricks) {
-            b->scenes[scene].use_bricks = ub;
-            b->graph.drop();
-            if ((rc = upload_scene_table(b))) return rc;
-        }
-    }
+    b->brick_choice[scene] = (uint8_t)choose_bricks(b->scenes[scene], b->n[scene], pos4, SPH_KERNEL_AUTO);
+    if (b->variant == SPH_KERNEL_AUTO) b->tuned_variant = -1;  // re-tune for the new state
     b->stepped = false;
     return SPH_OK;
 }
@@ -215,6 +264,10 @@ int sph_batch_step(sph_batch *b, uint32_t frames) {
     if (!b) return fail(SPH_ERR_INVALID_ARG, "batch is NULL");
     const uint64_t substeps = (uint64_t)frames * b->substeps;
     if (substeps == 0) return SPH_OK;
+    if (b->tuned_variant < 0) {
+        int rc = batch_autotune(b, 2);
+        if (rc) return rc;
+    }
     if (!b->graph.exec) {
         int rc = b->graph.ensure_begin(b->stream);
         if (rc) return rc;
@@ -276,14 +329,16 @@ int sph_batch_debug_dump(sph_batch *b, uint32_t scene, int stage, void *dst, siz
 
 int sph_batch_scene_uses_bricks(sph_batch *b, uint32_t scene) {
     if (!b || scene >= b->count) return fail(SPH_ERR_INVALID_ARG, "bad batch or scene");
-    return (int)b->scenes[scene].use_bricks;
+    return b->chain.bricks && b->scenes[scene].use_bricks ? 1 : 0;
 }
 
 int sph_batch_pass_times(sph_batch *b, uint32_t substeps, sph_pass_times *out) {
     if (!b || !out) return fail(SPH_ERR_INVALID_ARG, "NULL argument");
     if (substeps == 0) return fail(SPH_ERR_INVALID_ARG, "substeps must be >= 1");
+    int rc = SPH_OK;
+    if (b->tuned_variant < 0 && (rc = batch_autotune(b, 2))) return rc;
     PassEvents ev;
-    int rc = ev.create();
+    rc = ev.create();
     if (rc) { ev.destroy(); return rc; }
     double acc[5] = {0, 0, 0, 0, 0};
     for (uint32_t s = 0; s < substeps; ++s) {

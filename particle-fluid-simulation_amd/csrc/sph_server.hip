@@ -13,6 +13,7 @@
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -83,6 +84,7 @@ struct sph_server {
     bool stop = false;
     std::thread worker;
     std::string fatal;  // set when a device error stops the worker
+    bool cfg_bricks = false;  // brick kernels for the resident scenes (else one lane each)
 
     uint64_t resident_bytes = 0, resident_particles = 0, resident_cells = 0, resident_bricks = 0;
     uint64_t submitted = 0, completed = 0, failed = 0, frames = 0;
@@ -100,7 +102,7 @@ bool fits(const sph_server *s, const Request *r) {
            s->resident_bytes + r->footprint <= s->cfg.hbm_budget_bytes &&
            s->resident_particles + r->n <= s->cfg.max_particles &&
            s->resident_cells + r->T <= s->cfg.max_table_cells &&
-           s->resident_bricks + r->bricks <= s->chain.b_cap;
+           (!s->cfg_bricks || s->resident_bricks + r->bricks <= s->chain.b_cap);
 }
 
 // Lays the resident scenes out contiguously, moves continuing scenes' state to their new
@@ -113,7 +115,7 @@ int repack(sph_server *s, const std::vector<uint32_t> &old_begin) {
     for (size_t i = 0; i < count; ++i) {
         Request *r = s->resident[i];
         SceneConst c = make_scene_const(r->params, r->T, tb, pb, pb + r->n, bb);
-        if (r->bricks) bb += (uint32_t)r->bricks;
+        if (s->cfg_bricks && r->bricks) bb += (uint32_t)r->bricks;
         else c.use_bricks = 0u;
         table[i] = c;
         if (!r->fresh && old_begin[i] != pb) moved = true;
@@ -160,7 +162,7 @@ int repack(sph_server *s, const std::vector<uint32_t> &old_begin) {
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) return hip_fail(e, "repack");
     s->chain.configure(pb, tb, bb ? bb : 1u);
-    s->chain.bricks = true;
+    s->chain.bricks = s->cfg_bricks && bb > 0;
     s->graph.drop();
     return SPH_OK;
 }
@@ -284,6 +286,7 @@ int sph_server_create(const sph_server_config *cfg, sph_server **out) {
     if (!s) return fail(SPH_ERR_OUT_OF_MEMORY, "host allocation");
     s->cfg = *cfg;
     if (s->cfg.max_bricks == 0) s->cfg.max_bricks = 1u << 22;
+    s->cfg_bricks = std::getenv("SPH_SERVER_BRICKS") && std::strcmp(std::getenv("SPH_SERVER_BRICKS"), "1") == 0;
     hipError_t e = hipGetDevice(&s->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
@@ -332,7 +335,7 @@ int sph_server_submit(sph_server *s, const sph_params *params, uint32_t n, const
     r->bricks = scene_bricks(c);
     r->footprint = scene_footprint(n, r->T);
     if (r->footprint > s->cfg.hbm_budget_bytes || n > s->cfg.max_particles || r->T > s->cfg.max_table_cells ||
-        r->bricks > s->chain.b_cap)
+        (s->cfg_bricks && r->bricks > s->chain.b_cap))
         return fail(SPH_ERR_CAPACITY, "request exceeds the server's capacity");
     r->pos.assign(pos4, pos4 + (size_t)n * 4);
     r->vel.assign(vel4, vel4 + (size_t)n * 4);

@@ -79,16 +79,23 @@ class SceneSpec:
         return 1 << max(0, (self.n - 1).bit_length())
 
     def state(self):
-        """Uniform fill of the lower half of the box, zero velocity."""
+        """A resting block on the floor of the box at the scene's target density: number
+        density n / volume = rho0 (the density estimate of a uniform fill equals its number
+        density, since the kernel integrates to one), height n / (rho0 * Lx * Ly) capped at
+        the box height; zero velocity.  (A sparse fill far below rho0 is unstable with
+        these kernel parameters: pressure pulls isolated pairs together fast enough to blow
+        the velocities up.)"""
         bx, by, bz = self.box_max
-        return fill_block(self.n, self.seed, (0.0, 0.0, 0.0), (bx, by, bz * 0.5), 0.0)
+        h = min(bz, self.n / (self.target_density * bx * by))
+        return fill_block(self.n, self.seed, (0.0, 0.0, 0.0), (bx, by, float(h)), 0.0)
 
 
 def mixed_scenes(count: int = 256, seed: int = 3, n_min: int = 2048, n_max: int = 65536,
                  extent=(4.0, 16.0), rho0=(40.0, 70.0)) -> list:
     """The mixed-size batch: scene s draws from numpy's PCG64 seeded with seed + s a
     particle count log-uniform in [n_min, n_max], box extents uniform in `extent` per axis
-    and a target density uniform in `rho0`; its particles fill the lower half of its box.
+    and a target density uniform in `rho0`; its particles start as a resting block at the
+    target density on the floor of its box (SceneSpec.state).
     Other parameters are those of dam_break_16k (R 0.35, k 500, g 9.8, 1/120 s x 3)."""
     import numpy as np
     out = []
