@@ -63,7 +63,23 @@ def build(jobs: int = 4, verbose: bool = False) -> str:
     os.replace(tmp, lib_path())
     if verbose:
         print(f"built {lib_path()}")
+    build_driver(verbose)
     return lib_path()
+
+
+def build_driver(verbose: bool = False) -> str:
+    """The headless C driver (driver/sph_run.c) linked against libsph_hip.so."""
+    out = os.path.join(HERE, "bin", "sph_run")
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    cmd = ["gcc", "-O2", "-std=c11", "-ffp-contract=off", "-Wall", f"-I{inc}", os.path.join(HERE, "driver", "sph_run.c"),
+           f"-L{LIBDIR}", "-lsph_hip", f"-Wl,-rpath,$ORIGIN/../lib", "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {out}")
+    return out
 
 
 def clean() -> None:
