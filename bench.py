@@ -238,7 +238,8 @@ def main() -> int:
     world.upload(pos, vel)
     print(f"[bench] {W.name}: N={W.n} T={T} substeps/step={W.substeps}", file=sys.stderr, flush=True)
     chosen, tuned_ms = world.autotune(5)
-    print(f"[bench] autotune: {chosen} kernels, {tuned_ms:.3f} ms/substep", file=sys.stderr, flush=True)
+    chosen = world.kernel_config
+    print(f"[bench] autotune: {chosen}, {tuned_ms:.3f} ms/substep", file=sys.stderr, flush=True)
 
     world.step(args.warmup)
     world.sync()
@@ -255,7 +256,7 @@ def main() -> int:
     # Dominant kernel, timed with HIP events on the library's stream.
     pt = world.pass_times(30)
     variants = {}
-    for v in ("direct", "brick"):
+    for v in ("brick", "direct"):  # direct last: keeps the tuned per-lane configuration
         world.set_kernel(v)
         variants[v] = world.pass_times(10)
     force_s = pt["force_ms"] * 1e-3

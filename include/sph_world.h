@@ -113,8 +113,9 @@ typedef enum sph_dump_stage {
  *           6x6x6 halo's runs from LDS.
  *   AUTO:   the autotuner's choice (the default): before the first step of a size bucket
  *           (particle count, table size, brick count rounded to powers of two) each
- *           variant is timed on a saved copy of the world's state, and the fastest is
- *           cached for the process.  SPH_AUTOTUNE=0 in the environment disables the
+ *           configuration (the brick kernels, and the per-lane kernels with 1, 2 or 4
+ *           candidates per scan step and 64-, 128- or 256-lane workgroups) is timed on a
+ *           saved copy of the world's state, and the fastest is cached for the process.  SPH_AUTOTUNE=0 in the environment disables the
  *           measurement (DIRECT is used). */
 typedef enum sph_kernel_variant {
     SPH_KERNEL_AUTO = 0,
@@ -127,6 +128,9 @@ int sph_world_set_kernel(sph_world *w, int variant);
 int sph_world_autotune(sph_world *w, uint32_t substeps, int *chosen, double *ms_per_substep);
 /* The variant the next step will use (DIRECT or BRICK), or < 0 on error. */
 int sph_world_kernel(const sph_world *w);
+/* Human-readable launch configuration the next step will use (the autotuner also chooses
+ * the per-lane kernels' candidates per scan step and workgroup size). */
+int sph_world_kernel_config(const sph_world *w, char *buf, size_t bytes);
 
 int sph_world_set_debug(sph_world *w, int enable);
 int sph_world_debug_dump(sph_world *w, int stage, void *dst, size_t bytes);

@@ -199,16 +199,20 @@ static int batch_autotune(sph_batch *b, uint32_t substeps) {
     if (e == hipSuccess) e = hipMalloc((void **)&sv, bytes);
     if (e == hipSuccess) e = hipMemcpyAsync(sp, b->chain.b.pos, bytes, hipMemcpyDeviceToDevice, b->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(sv, b->chain.b.vel, bytes, hipMemcpyDeviceToDevice, b->stream);
-    int best = SPH_KERNEL_DIRECT;
+    // Candidates: the per-lane kernels in each launch configuration (code = config index),
+    // then the brick mix (code = -1).
+    int best = kDefaultLaneConfig;
     float best_ms = 3.0e38f;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     if (e == hipSuccess) e = hipEventCreate(&t0);
     if (e == hipSuccess) e = hipEventCreate(&t1);
-    const int cands[2] = {SPH_KERNEL_DIRECT, SPH_KERNEL_BRICK};
+    std::vector<int> cands;
+    for (int c = 0; c < kNumLaneConfigs; ++c) cands.push_back(c);
+    if (b->b_total) cands.push_back(-1);
     for (int v : cands) {
         if (e != hipSuccess) break;
-        if (v == SPH_KERNEL_BRICK && !b->b_total) continue;
-        int rc = batch_apply(b, v, false);
+        if (v >= 0) b->chain.lane_cfg = v;
+        int rc = batch_apply(b, v < 0 ? SPH_KERNEL_BRICK : SPH_KERNEL_DIRECT, false);
         if (rc) { e = hipErrorUnknown; break; }
         e = hipMemcpyAsync(b->chain.b.pos, sp, bytes, hipMemcpyDeviceToDevice, b->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(b->chain.b.vel, sv, bytes, hipMemcpyDeviceToDevice, b->stream);
@@ -229,8 +233,9 @@ static int batch_autotune(sph_batch *b, uint32_t substeps) {
     if (sp) (void)hipFree(sp);
     if (sv) (void)hipFree(sv);
     if (e != hipSuccess) return hip_fail(e, "batch autotune");
-    b->tuned_variant = best;
-    return batch_apply(b, best, false);
+    b->tuned_variant = best < 0 ? SPH_KERNEL_BRICK : SPH_KERNEL_DIRECT;
+    if (best >= 0) b->chain.lane_cfg = best;
+    return batch_apply(b, b->tuned_variant, false);
 }
 
 int sph_batch_set_kernel(sph_batch *b, int variant) {

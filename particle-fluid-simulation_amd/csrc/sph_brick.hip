@@ -208,7 +208,7 @@ __global__ __launch_bounds__(kBlock) void k_density_brick(
                                                 sample_axis(s.x, c.minx, c.maxx, c.R),
                                                 sample_axis(s.y, c.miny, c.maxy, c.R),
                                                 sample_axis(s.z, c.minz, c.maxz, c.R)};
-                dens[j] = density_walk<false>(c, s, pred_pos, nullptr, cells);
+                dens[j] = density_walk<false, 2>(c, s, pred_pos, nullptr, cells);
             }
             continue;
         }
@@ -238,7 +238,7 @@ __global__ __launch_bounds__(kBlock) void k_density_brick(
                                                sample_axis(s.x, c.minx, c.maxx, c.R),
                                                sample_axis(s.y, c.miny, c.maxy, c.R),
                                                sample_axis(s.z, c.minz, c.maxz, c.R)};
-                dens[j] = density_walk<true>(c, s, pred_pos, s_pos, cells);
+                dens[j] = density_walk<true, 2>(c, s, pred_pos, s_pos, cells);
             }
             __syncthreads();
         }
@@ -266,7 +266,7 @@ __device__ __forceinline__ void force_of(const SceneConst &c, uint32_t j,
     const ForceCells<kHaloLds> cells{CellRuns<kHaloLds>{c.table_base, c.mask, table, H, bc.ox, bc.oy, bc.oz},
                                      force_cell(c, s)};
     float3 Fp = make_float3(0.0f, 0.0f, 0.0f), Fv = make_float3(0.0f, 0.0f, 0.0f);
-    force_walk<kHaloLds>(c, me, pred_pos, vel_sorted, dens, s_a, s_b, cells, q, kBlock, Fp, Fv);
+    force_walk<kHaloLds, 2>(c, me, pred_pos, vel_sorted, dens, s_a, s_b, cells, q, kBlock, Fp, Fv);
     float4 po, vo, vf;
     force_finish(c, P, V, myd.x, Fp, Fv, po, vo, vf);
     if (dbg_vel_force) dbg_vel_force[j] = vf;

@@ -35,6 +35,7 @@ struct Chain {
     uint32_t n = 0, table_total = 0, key_bits = 0, passes = 0, tiles = 0, lookback_words = 0;
     uint32_t brick_total = 0;  // bricks over all scenes; 0 disables the brick kernels
     bool bricks = false;       // density/force through the brick kernels (else per lane)
+    int lane_cfg = kDefaultLaneConfig;  // launch configuration of the per-lane kernels
     uint32_t n_cap = 0, t_cap = 0, b_cap = 0;  // allocated capacity
 
     // Allocates for the given capacity and configures the chain to use all of it.

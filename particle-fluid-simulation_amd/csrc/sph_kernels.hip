@@ -130,11 +130,12 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
 
 // ---------------------------------------------------------------------------------------
 // Density: calculateDensities.comp:96-155, one lane per particle, runs read from global.
-template <class Scenes>
-__global__ __launch_bounds__(kBlock) void k_density(
+// kU: candidates loaded per scan iteration; kB: workgroup size (tuned per size bucket).
+template <class Scenes, int kU, int kB>
+__global__ __launch_bounds__(kB) void k_density(
     Scenes sc, const float4 *__restrict__ pred_pos, const int2 *__restrict__ table,
     float2 *__restrict__ dens, uint32_t n) {
-    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t j = blockIdx.x * kB + threadIdx.x;
     if (j >= n) return;
     const SceneConst c = sc.scene(j);
     const float4 s = pred_pos[j];
@@ -142,21 +143,21 @@ __global__ __launch_bounds__(kBlock) void k_density(
                                     sample_axis(s.x, c.minx, c.maxx, c.R),
                                     sample_axis(s.y, c.miny, c.maxy, c.R),
                                     sample_axis(s.z, c.minz, c.maxz, c.R)};
-    dens[j] = density_walk<false>(c, s, pred_pos, nullptr, cells);
+    dens[j] = density_walk<false, kU>(c, s, pred_pos, nullptr, cells);
 }
 
 // ---------------------------------------------------------------------------------------
 // Force + integrate: applyPressureAndViscosityForce.comp:275-337 followed by
 // updateVelocities.comp:71-119.  Neighbour velocities come from vel_sorted, which no lane
 // writes (the reference races on that buffer; here every read sees the pre-force value).
-template <class Scenes>
-__global__ __launch_bounds__(kBlock) void k_force(
+template <class Scenes, int kU, int kB>
+__global__ __launch_bounds__(kB) void k_force(
     Scenes sc, const float4 *__restrict__ pred_pos, const float4 *__restrict__ pos_sorted,
     const float4 *__restrict__ vel_sorted, const float2 *__restrict__ dens,
     const int2 *__restrict__ table, float4 *__restrict__ pos_out, float4 *__restrict__ vel_out,
     float4 *__restrict__ dbg_vel_force, uint32_t n) {
-    __shared__ uint32_t s_queue[kQueue * kBlock];
-    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    __shared__ uint32_t s_queue[kQueue * kB];
+    const uint32_t j = blockIdx.x * kB + threadIdx.x;
     if (j >= n) return;
     const SceneConst c = sc.scene(j);
     const float4 s = pred_pos[j];
@@ -166,8 +167,8 @@ __global__ __launch_bounds__(kBlock) void k_force(
     const ForceSelf me = force_self(c, j, s, P, V, myd);
     const ForceCells<false> cells{CellRuns<false>{c.table_base, c.mask, table, nullptr, 0, 0, 0}, force_cell(c, s)};
     float3 Fp = make_float3(0.0f, 0.0f, 0.0f), Fv = make_float3(0.0f, 0.0f, 0.0f);
-    force_walk<false>(c, me, pred_pos, vel_sorted, dens, nullptr, nullptr, cells, s_queue + threadIdx.x,
-               kBlock, Fp, Fv);
+    force_walk<false, kU>(c, me, pred_pos, vel_sorted, dens, nullptr, nullptr, cells,
+                          s_queue + threadIdx.x, kB, Fp, Fv);
     float4 po, vo, vf;
     force_finish(c, P, V, myd.x, Fp, Fv, po, vo, vf);
     if (dbg_vel_force) dbg_vel_force[j] = vf;
@@ -206,19 +207,48 @@ hipError_t launch_reorder(hipStream_t stream, const Scenes &sc, const ChainBuffe
     return hipGetLastError();
 }
 
-template <class Scenes>
-hipError_t launch_density(hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n) {
-    hipLaunchKernelGGL(k_density<Scenes>, dim3(blocks_for(n)), dim3(kBlock), 0, stream, sc,
+const LaneConfig kLaneConfigs[kNumLaneConfigs] = {
+    {1, 256}, {2, 256}, {4, 256}, {2, 128}, {4, 128}, {2, 64},
+};
+
+template <class Scenes, int kU, int kB>
+hipError_t launch_density_t(hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n) {
+    hipLaunchKernelGGL((k_density<Scenes, kU, kB>), dim3((n + kB - 1u) / kB), dim3(kB), 0, stream, sc,
                        b.pred_pos, b.table, b.dens, n);
     return hipGetLastError();
 }
 
-template <class Scenes>
-hipError_t launch_force(hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n) {
-    hipLaunchKernelGGL(k_force<Scenes>, dim3(blocks_for(n)), dim3(kBlock), 0, stream, sc,
+template <class Scenes, int kU, int kB>
+hipError_t launch_force_t(hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n) {
+    hipLaunchKernelGGL((k_force<Scenes, kU, kB>), dim3((n + kB - 1u) / kB), dim3(kB), 0, stream, sc,
                        b.pred_pos, b.pos_sorted, b.vel_sorted, b.dens, b.table, b.pos, b.vel,
                        b.dbg_vel_force, n);
     return hipGetLastError();
+}
+
+template <class Scenes>
+hipError_t launch_density(hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n,
+                          int cfg) {
+    switch (cfg) {
+        case 0: return launch_density_t<Scenes, 1, 256>(stream, sc, b, n);
+        case 2: return launch_density_t<Scenes, 4, 256>(stream, sc, b, n);
+        case 3: return launch_density_t<Scenes, 2, 128>(stream, sc, b, n);
+        case 4: return launch_density_t<Scenes, 4, 128>(stream, sc, b, n);
+        case 5: return launch_density_t<Scenes, 2, 64>(stream, sc, b, n);
+        default: return launch_density_t<Scenes, 2, 256>(stream, sc, b, n);
+    }
+}
+
+template <class Scenes>
+hipError_t launch_force(hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n, int cfg) {
+    switch (cfg) {
+        case 0: return launch_force_t<Scenes, 1, 256>(stream, sc, b, n);
+        case 2: return launch_force_t<Scenes, 4, 256>(stream, sc, b, n);
+        case 3: return launch_force_t<Scenes, 2, 128>(stream, sc, b, n);
+        case 4: return launch_force_t<Scenes, 4, 128>(stream, sc, b, n);
+        case 5: return launch_force_t<Scenes, 2, 64>(stream, sc, b, n);
+        default: return launch_force_t<Scenes, 2, 256>(stream, sc, b, n);
+    }
 }
 
 #define SPH_INSTANTIATE(S)                                                                       \
@@ -226,8 +256,8 @@ hipError_t launch_force(hipStream_t stream, const Scenes &sc, const ChainBuffers
                                        uint32_t, uint32_t, uint32_t);                            \
     template hipError_t launch_reorder<S>(hipStream_t, const S &, const ChainBuffers &,          \
                                           const uint32_t *, const uint32_t *, uint32_t, bool);   \
-    template hipError_t launch_density<S>(hipStream_t, const S &, const ChainBuffers &, uint32_t); \
-    template hipError_t launch_force<S>(hipStream_t, const S &, const ChainBuffers &, uint32_t);
+    template hipError_t launch_density<S>(hipStream_t, const S &, const ChainBuffers &, uint32_t, int); \
+    template hipError_t launch_force<S>(hipStream_t, const S &, const ChainBuffers &, uint32_t, int);
 
 SPH_INSTANTIATE(OneScene)
 SPH_INSTANTIATE(ManyScenes)

@@ -56,14 +56,24 @@ hipError_t launch_reorder(hipStream_t stream, const Scenes &sc, const ChainBuffe
                           const uint32_t *sorted_keys, const uint32_t *sorted_idx, uint32_t n,
                           bool bricks);
 
+// Launch configurations of the one-lane-per-particle density / force kernels: candidates
+// loaded per scan iteration and workgroup size.  The autotuner picks one per size bucket.
+struct LaneConfig {
+    int unroll;
+    int block;
+};
+constexpr int kNumLaneConfigs = 6;
+constexpr int kDefaultLaneConfig = 1;
+extern const LaneConfig kLaneConfigs[kNumLaneConfigs];
+
 template <class Scenes>
 hipError_t launch_density(hipStream_t stream, const Scenes &sc, const ChainBuffers &b,
-                          uint32_t n);
+                          uint32_t n, int cfg);
 
 // Pressure + viscosity force, then integrate; writes the new state into b.pos / b.vel.
 template <class Scenes>
 hipError_t launch_force(hipStream_t stream, const Scenes &sc, const ChainBuffers &b,
-                        uint32_t n);
+                        uint32_t n, int cfg);
 
 // Brick path (sph_brick.hip): compacts the non-empty bricks into the work queue, then
 // persistent workgroups drain it for the density and the force + integrate passes.

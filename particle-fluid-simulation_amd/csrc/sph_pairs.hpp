@@ -114,10 +114,29 @@ struct ForceCells {
     }
 };
 
+// One density candidate at distance^2 d2 (calculateDensities.comp:123-127).  d2 > R2_pre
+// implies sqrt(d2) >= R: both kernels are 0 and M + m*0 == M for M >= +0, so such a
+// candidate changes nothing and its sqrt is skipped.
+__device__ __forceinline__ void density_accumulate(const SceneConst &c, float d2, float &M, float &Mn) {
+    if (d2 > c.R2_pre) return;
+    const float dist = sqrtf(d2);
+    if (dist >= c.R) return;
+    const float t = c.R - dist;
+    M = M + c.m * (t * t);
+    Mn = Mn + c.m * (t * t * t);
+}
+
+__device__ __forceinline__ float dist2(const float4 &p, const float4 &s) {
+    const float ex = p.x - s.x, ey = p.y - s.y, ez = p.z - s.z;
+    return ex * ex + ey * ey + ez * ez;
+}
+
 // calculateDensities.comp:114-134 for sample point s.  With kLds false every run is read
-// from global memory and s_pos is unused.  The density's in-range arithmetic is short, so
-// it is done in the scan itself (no queue).
-template <bool kLds, class Cells>
+// from global memory and s_pos is unused.  kU candidates of a run are loaded before any
+// is used (more loads in flight, fewer loop branches); they are still accumulated one by
+// one in run order.  The density's in-range arithmetic is short, so it is done in the
+// scan itself (no queue).
+template <bool kLds, int kU, class Cells>
 __device__ __forceinline__ float2 density_walk(const SceneConst &c, const float4 &s,
                                                const float4 *__restrict__ pred_pos,
                                                const float4 *s_pos, const Cells &cells) {
@@ -126,20 +145,24 @@ __device__ __forceinline__ float2 density_walk(const SceneConst &c, const float4
         for (int dy = 0; dy < 3; ++dy)
             for (int dx = 0; dx < 3; ++dx) {
                 const RunRef r = cells(dz, dy, dx);
-                const bool global = !kLds || r.global;
-                for (int k = 0; k < r.cnt; ++k) {
-                    const int idx = r.base + k;
-                    const float4 p = global ? pred_pos[idx] : s_pos[idx];
-                    const float ex = p.x - s.x, ey = p.y - s.y, ez = p.z - s.z;
-                    const float d2 = ex * ex + ey * ey + ez * ez;
-                    // d2 > R2_pre implies sqrt(d2) >= R: both kernels are 0 and
-                    // M + m*0 == M for M >= +0, so the candidate changes nothing.
-                    if (d2 > c.R2_pre) continue;
-                    const float dist = sqrtf(d2);
-                    if (dist >= c.R) continue;
-                    const float t = c.R - dist;
-                    M = M + c.m * (t * t);
-                    Mn = Mn + c.m * (t * t * t);
+                // Separate loops per source keep global and LDS loads in their own
+                // address spaces (no flat loads).
+                auto scan = [&](const float4 *src) {
+                    int k = 0;
+                    for (; k + kU <= r.cnt; k += kU) {
+                        float4 p[kU];
+#pragma unroll
+                        for (int u = 0; u < kU; ++u) p[u] = src[k + u];
+#pragma unroll
+                        for (int u = 0; u < kU; ++u) density_accumulate(c, dist2(p[u], s), M, Mn);
+                    }
+                    for (; k < r.cnt; ++k) density_accumulate(c, dist2(src[k], s), M, Mn);
+                };
+                if constexpr (!kLds) {
+                    scan(pred_pos + r.base);
+                } else {
+                    if (r.global) scan(pred_pos + r.base);
+                    else scan(s_pos + r.base);
                 }
             }
     return density_finish(c, M, Mn);
@@ -193,8 +216,9 @@ __device__ __forceinline__ void force_term(const SceneConst &c, const ForceSelf 
     Fp.z = Fp.z + cn * dirz;
 }
 
-// applyPressureAndViscosityForce.comp:288-325: accumulates Fp and Fv for `me`.
-template <bool kLds, class Cells>
+// applyPressureAndViscosityForce.comp:288-325: accumulates Fp and Fv for `me`.  The scan
+// loads kU candidates of a run before testing them in run order.
+template <bool kLds, int kU, class Cells>
 __device__ __forceinline__ void force_walk(const SceneConst &c, const ForceSelf &me,
                                            const float4 *__restrict__ pred_pos,
                                            const float4 *__restrict__ vel_sorted,
@@ -203,33 +227,43 @@ __device__ __forceinline__ void force_walk(const SceneConst &c, const ForceSelf 
                                            const Cells &cells, uint32_t *q, int stride,
                                            float3 &Fp, float3 &Fv) {
     int qc = 0;
+    auto drain = [&]() {
+        for (int i = 0; i < qc; ++i)
+            force_term<kLds>(c, me, q[i * stride], pred_pos, vel_sorted, dens, s_a, s_b, Fp, Fv);
+        qc = 0;
+    };
+    auto consider = [&](const float4 &p, int k, const RunRef &r, bool global) {
+        if ((uint32_t)(r.first + k) == me.j) return;
+        if (dist2(p, me.s) > c.R2) return;
+        const uint32_t idx = (uint32_t)(r.base + k);
+        q[qc * stride] = global ? (kGlobalBit | idx) : idx;
+        if (++qc == kQueue) drain();
+    };
     for (int dz = 0; dz < 3; ++dz)
         for (int dy = 0; dy < 3; ++dy)
             for (int dx = 0; dx < 3; ++dx) {
                 const RunRef r = cells(dz, dy, dx);
                 const bool global = !kLds || r.global;
-                for (int k = 0; k < r.cnt; ++k) {
-                    if ((uint32_t)(r.first + k) == me.j) continue;
-                    const int idx = r.base + k;
-                    const float4 p = global ? pred_pos[idx] : s_a[idx];
-                    const float ex = p.x - me.s.x, ey = p.y - me.s.y, ez = p.z - me.s.z;
-                    const float d2 = ex * ex + ey * ey + ez * ez;
-                    if (d2 > c.R2) continue;
-                    q[qc * stride] = global ? (kGlobalBit | (uint32_t)idx) : (uint32_t)idx;
-                    if (++qc == kQueue) {
-                        for (int i = 0; i < qc; ++i)
-                            force_term<kLds>(c, me, q[i * stride], pred_pos, vel_sorted, dens, s_a, s_b, Fp, Fv);
-                        qc = 0;
+                auto scan = [&](const float4 *src) {
+                    int k = 0;
+                    for (; k + kU <= r.cnt; k += kU) {
+                        float4 p[kU];
+#pragma unroll
+                        for (int u = 0; u < kU; ++u) p[u] = src[k + u];
+#pragma unroll
+                        for (int u = 0; u < kU; ++u) consider(p[u], k + u, r, global);
                     }
+                    for (; k < r.cnt; ++k) consider(src[k], k, r, global);
+                };
+                if constexpr (!kLds) {
+                    scan(pred_pos + r.base);
+                } else {
+                    if (global) scan(pred_pos + r.base);
+                    else scan(s_a + r.base);
                 }
-                if (__ballot(qc >= kFlushAt)) {
-                    for (int i = 0; i < qc; ++i)
-                        force_term<kLds>(c, me, q[i * stride], pred_pos, vel_sorted, dens, s_a, s_b, Fp, Fv);
-                    qc = 0;
-                }
+                if (__ballot(qc >= kFlushAt)) drain();
             }
-    for (int i = 0; i < qc; ++i)
-        force_term<kLds>(c, me, q[i * stride], pred_pos, vel_sorted, dens, s_a, s_b, Fp, Fv);
+    drain();
 }
 
 // Self terms of the force pass for slot j (applyPressureAndViscosityForce.comp:276-286).

@@ -228,10 +228,10 @@ hipError_t Chain::enqueue(hipStream_t s, const Scenes &sc, PassEvents *ev) {
     if ((e = launch_reorder(s, sc, b, sorted_keys(), sorted_idx(), n, bricks)) != hipSuccess) return e;
     if (bricks && (e = launch_queue(s, b, brick_total)) != hipSuccess) return e;
     if (ev) hipEventRecord(ev->ev[3], s);
-    e = bricks ? launch_density_brick(s, sc, b, n, brick_total) : launch_density(s, sc, b, n);
+    e = bricks ? launch_density_brick(s, sc, b, n, brick_total) : launch_density(s, sc, b, n, lane_cfg);
     if (e != hipSuccess) return e;
     if (ev) hipEventRecord(ev->ev[4], s);
-    e = bricks ? launch_force_brick(s, sc, b, n, brick_total) : launch_force(s, sc, b, n);
+    e = bricks ? launch_force_brick(s, sc, b, n, brick_total) : launch_force(s, sc, b, n, lane_cfg);
     if (e != hipSuccess) return e;
     if (ev) hipEventRecord(ev->ev[5], s);
     return hipSuccess;
@@ -329,13 +329,18 @@ uint64_t tune_bucket(uint32_t n, uint32_t T, uint64_t bricks) {
            (uint64_t)key_bits_for(bricks ? bricks : 1);
 }
 
-int world_apply_variant(sph_world *w, int v) {
-    w->chain.bricks = v == SPH_KERNEL_BRICK && w->chain.brick_total > 0;
+// A tuning code names one configuration: 0..kNumLaneConfigs-1 = the per-lane kernels with
+// that launch configuration, kCodeBrick = the brick kernels.
+constexpr int kCodeBrick = 100;
+
+int world_apply_code(sph_world *w, int code) {
+    w->chain.bricks = code == kCodeBrick && w->chain.brick_total > 0;
+    if (code >= 0 && code < kNumLaneConfigs) w->chain.lane_cfg = code;
     w->graph.drop();
     return SPH_OK;
 }
 
-// Times `substeps` eager substeps of each variant; restores the state afterwards.
+// Times `substeps` eager substeps of each configuration; restores the state afterwards.
 int world_autotune(sph_world *w, uint32_t substeps, int *chosen, double *ms_out) {
     const size_t bytes = (size_t)w->n * 16;
     float4 *save_p = nullptr, *save_v = nullptr;
@@ -348,15 +353,16 @@ int world_autotune(sph_world *w, uint32_t substeps, int *chosen, double *ms_out)
         if (save_v) (void)hipFree(save_v);
         return hip_fail(e, "autotune state copy");
     }
-    const int cands[2] = {SPH_KERNEL_DIRECT, SPH_KERNEL_BRICK};
-    int best = SPH_KERNEL_DIRECT;
+    std::vector<int> cands;
+    for (int c = 0; c < kNumLaneConfigs; ++c) cands.push_back(c);
+    if (w->chain.brick_total) cands.push_back(kCodeBrick);
+    int best = kDefaultLaneConfig;
     double best_ms = 1e30;
     hipEvent_t t0 = nullptr, t1 = nullptr;
     (void)hipEventCreate(&t0);
     (void)hipEventCreate(&t1);
-    for (int v : cands) {
-        if (v == SPH_KERNEL_BRICK && w->chain.brick_total == 0) continue;
-        world_apply_variant(w, v);
+    for (int code : cands) {
+        world_apply_code(w, code);
         // one untimed substep, then the timed ones, each from the saved state
         e = hipMemcpyAsync(w->chain.b.pos, save_p, bytes, hipMemcpyDeviceToDevice, w->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(w->chain.b.vel, save_v, bytes, hipMemcpyDeviceToDevice, w->stream);
@@ -370,7 +376,7 @@ int world_autotune(sph_world *w, uint32_t substeps, int *chosen, double *ms_out)
         (void)hipEventElapsedTime(&ms, t0, t1);
         if (ms < best_ms) {
             best_ms = ms;
-            best = v;
+            best = code;
         }
     }
     if (e == hipSuccess) e = hipMemcpyAsync(w->chain.b.pos, save_p, bytes, hipMemcpyDeviceToDevice, w->stream);
@@ -381,7 +387,7 @@ int world_autotune(sph_world *w, uint32_t substeps, int *chosen, double *ms_out)
     (void)hipFree(save_p);
     (void)hipFree(save_v);
     if (e != hipSuccess) return hip_fail(e, "autotune");
-    world_apply_variant(w, best);
+    world_apply_code(w, best);
     *chosen = best;
     if (ms_out) *ms_out = best_ms / substeps;
     return SPH_OK;
@@ -391,24 +397,24 @@ int world_autotune(sph_world *w, uint32_t substeps, int *chosen, double *ms_out)
 int world_resolve_variant(sph_world *w) {
     if (w->tuned || w->variant != SPH_KERNEL_AUTO) return SPH_OK;
     const uint64_t key = tune_bucket(w->n, w->T, w->chain.brick_total);
-    int v = -1;
+    int code = -1;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune_cache.find(key);
-        if (it != g_tune_cache.end()) v = it->second;
+        if (it != g_tune_cache.end()) code = it->second;
     }
-    if (v < 0) {
+    if (code < 0) {
         const char *off = std::getenv("SPH_AUTOTUNE");
         if (off && std::strcmp(off, "0") == 0) {
-            v = SPH_KERNEL_DIRECT;
+            code = kDefaultLaneConfig;
         } else {
-            int rc = world_autotune(w, 3, &v, nullptr);
+            int rc = world_autotune(w, 3, &code, nullptr);
             if (rc) return rc;
         }
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        g_tune_cache[key] = v;
+        g_tune_cache[key] = code;
     }
-    world_apply_variant(w, v);
+    world_apply_code(w, code);
     w->tuned = true;
     return SPH_OK;
 }
@@ -458,7 +464,7 @@ int sph_world_create(const sph_params *params, uint32_t n, sph_world **out) {
         delete w;
         return rc;
     }
-    world_apply_variant(w, SPH_KERNEL_DIRECT);
+    world_apply_code(w, kDefaultLaneConfig);
     *out = w;
     return SPH_OK;
 }
@@ -547,7 +553,7 @@ int sph_world_set_kernel(sph_world *w, int variant) {
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
     w->variant = variant;
     w->tuned = variant != SPH_KERNEL_AUTO;
-    world_apply_variant(w, variant == SPH_KERNEL_AUTO ? SPH_KERNEL_DIRECT : variant);
+    world_apply_code(w, variant == SPH_KERNEL_BRICK ? kCodeBrick : w->chain.lane_cfg);
     return SPH_OK;
 }
 
@@ -556,18 +562,30 @@ int sph_world_autotune(sph_world *w, uint32_t substeps, int *chosen, double *ms_
     if (substeps == 0) return fail(SPH_ERR_INVALID_ARG, "substeps must be >= 1");
     hipError_t e = hipStreamSynchronize(w->stream);
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
-    int rc = world_autotune(w, substeps, chosen, ms_per_substep);
+    int code = 0;
+    int rc = world_autotune(w, substeps, &code, ms_per_substep);
     if (rc) return rc;
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune_cache[tune_bucket(w->n, w->T, w->chain.brick_total)] = *chosen;
+    g_tune_cache[tune_bucket(w->n, w->T, w->chain.brick_total)] = code;
     w->variant = SPH_KERNEL_AUTO;
     w->tuned = true;
+    *chosen = code == kCodeBrick ? SPH_KERNEL_BRICK : SPH_KERNEL_DIRECT;
     return SPH_OK;
 }
 
 int sph_world_kernel(const sph_world *w) {
     if (!w) return fail(SPH_ERR_INVALID_ARG, "world is NULL");
     return w->chain.bricks ? SPH_KERNEL_BRICK : SPH_KERNEL_DIRECT;
+}
+
+int sph_world_kernel_config(const sph_world *w, char *buf, size_t bytes) {
+    if (!w || !buf || bytes == 0) return fail(SPH_ERR_INVALID_ARG, "NULL argument");
+    if (w->chain.bricks)
+        snprintf(buf, bytes, "brick %dx%dx%d cells, persistent queue", kBrick, kBrick, kBrick);
+    else
+        snprintf(buf, bytes, "per-lane, %d candidates per scan step, %d-lane workgroups",
+                 kLaneConfigs[w->chain.lane_cfg].unroll, kLaneConfigs[w->chain.lane_cfg].block);
+    return SPH_OK;
 }
 
 int sph_world_set_debug(sph_world *w, int enable) {

@@ -79,6 +79,12 @@ class World:
         check(min(r, 0))
         return {1: "direct", 2: "brick"}[r]
 
+    @property
+    def kernel_config(self) -> str:
+        buf = C.create_string_buffer(256)
+        check(self._lib.sph_world_kernel_config(self._h, buf, 256))
+        return buf.value.decode()
+
     def set_debug(self, on: bool) -> None:
         check(self._lib.sph_world_set_debug(self._h, 1 if on else 0))
 
