@@ -183,9 +183,10 @@ void sph_batch_destroy(sph_batch *b);
  * Serving: a request loop over the batched step.  No reference counterpart (the reference
  * runs one interactive scene).  Requests carry their parameters, initial state and step
  * count; a worker thread admits them in arrival order while the sum of resident footprints
- * (128 B per particle + 8 B per table cell) stays within hbm_budget_bytes and the
- * capacities hold, keeps their state resident in device memory, steps every resident scene
- * together one frame at a time, and retires finished scenes with their state copied back.
+ * (136 B per particle + 8 B per table cell and per halo-grid cell) stays within
+ * hbm_budget_bytes and the capacities hold, keeps their state resident in device memory,
+ * steps every resident scene together one frame at a time, and retires finished scenes
+ * with their state copied back.
  * Requests arriving while others run are admitted at the next frame boundary (continuous
  * batching); each result is bit-identical to stepping that scene alone.  Every request
  * must use the server's substep count.  Calls are thread-safe.  Resident scenes run on the

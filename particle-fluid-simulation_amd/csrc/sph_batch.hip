@@ -124,7 +124,7 @@ int sph_batch_create(uint32_t count, const sph_params *params, const uint32_t *c
     b->count = count;
     b->params.assign(params, params + count);
     b->substeps = params[0].substeps;
-    uint32_t pb = 0, tb = 0, bb = 0;
+    uint32_t pb = 0, tb = 0, bb = 0, ub = 0, gb = 0;
     for (uint32_t s = 0; s < count; ++s) {
         const uint32_t T = params[s].table_cells ? params[s].table_cells : counts[s];
         b->params[s].table_cells = T;
@@ -132,7 +132,10 @@ int sph_batch_create(uint32_t count, const sph_params *params, const uint32_t *c
         b->T.push_back(T);
         b->p_begin.push_back(pb);
         b->t_begin.push_back(tb);
-        SceneConst c = make_scene_const(b->params[s], T, tb, pb, pb + counts[s], bb);
+        SceneConst c = make_scene_const(b->params[s], T, tb, pb, pb + counts[s], bb, ub, gb);
+        ub += scene_buckets(c);  // <= 2 n_s + 1024 each: below 2^31 + 2^26 in total
+        if ((uint64_t)gb + c.grid_cells >= (1ull << 31)) c.grid_cells = 0u;  // hash lookups only
+        gb += c.grid_cells;
         const uint64_t nb = scene_bricks(c);
         if (nb == 0 || b_total + nb >= (1ull << 31)) c.use_bricks = 0u;  // stays on lanes
         else { bb += (uint32_t)nb; b_total += nb; }
@@ -149,7 +152,7 @@ int sph_batch_create(uint32_t count, const sph_params *params, const uint32_t *c
         delete b;
         return hip_fail(e, "stream creation");
     }
-    int rc = b->chain.allocate(b->n_total, b->t_total, b->b_total ? b->b_total : 1u);
+    int rc = b->chain.allocate(b->n_total, b->t_total, b->b_total ? b->b_total : 1u, ub, gb);
     if (rc) {
         hipStreamDestroy(b->stream);
         delete b;

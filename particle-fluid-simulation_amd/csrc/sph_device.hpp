@@ -34,6 +34,16 @@ struct SceneConst {
     uint32_t nbx, nby, nbz;  // bricks per axis (brick = kBrick^3 cells)
     uint32_t brick_base;     // first global brick id of this scene
     uint32_t use_bricks;     // 1: brick kernels; 0: one lane per particle (sparse scenes)
+    // Buckets of the cell-ordered layout: cells grouped 2^bshift per axis, numbered
+    // x-fastest; this scene owns buckets [bucket_base, bucket_base + bgx*bgy*bgz).
+    uint32_t bucket_base;
+    uint32_t bshift;
+    uint32_t bgx, bgy;
+    // Cell-ordered run of every cell of the grid grown by one halo cell per side,
+    // (gx+2) x (gy+2) x (gz+2) entries from grid_base; grid_cells = 0: no such grid (the
+    // box is large for the particle count) and every lookup goes through the hash table.
+    uint32_t grid_base;
+    uint32_t grid_cells;
 };
 
 // cellCoordToHash (updateCellEntries.comp:36-52): u32 arithmetic, wraps mod 2^32.
@@ -50,6 +60,15 @@ __device__ __forceinline__ uint32_t cell_hash(uint32_t x, uint32_t y, uint32_t z
 // floor((v - lo) / R) as a cell coordinate; true division, as the reference divides.
 __device__ __forceinline__ int32_t cell_of(float v, float lo, float R) {
     return (int32_t)floorf((v - lo) / R);
+}
+
+// Bucket of a run whose particles sit in cell (cx, cy, cz): the cell is clamped to the
+// scene's grid (positions clamped outside the box by the walls still get a bucket).
+__device__ __forceinline__ uint32_t bucket_of(const SceneConst &c, int32_t cx, int32_t cy, int32_t cz) {
+    const uint32_t x = (uint32_t)min(max(cx, 0), c.gx - 1) >> c.bshift;
+    const uint32_t y = (uint32_t)min(max(cy, 0), c.gy - 1) >> c.bshift;
+    const uint32_t z = (uint32_t)min(max(cz, 0), c.gz - 1) >> c.bshift;
+    return c.bucket_base + x + c.bgx * (y + c.bgy * z);
 }
 
 // Wall clamp of predictPositions.comp:81-113 / updateVelocities.comp:83-115.
@@ -90,6 +109,7 @@ struct OneScene {
     SceneConst c;
     __device__ __forceinline__ const SceneConst &scene(uint32_t) const { return c; }
     __device__ __forceinline__ const SceneConst &scene_of_brick(uint32_t) const { return c; }
+    __device__ __forceinline__ const SceneConst &scene_of_grid(uint32_t) const { return c; }
 };
 
 struct ManyScenes {
@@ -103,6 +123,16 @@ struct ManyScenes {
         while (hi - lo > 1u) {
             const uint32_t mid = (lo + hi) >> 1;
             if (table[mid].brick_base <= b) lo = mid; else hi = mid;
+        }
+        return table[lo];
+    }
+    // Scene whose grid range [grid_base, grid_base + grid_cells) holds grid entry `g`
+    // (scenes without a grid share the next scene's grid_base).
+    __device__ __forceinline__ SceneConst scene_of_grid(uint32_t g) const {
+        uint32_t lo = 0, hi = count;
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (table[mid].grid_base <= g) lo = mid; else hi = mid;
         }
         return table[lo];
     }

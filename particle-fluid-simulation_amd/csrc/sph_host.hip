@@ -36,6 +36,15 @@ int dump_stage(const Chain &c, hipStream_t s, int stage, uint32_t p_begin, uint3
     hipError_t e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
     int rc = SPH_OK;
+    // Per-slot intermediates of the per-lane path are in cell order: hash slot of each.
+    std::vector<uint32_t> hs;
+    if (c.cell_order && (stage == SPH_DUMP_DENSITY || stage == SPH_DUMP_NEAR_DENSITY ||
+                         stage == SPH_DUMP_PRED_POS || stage == SPH_DUMP_PRED_VEL ||
+                         stage == SPH_DUMP_VEL_FORCE)) {
+        if ((rc = copy_back(s, hs, c.b.hslot + p_begin, n))) return rc;
+        for (uint32_t &h : hs) h -= p_begin;
+    }
+    auto at = [&](size_t i) { return hs.empty() ? i : (size_t)hs[i]; };
     switch (stage) {
         case SPH_DUMP_CELL_KEYS: case SPH_DUMP_SORTED_KEYS: {
             std::vector<uint32_t> k;
@@ -72,7 +81,7 @@ int dump_stage(const Chain &c, hipStream_t s, int stage, uint32_t p_begin, uint3
             if ((rc = copy_back(s, d, c.b.dens + p_begin, 2 * n))) return rc;
             float *o = (float *)dst;
             const size_t off = stage == SPH_DUMP_DENSITY ? 0 : 1;
-            for (size_t i = 0; i < n; ++i) o[i] = d[2 * i + off];
+            for (size_t i = 0; i < n; ++i) o[at(i)] = d[2 * i + off];
             break;
         }
         case SPH_DUMP_PRED_POS: case SPH_DUMP_PRED_VEL: case SPH_DUMP_VEL_FORCE: {
@@ -81,7 +90,7 @@ int dump_stage(const Chain &c, hipStream_t s, int stage, uint32_t p_begin, uint3
             if (!src) return fail(SPH_ERR_STATE, "stage %d needs debug capture (sph_world_set_debug)", stage);
             std::vector<float> v;
             if ((rc = copy_back(s, v, src + p_begin, 4 * n))) return rc;
-            std::memcpy(dst, v.data(), n * 16);
+            for (size_t i = 0; i < n; ++i) std::memcpy((float *)dst + 4 * at(i), &v[4 * i], 16);
             break;
         }
     }

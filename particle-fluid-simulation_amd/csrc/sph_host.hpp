@@ -14,9 +14,12 @@ int hip_fail(hipError_t e, const char *what);
 const char *last_error();
 
 SceneConst make_scene_const(const sph_params &p, uint32_t table_cells, uint32_t table_base,
-                            uint32_t p_begin, uint32_t p_end, uint32_t brick_base);
+                            uint32_t p_begin, uint32_t p_end, uint32_t brick_base,
+                            uint32_t bucket_base, uint32_t grid_base);
 // Bricks of one scene's cell grid (0 when the grid is too large to tile).
 uint64_t scene_bricks(const SceneConst &c);
+// Buckets of one scene's cell-ordered layout (at most 2n + 1024).
+uint32_t scene_buckets(const SceneConst &c);
 int validate_params(const sph_params *p);
 int validate_state(const sph_params &p, uint32_t n, const float *pos4, const float *vel4);
 uint32_t key_bits_for(uint64_t table_total);
@@ -34,14 +37,20 @@ struct Chain {
     ChainBuffers b{};
     uint32_t n = 0, table_total = 0, key_bits = 0, passes = 0, tiles = 0, lookback_words = 0;
     uint32_t brick_total = 0;  // bricks over all scenes; 0 disables the brick kernels
+    uint32_t bucket_total = 0; // buckets of the cell-ordered layout over all scenes (>= 1)
+    uint32_t grid_total = 0;   // halo-grid cells over all scenes
     bool bricks = false;       // density/force through the brick kernels (else per lane)
+    bool cell_order = true;    // the last enqueued substep left its arrays in cell order
     int lane_cfg = kDefaultLaneConfig;  // launch configuration of the per-lane kernels
-    uint32_t n_cap = 0, t_cap = 0, b_cap = 0;  // allocated capacity
+    uint32_t n_cap = 0, t_cap = 0, b_cap = 0, u_cap = 0, g_cap = 0;  // allocated capacity
 
     // Allocates for the given capacity and configures the chain to use all of it.
-    int allocate(uint32_t n, uint32_t table_total, uint32_t brick_total);
-    // Uses the first n slots, table_total cells and brick_total bricks (within capacity).
-    void configure(uint32_t n, uint32_t table_total, uint32_t brick_total);
+    int allocate(uint32_t n, uint32_t table_total, uint32_t brick_total, uint32_t bucket_total,
+                 uint32_t grid_total);
+    // Uses the first n slots, table_total cells, brick_total bricks, bucket_total buckets
+    // and grid_total halo-grid cells (within capacity).
+    void configure(uint32_t n, uint32_t table_total, uint32_t brick_total, uint32_t bucket_total,
+                   uint32_t grid_total);
     int set_debug(bool on);
     void release();
     const uint32_t *sorted_keys() const;

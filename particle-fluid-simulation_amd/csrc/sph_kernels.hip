@@ -23,7 +23,9 @@ __global__ __launch_bounds__(kBlock) void k_hash(
     Scenes sc, const float4 *__restrict__ pos, uint32_t *__restrict__ keys, uint32_t n,
     uint32_t passes, uint32_t *__restrict__ ghist, uint32_t hash_blocks,
     int2 *__restrict__ table, uint32_t table_total, uint32_t *__restrict__ lookback,
-    uint32_t lookback_words, uint32_t *__restrict__ tile_ctr, uint32_t *__restrict__ counters) {
+    uint32_t lookback_words, uint32_t *__restrict__ tile_ctr, uint32_t *__restrict__ counters,
+    uint32_t *__restrict__ bcount, uint32_t bucket_total, uint32_t *__restrict__ blookback,
+    uint32_t blookback_words) {
     __shared__ uint32_t s_hist[4][256];
     const uint32_t tid = threadIdx.x;
     if (blockIdx.x < hash_blocks) {
@@ -50,17 +52,19 @@ __global__ __launch_bounds__(kBlock) void k_hash(
     } else {
         const uint32_t b = blockIdx.x - hash_blocks;
         const uint32_t nb = gridDim.x - hash_blocks;
-        if (b == 0 && tid < 4) tile_ctr[tid] = 0u;
+        if (b == 0 && tid < kTileCounters) tile_ctr[tid] = 0u;
         if (b == 0 && tid < 2 && counters) counters[tid] = 0u;
         for (uint32_t i = b * kBlock + tid; i < table_total; i += nb * kBlock) table[i] = make_int2(-1, -1);
         for (uint32_t i = b * kBlock + tid; i < lookback_words; i += nb * kBlock) lookback[i] = 0u;
+        for (uint32_t i = b * kBlock + tid; i < bucket_total; i += nb * kBlock) bcount[i] = 0u;
+        for (uint32_t i = b * kBlock + tid; i < blookback_words; i += nb * kBlock) blookback[i] = 0u;
     }
 }
 
 // ---------------------------------------------------------------------------------------
-// Reorder: scatterPass.comp:50-65 gathers the state into key order; predictPositions.comp
-// is a pure per-particle function, so it is evaluated here on the gathered state instead
-// of being gathered; updateStartIndices.comp:40-64 marks the run boundaries.
+// Reorder: updateStartIndices.comp:40-64 marks the run boundaries.  On the brick path it
+// also does what scatterPass.comp:50-65 does, gathering the state into key order, and
+// evaluates predictPositions.comp (a pure per-particle function) on the gathered state.
 template <class Scenes>
 __global__ __launch_bounds__(kBlock) void k_reorder(
     Scenes sc, const uint32_t *__restrict__ skeys, const uint32_t *__restrict__ sidx,
@@ -76,16 +80,6 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
     bool to_stray = false;
     if (j < n) {
         const uint32_t key = skeys[j];
-        const uint32_t idx = sidx ? sidx[j] : j;
-        const float4 P = pos[idx];
-        const float4 V = vel[idx];
-        const SceneConst c = sc.scene(j);
-        float3 p = make_float3(P.x, P.y, P.z), v = make_float3(V.x, V.y, V.z);
-        kick_drift<false>(c, p, v);
-        pos_sorted[j] = P;
-        vel_sorted[j] = V;
-        pred_pos[j] = make_float4(p.x, p.y, p.z, 1.0f);
-        if (dbg_pred_vel) dbg_pred_vel[j] = make_float4(v.x, v.y, v.z, 1.0f);
         int *tbl = reinterpret_cast<int *>(table);
         if (j == 0) {
             tbl[2u * key] = 0;
@@ -98,6 +92,16 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
         }
         if (j == n - 1u) tbl[2u * key + 1u] = (int)n;
         if (brick_flag) {
+            const uint32_t idx = sidx ? sidx[j] : j;
+            const float4 P = pos[idx];
+            const float4 V = vel[idx];
+            const SceneConst c = sc.scene(j);
+            float3 p = make_float3(P.x, P.y, P.z), v = make_float3(V.x, V.y, V.z);
+            kick_drift<false>(c, p, v);
+            pos_sorted[j] = P;
+            vel_sorted[j] = V;
+            pred_pos[j] = make_float4(p.x, p.y, p.z, 1.0f);
+            if (dbg_pred_vel) dbg_pred_vel[j] = make_float4(v.x, v.y, v.z, 1.0f);
             // Work for the brick kernels: the full hash of the particle's current cell
             // tells its own cell's run apart from colliding cells; its brick is marked
             // non-empty.  Particles of one-lane-per-particle scenes and particles outside
@@ -130,50 +134,62 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
 
 // ---------------------------------------------------------------------------------------
 // Density: calculateDensities.comp:96-155, one lane per particle, runs read from global.
+// Slots, pred_pos and `table` (the run table) are in cell order.
 // kU: candidates loaded per scan iteration; kB: workgroup size (tuned per size bucket).
+// Run lookup of the cell-ordered layout for scene c.
+__device__ __forceinline__ GridRuns grid_runs(const SceneConst &c, const ChainBuffers &b) {
+    return GridRuns{b.grid, c.grid_base, (uint32_t)c.gx + 2u, (uint32_t)c.gy + 2u,
+                    c.grid_cells ? (uint32_t)c.gz + 2u : 0u, c.table_base, c.mask, b.table,
+                    b.bcount, b.run_off, b.run_bucket};
+}
+
+// Workgroups are handed to the 8 XCDs round-robin; this renumbering gives each XCD one
+// contiguous eighth of the slots, so that neighbouring cells share that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_block() {
+    constexpr uint32_t kXcds = 8;
+    const uint32_t b = blockIdx.x, nb = gridDim.x;
+    const uint32_t q = nb / kXcds, r = nb % kXcds, x = b % kXcds;
+    return x * q + min(x, r) + b / kXcds;
+}
+
 template <class Scenes, int kU, int kB>
-__global__ __launch_bounds__(kB) void k_density(
-    Scenes sc, const float4 *__restrict__ pred_pos, const int2 *__restrict__ table,
-    float2 *__restrict__ dens, uint32_t n) {
-    const uint32_t j = blockIdx.x * kB + threadIdx.x;
+__global__ __launch_bounds__(kB) void k_density(Scenes sc, const ChainBuffers b, uint32_t n) {
+    const uint32_t j = xcd_block() * kB + threadIdx.x;
     if (j >= n) return;
     const SceneConst c = sc.scene(j);
-    const float4 s = pred_pos[j];
-    const DensityCells<false> cells{CellRuns<false>{c.table_base, c.mask, table, nullptr, 0, 0, 0},
-                                    sample_axis(s.x, c.minx, c.maxx, c.R),
-                                    sample_axis(s.y, c.miny, c.maxy, c.R),
-                                    sample_axis(s.z, c.minz, c.maxz, c.R)};
-    dens[j] = density_walk<false, kU>(c, s, pred_pos, nullptr, cells);
+    const float4 s = b.pred_pos[j];
+    const DensityCellsOf<GridRuns> cells{grid_runs(c, b), sample_axis(s.x, c.minx, c.maxx, c.R),
+                                         sample_axis(s.y, c.miny, c.maxy, c.R),
+                                         sample_axis(s.z, c.minz, c.maxz, c.R)};
+    b.dens[j] = density_walk<false, kU>(c, s, b.pred_pos, nullptr, cells);
 }
 
 // ---------------------------------------------------------------------------------------
 // Force + integrate: applyPressureAndViscosityForce.comp:275-337 followed by
 // updateVelocities.comp:71-119.  Neighbour velocities come from vel_sorted, which no lane
 // writes (the reference races on that buffer; here every read sees the pre-force value).
+// Works in cell order; the new state of cell-ordered slot j goes to hash slot out_slot[j].
 template <class Scenes, int kU, int kB>
-__global__ __launch_bounds__(kB) void k_force(
-    Scenes sc, const float4 *__restrict__ pred_pos, const float4 *__restrict__ pos_sorted,
-    const float4 *__restrict__ vel_sorted, const float2 *__restrict__ dens,
-    const int2 *__restrict__ table, float4 *__restrict__ pos_out, float4 *__restrict__ vel_out,
-    float4 *__restrict__ dbg_vel_force, uint32_t n) {
+__global__ __launch_bounds__(kB) void k_force(Scenes sc, const ChainBuffers b, uint32_t n) {
     __shared__ uint32_t s_queue[kQueue * kB];
-    const uint32_t j = blockIdx.x * kB + threadIdx.x;
+    const uint32_t j = xcd_block() * kB + threadIdx.x;
     if (j >= n) return;
     const SceneConst c = sc.scene(j);
-    const float4 s = pred_pos[j];
-    const float4 P = pos_sorted[j];
-    const float4 V = vel_sorted[j];
-    const float2 myd = dens[j];
+    const float4 s = b.pred_pos[j];
+    const float4 P = b.pos_sorted[j];
+    const float4 V = b.vel_sorted[j];
+    const float2 myd = b.dens[j];
     const ForceSelf me = force_self(c, j, s, P, V, myd);
-    const ForceCells<false> cells{CellRuns<false>{c.table_base, c.mask, table, nullptr, 0, 0, 0}, force_cell(c, s)};
+    const ForceCellsOf<GridRuns> cells{grid_runs(c, b), force_cell(c, s)};
     float3 Fp = make_float3(0.0f, 0.0f, 0.0f), Fv = make_float3(0.0f, 0.0f, 0.0f);
-    force_walk<false, kU>(c, me, pred_pos, vel_sorted, dens, nullptr, nullptr, cells,
+    force_walk<false, kU>(c, me, b.pred_pos, b.vel_sorted, b.dens, nullptr, nullptr, cells,
                           s_queue + threadIdx.x, kB, Fp, Fv);
     float4 po, vo, vf;
     force_finish(c, P, V, myd.x, Fp, Fv, po, vo, vf);
-    if (dbg_vel_force) dbg_vel_force[j] = vf;
-    pos_out[j] = po;
-    vel_out[j] = vo;
+    if (b.dbg_vel_force) b.dbg_vel_force[j] = vf;
+    const uint32_t o = b.hslot[j];
+    b.pos[o] = po;
+    b.vel[o] = vo;
 }
 
 inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1u) / kBlock; }
@@ -182,17 +198,21 @@ inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1u) / kBlock; }
 
 template <class Scenes>
 hipError_t launch_hash(hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n,
-                       uint32_t table_total, uint32_t passes, uint32_t lookback_words) {
+                       uint32_t table_total, uint32_t passes, uint32_t lookback_words,
+                       uint32_t bucket_total) {
     // ~8K keys per hashing workgroup keeps the global histogram atomics few; the clearing
     // workgroups run beside them.
     const uint32_t hash_blocks = (n + 8191u) / 8192u;
-    const uint32_t clear_work = table_total > lookback_words ? table_total : lookback_words;
+    const uint32_t btiles = (bucket_total + kBucketTile - 1u) / kBucketTile;
+    uint32_t clear_work = table_total > lookback_words ? table_total : lookback_words;
+    if (bucket_total > clear_work) clear_work = bucket_total;
     uint32_t clear_blocks = (clear_work + 4u * kBlock - 1u) / (4u * kBlock);
     if (clear_blocks < 1u) clear_blocks = 1u;
     if (clear_blocks > 2048u) clear_blocks = 2048u;
     hipLaunchKernelGGL(k_hash<Scenes>, dim3(hash_blocks + clear_blocks), dim3(kBlock), 0, stream,
                        sc, b.pos, b.keys, n, passes, b.ghist, hash_blocks, b.table, table_total,
-                       b.lookback, lookback_words, b.tile_ctr, b.counters);
+                       b.lookback, lookback_words, b.tile_ctr, b.counters, b.bcount, bucket_total,
+                       b.blookback, btiles);
     return hipGetLastError();
 }
 
@@ -213,16 +233,13 @@ const LaneConfig kLaneConfigs[kNumLaneConfigs] = {
 
 template <class Scenes, int kU, int kB>
 hipError_t launch_density_t(hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n) {
-    hipLaunchKernelGGL((k_density<Scenes, kU, kB>), dim3((n + kB - 1u) / kB), dim3(kB), 0, stream, sc,
-                       b.pred_pos, b.table, b.dens, n);
+    hipLaunchKernelGGL((k_density<Scenes, kU, kB>), dim3((n + kB - 1u) / kB), dim3(kB), 0, stream, sc, b, n);
     return hipGetLastError();
 }
 
 template <class Scenes, int kU, int kB>
 hipError_t launch_force_t(hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n) {
-    hipLaunchKernelGGL((k_force<Scenes, kU, kB>), dim3((n + kB - 1u) / kB), dim3(kB), 0, stream, sc,
-                       b.pred_pos, b.pos_sorted, b.vel_sorted, b.dens, b.table, b.pos, b.vel,
-                       b.dbg_vel_force, n);
+    hipLaunchKernelGGL((k_force<Scenes, kU, kB>), dim3((n + kB - 1u) / kB), dim3(kB), 0, stream, sc, b, n);
     return hipGetLastError();
 }
 
@@ -253,7 +270,7 @@ hipError_t launch_force(hipStream_t stream, const Scenes &sc, const ChainBuffers
 
 #define SPH_INSTANTIATE(S)                                                                       \
     template hipError_t launch_hash<S>(hipStream_t, const S &, const ChainBuffers &, uint32_t,   \
-                                       uint32_t, uint32_t, uint32_t);                            \
+                                       uint32_t, uint32_t, uint32_t, uint32_t);                  \
     template hipError_t launch_reorder<S>(hipStream_t, const S &, const ChainBuffers &,          \
                                           const uint32_t *, const uint32_t *, uint32_t, bool);   \
     template hipError_t launch_density<S>(hipStream_t, const S &, const ChainBuffers &, uint32_t, int); \

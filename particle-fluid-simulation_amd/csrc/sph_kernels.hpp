@@ -11,6 +11,14 @@ namespace sph {
 
 // Device buffers of one substep chain.  Particle arrays are indexed by slot; for a batch,
 // scene s owns slots [p_begin, p_end) and table cells [table_base, table_base + T_s).
+//
+// Two slot orders exist.  Hash order is the reference's: slots sorted by (cell key,
+// input index); the table, the sort outputs and the state written back are in it.  The
+// per-lane neighbour passes work in cell order instead: whole key runs, each kept in its
+// hash order, placed by the grid bucket of the run's first particle, so that particles of
+// neighbouring cells sit next to each other in memory and in a wave.  pred_pos,
+// pos_sorted, vel_sorted, dens and the debug captures are in cell order on the per-lane
+// path and in hash order on the brick path.
 struct ChainBuffers {
     float4 *pos;         // state positions (vec4), input order of the substep
     float4 *vel;         // state velocities
@@ -33,9 +41,18 @@ struct ChainBuffers {
     uint32_t *queue;        // [brick_total] non-empty bricks of this substep
     uint32_t *stray;        // [n] slots outside every brick grid
     uint32_t *counters;     // [0] queue length [1] strays [2] density drain [3] force drain
+    // Cell-ordered layout (sph_layout.hip).
+    uint32_t *bcount;       // [bucket_total] particles per bucket, then the bucket's first slot
+    uint32_t *blookback;    // [bucket tiles] look-back words of the bucket scan
+    uint32_t *run_off;      // [n] at a run's first hash slot: the run's offset in its bucket
+    uint32_t *run_bucket;   // [n] at a run's first hash slot: the run's bucket
+    uint32_t *hslot;        // [n] hash slot of each cell-ordered slot
+    int2 *grid;             // [grid_total] (start, end) in cell order of each halo-grid cell's run
 };
 
 uint32_t radix_tile_size();
+// tile_ctr[0..3]: radix passes; [4]: bucket scan.
+constexpr uint32_t kTileCounters = 8;
 
 hipError_t radix_pass(hipStream_t stream, const uint32_t *kin, const uint32_t *vin,
                       uint32_t *kout, uint32_t *vout, uint32_t n, uint32_t shift,
@@ -43,18 +60,33 @@ hipError_t radix_pass(hipStream_t stream, const uint32_t *kin, const uint32_t *v
                       uint32_t *tile_counter);
 
 // Hash pass: keys of the current positions, digit histograms for `passes` passes, and
-// clearing of the cell table, the look-back words and the tile counters.
+// clearing of the cell table, the look-back words and the tile counters; with
+// `bucket_total` > 0 also of the bucket counts.
 template <class Scenes>
 hipError_t launch_hash(hipStream_t stream, const Scenes &sc, const ChainBuffers &b,
                        uint32_t n, uint32_t table_total, uint32_t passes,
-                       uint32_t lookback_words);
+                       uint32_t lookback_words, uint32_t bucket_total);
 
-// Gather into sorted order, predicted positions, cell start/end table; clears histograms.
-// With `bricks`, also marks non-empty bricks and lists stray particles.
+// Cell start/end table from the sorted keys; clears histograms.  With `bricks`, also
+// gathers the state into hash order with predicted positions, marks non-empty bricks and
+// lists stray particles; without, the state is gathered later by launch_place.
 template <class Scenes>
 hipError_t launch_reorder(hipStream_t stream, const Scenes &sc, const ChainBuffers &b,
                           const uint32_t *sorted_keys, const uint32_t *sorted_idx, uint32_t n,
                           bool bricks);
+
+// Cell-ordered layout (sph_layout.hip): bucket and in-bucket offset of every run, the
+// bucket scan, then the gather of the state into cell order with predicted positions and
+// the halo grids' run table.
+constexpr uint32_t kBucketTile = 2048;
+template <class Scenes>
+hipError_t launch_runs(hipStream_t stream, const Scenes &sc, const ChainBuffers &b,
+                       const uint32_t *sorted_keys, const uint32_t *sorted_idx, uint32_t n);
+hipError_t launch_bucket_scan(hipStream_t stream, const ChainBuffers &b, uint32_t bucket_total);
+template <class Scenes>
+hipError_t launch_place(hipStream_t stream, const Scenes &sc, const ChainBuffers &b,
+                        const uint32_t *sorted_keys, const uint32_t *sorted_idx, uint32_t n,
+                        uint32_t grid_total);
 
 // Launch configurations of the one-lane-per-particle density / force kernels: candidates
 // loaded per scan iteration and workgroup size.  The autotuner picks one per size bucket.
@@ -66,11 +98,13 @@ constexpr int kNumLaneConfigs = 6;
 constexpr int kDefaultLaneConfig = 1;
 extern const LaneConfig kLaneConfigs[kNumLaneConfigs];
 
+// Density over the cell-ordered layout.
 template <class Scenes>
 hipError_t launch_density(hipStream_t stream, const Scenes &sc, const ChainBuffers &b,
                           uint32_t n, int cfg);
 
-// Pressure + viscosity force, then integrate; writes the new state into b.pos / b.vel.
+// Pressure + viscosity force, then integrate, over the cell-ordered layout; writes the new
+// state into b.pos / b.vel at the hash slots.
 template <class Scenes>
 hipError_t launch_force(hipStream_t stream, const Scenes &sc, const ChainBuffers &b,
                         uint32_t n, int cfg);

@@ -91,11 +91,37 @@ struct CellRuns {
     }
 };
 
+// Finds cell runs in the cell-ordered layout: through the scene's halo grid when the cell
+// lies in it (hz = 0: the scene has none), else through the hash-order table and the
+// run's placement (first cell-ordered slot = bucket start + offset in the bucket).
+struct GridRuns {
+    const int2 *__restrict__ grid;
+    uint32_t grid_base, hx, hy, hz;
+    uint32_t table_base, mask;
+    const int2 *__restrict__ table;
+    const uint32_t *__restrict__ bstart;
+    const uint32_t *__restrict__ run_off;
+    const uint32_t *__restrict__ run_bucket;
+    __device__ __forceinline__ RunRef get(int cx, int cy, int cz) const {
+        const uint32_t lx = (uint32_t)(cx + 1), ly = (uint32_t)(cy + 1), lz = (uint32_t)(cz + 1);
+        if (lx < hx && ly < hy && lz < hz) {
+            const int2 se = grid[grid_base + lx + hx * (ly + hy * lz)];
+            if (se.x < 0) return RunRef{0, 0, 0, true};
+            return RunRef{se.x, se.y - se.x, se.x, true};
+        }
+        const int2 se = table[table_base + (cell_hash((uint32_t)cx, (uint32_t)cy, (uint32_t)cz) & mask)];
+        if (se.x == -1) return RunRef{0, 0, 0, true};
+        const uint32_t h = (uint32_t)se.x;
+        const int first = (int)(bstart[run_bucket[h]] + run_off[h]);
+        return RunRef{first, se.y - se.x, first, true};
+    }
+};
+
 // The density pass's 27 cells (calculateDensities.comp:114-118): sample points
 // s + R*(dx, dy, dz); a sample point outside the box skips its cell.
-template <bool kHaloLds>
-struct DensityCells {
-    CellRuns<kHaloLds> runs;
+template <class Runs>
+struct DensityCellsOf {
+    Runs runs;
     AxisCells ax, ay, az;
     __device__ __forceinline__ RunRef operator()(int dz, int dy, int dx) const {
         if (!(ax.in(dx) && ay.in(dy) && az.in(dz))) return RunRef{0, 0, 0, true};
@@ -105,14 +131,19 @@ struct DensityCells {
 
 // The force pass's 27 cells (applyPressure...comp:283-292): the sample point's cell plus
 // integer offsets, no bounds skip.
-template <bool kHaloLds>
-struct ForceCells {
-    CellRuns<kHaloLds> runs;
+template <class Runs>
+struct ForceCellsOf {
+    Runs runs;
     int3 c0;
     __device__ __forceinline__ RunRef operator()(int dz, int dy, int dx) const {
         return runs.get(c0.x + dx - 1, c0.y + dy - 1, c0.z + dz - 1);
     }
 };
+
+template <bool kHaloLds>
+using DensityCells = DensityCellsOf<CellRuns<kHaloLds>>;
+template <bool kHaloLds>
+using ForceCells = ForceCellsOf<CellRuns<kHaloLds>>;
 
 // One density candidate at distance^2 d2 (calculateDensities.comp:123-127).  d2 > R2_pre
 // implies sqrt(d2) >= R: both kernels are 0 and M + m*0 == M for M >= +0, so such a

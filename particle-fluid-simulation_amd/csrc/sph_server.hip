@@ -95,7 +95,9 @@ struct sph_server {
 namespace {
 
 // Device bytes a resident scene occupies (the chain's per-slot buffers and its table).
-uint64_t scene_footprint(uint32_t n, uint32_t T) { return (uint64_t)n * 128u + (uint64_t)T * 8u; }
+uint64_t scene_footprint(uint32_t n, uint32_t T, uint32_t grid_cells) {
+    return (uint64_t)n * 136u + (uint64_t)T * 8u + (uint64_t)grid_cells * 8u;
+}
 
 bool fits(const sph_server *s, const Request *r) {
     return s->resident.size() < s->cfg.max_scenes &&
@@ -110,11 +112,14 @@ bool fits(const sph_server *s, const Request *r) {
 int repack(sph_server *s, const std::vector<uint32_t> &old_begin) {
     const size_t count = s->resident.size();
     std::vector<SceneConst> table(count);
-    uint32_t pb = 0, tb = 0, bb = 0;
+    uint32_t pb = 0, tb = 0, bb = 0, ub = 0, gb = 0;
     bool moved = false;
     for (size_t i = 0; i < count; ++i) {
         Request *r = s->resident[i];
-        SceneConst c = make_scene_const(r->params, r->T, tb, pb, pb + r->n, bb);
+        SceneConst c = make_scene_const(r->params, r->T, tb, pb, pb + r->n, bb, ub, gb);
+        // <= 2n + 1024 buckets and 4n + 4096 grid cells per scene: within the capacity
+        ub += scene_buckets(c);
+        gb += c.grid_cells;
         if (s->cfg_bricks && r->bricks) bb += (uint32_t)r->bricks;
         else c.use_bricks = 0u;
         table[i] = c;
@@ -161,7 +166,7 @@ int repack(sph_server *s, const std::vector<uint32_t> &old_begin) {
     if (e == hipSuccess && count) e = launch_fill_scene_of(s->stream, s->d_scenes, (uint32_t)count, s->d_scene_of, pb);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) return hip_fail(e, "repack");
-    s->chain.configure(pb, tb, bb ? bb : 1u);
+    s->chain.configure(pb, tb, bb ? bb : 1u, ub ? ub : 1u, gb);
     s->chain.bricks = s->cfg_bricks && bb > 0;
     s->graph.drop();
     return SPH_OK;
@@ -293,7 +298,15 @@ int sph_server_create(const sph_server_config *cfg, sph_server **out) {
         delete s;
         return hip_fail(e, "stream creation");
     }
-    int rc = s->chain.allocate(cfg->max_particles, cfg->max_table_cells, s->cfg.max_bricks);
+    const uint64_t bucket_cap = 2ull * cfg->max_particles + 1024ull * cfg->max_scenes;
+    const uint64_t grid_cap = 4ull * cfg->max_particles + 4096ull * cfg->max_scenes;
+    if (grid_cap >= (1ull << 31)) {
+        hipStreamDestroy(s->stream);
+        delete s;
+        return fail(SPH_ERR_CAPACITY, "max_particles / max_scenes too large");
+    }
+    int rc = s->chain.allocate(cfg->max_particles, cfg->max_table_cells, s->cfg.max_bricks,
+                               (uint32_t)bucket_cap, (uint32_t)grid_cap);
     if (rc) {
         hipStreamDestroy(s->stream);
         delete s;
@@ -308,7 +321,7 @@ int sph_server_create(const sph_server_config *cfg, sph_server **out) {
         delete s;
         return fail(SPH_ERR_OUT_OF_MEMORY, "scene tables: %s", hipGetErrorString(e));
     }
-    s->chain.configure(0, 1, 1);
+    s->chain.configure(0, 1, 1, 1, 0);
     s->worker = std::thread(worker_main, s);
     *out = s;
     return SPH_OK;
@@ -331,9 +344,9 @@ int sph_server_submit(sph_server *s, const sph_params *params, uint32_t n, const
     r->n = n;
     r->T = params->table_cells ? params->table_cells : n;
     r->params.table_cells = r->T;
-    const SceneConst c = make_scene_const(r->params, r->T, 0, 0, n, 0);
+    const SceneConst c = make_scene_const(r->params, r->T, 0, 0, n, 0, 0, 0);
     r->bricks = scene_bricks(c);
-    r->footprint = scene_footprint(n, r->T);
+    r->footprint = scene_footprint(n, r->T, c.grid_cells);
     if (r->footprint > s->cfg.hbm_budget_bytes || n > s->cfg.max_particles || r->T > s->cfg.max_table_cells ||
         (s->cfg_bricks && r->bricks > s->chain.b_cap))
         return fail(SPH_ERR_CAPACITY, "request exceeds the server's capacity");

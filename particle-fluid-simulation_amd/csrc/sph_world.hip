@@ -43,7 +43,8 @@ const char *last_error() { return g_last_error.c_str(); }
 // Constants derived from the parameter block, computed on the host in binary32 exactly as
 // the shaders derive them (calculateDensities.comp:110, applyPressure...comp:243,258,282).
 SceneConst make_scene_const(const sph_params &p, uint32_t table_cells, uint32_t table_base,
-                            uint32_t p_begin, uint32_t p_end, uint32_t brick_base) {
+                            uint32_t p_begin, uint32_t p_end, uint32_t brick_base,
+                            uint32_t bucket_base, uint32_t grid_base) {
     const float kPi = 3.14159265359f;
     SceneConst c{};
     const float R = p.smoothing_radius;
@@ -81,12 +82,34 @@ SceneConst make_scene_const(const sph_params &p, uint32_t table_cells, uint32_t 
     c.nbz = (uint32_t)((c.gz + kBrick - 1) / kBrick);
     c.brick_base = brick_base;
     c.use_bricks = 1u;
+    // Buckets of the cell-ordered layout: single cells unless the grid has more than
+    // 2n + 1024 of them, then blocks of 2^s cells per axis until it has no more.
+    const double cap = 2.0 * (double)(p_end - p_begin) + 1024.0;
+    uint32_t sh = 0;
+    for (;; ++sh) {
+        const double bx = (double)(((uint32_t)c.gx - 1u) >> sh) + 1.0;
+        const double by = (double)(((uint32_t)c.gy - 1u) >> sh) + 1.0;
+        const double bz = (double)(((uint32_t)c.gz - 1u) >> sh) + 1.0;
+        if (bx * by * bz <= cap || sh == 31u) break;
+    }
+    c.bshift = sh;
+    c.bgx = (((uint32_t)c.gx - 1u) >> sh) + 1u;
+    c.bgy = (((uint32_t)c.gy - 1u) >> sh) + 1u;
+    c.bucket_base = bucket_base;
+    // Halo grid when it has at most 4n + 4096 cells.
+    const double halo = ((double)c.gx + 2.0) * ((double)c.gy + 2.0) * ((double)c.gz + 2.0);
+    c.grid_base = grid_base;
+    c.grid_cells = halo <= 4.0 * (double)(p_end - p_begin) + 4096.0 ? (uint32_t)halo : 0u;
     return c;
 }
 
 uint64_t scene_bricks(const SceneConst &c) {
     const uint64_t b = (uint64_t)c.nbx * c.nby * c.nbz;
     return b > (1ull << 28) ? 0 : b;
+}
+
+uint32_t scene_buckets(const SceneConst &c) {
+    return c.bgx * c.bgy * ((((uint32_t)c.gz - 1u) >> c.bshift) + 1u);
 }
 
 int validate_params(const sph_params *p) {
@@ -133,21 +156,27 @@ uint32_t key_bits_for(uint64_t table_total) {
 // ---------------------------------------------------------------------------------------
 // Chain: buffers and launches of one substep, shared by the single world and the batch.
 
-void Chain::configure(uint32_t n_, uint32_t table_total_, uint32_t brick_total_) {
+void Chain::configure(uint32_t n_, uint32_t table_total_, uint32_t brick_total_,
+                      uint32_t bucket_total_, uint32_t grid_total_) {
     n = n_;
     table_total = table_total_;
     brick_total = brick_total_;
+    bucket_total = bucket_total_;
+    grid_total = grid_total_;
     key_bits = key_bits_for(table_total);
     passes = (key_bits + 7u) / 8u;
     tiles = (n + radix_tile_size() - 1u) / radix_tile_size();
     lookback_words = passes * tiles * 256u;
 }
 
-int Chain::allocate(uint32_t n_, uint32_t table_total_, uint32_t brick_total_) {
+int Chain::allocate(uint32_t n_, uint32_t table_total_, uint32_t brick_total_,
+                    uint32_t bucket_total_, uint32_t grid_total_) {
     n_cap = n_;
     t_cap = table_total_;
     b_cap = brick_total_;
-    configure(n_, table_total_, brick_total_);
+    u_cap = bucket_total_;
+    g_cap = grid_total_;
+    configure(n_, table_total_, brick_total_, bucket_total_, grid_total_);
     bricks = brick_total > 0;
     struct Req { void **p; size_t bytes; };
     const size_t N = n;
@@ -159,7 +188,11 @@ int Chain::allocate(uint32_t n_, uint32_t table_total_, uint32_t brick_total_) {
         {(void **)&b.vel_sorted, N * 16}, {(void **)&b.dens, N * 8},
         {(void **)&b.table, (size_t)table_total * 8}, {(void **)&b.ghist, 4 * 256 * 4},
         {(void **)&b.lookback, (size_t)(lookback_words ? lookback_words : 1) * 4},
-        {(void **)&b.tile_ctr, 4 * 4}, {(void **)&b.hfull, N * 4}, {(void **)&b.stray, N * 4},
+        {(void **)&b.tile_ctr, kTileCounters * 4}, {(void **)&b.hfull, N * 4}, {(void **)&b.stray, N * 4},
+        {(void **)&b.bcount, (size_t)bucket_total * 4},
+        {(void **)&b.blookback, (size_t)((bucket_total + kBucketTile - 1u) / kBucketTile) * 4},
+        {(void **)&b.run_off, N * 4}, {(void **)&b.run_bucket, N * 4}, {(void **)&b.hslot, N * 4},
+        {(void **)&b.grid, (size_t)(grid_total ? grid_total : 1) * 8},
         {(void **)&b.counters, 8 * 4},
         {(void **)&b.brick_flag, (size_t)(brick_total ? brick_total : 1) * 4},
         {(void **)&b.queue, (size_t)(brick_total ? brick_total : 1) * 4},
@@ -201,7 +234,8 @@ void Chain::release() {
     void *ptrs[] = {b.pos, b.vel, b.keys, b.sort_k[0], b.sort_v[0], b.sort_k[1], b.sort_v[1],
                     b.pred_pos, b.pos_sorted, b.vel_sorted, b.dens, b.table, b.ghist,
                     b.lookback, b.tile_ctr, b.dbg_pred_vel, b.dbg_vel_force, b.hfull,
-                    b.brick_flag, b.queue, b.stray, b.counters};
+                    b.brick_flag, b.queue, b.stray, b.counters, b.bcount, b.blookback,
+                    b.run_off, b.run_bucket, b.hslot, b.grid};
     for (void *p : ptrs)
         if (p) hipFree(p);
     b = ChainBuffers{};
@@ -214,7 +248,9 @@ template <class Scenes>
 hipError_t Chain::enqueue(hipStream_t s, const Scenes &sc, PassEvents *ev) {
     hipError_t e;
     if (ev) hipEventRecord(ev->ev[0], s);
-    if ((e = launch_hash(s, sc, b, n, table_total, passes, lookback_words)) != hipSuccess) return e;
+    cell_order = !bricks;
+    e = launch_hash(s, sc, b, n, table_total, passes, lookback_words, cell_order ? bucket_total : 0u);
+    if (e != hipSuccess) return e;
     if (ev) hipEventRecord(ev->ev[1], s);
     for (uint32_t q = 0; q < passes; ++q) {
         const uint32_t *kin = q == 0 ? b.keys : b.sort_k[(q - 1u) & 1u];
@@ -226,7 +262,13 @@ hipError_t Chain::enqueue(hipStream_t s, const Scenes &sc, PassEvents *ev) {
     }
     if (ev) hipEventRecord(ev->ev[2], s);
     if ((e = launch_reorder(s, sc, b, sorted_keys(), sorted_idx(), n, bricks)) != hipSuccess) return e;
-    if (bricks && (e = launch_queue(s, b, brick_total)) != hipSuccess) return e;
+    if (bricks) {
+        if ((e = launch_queue(s, b, brick_total)) != hipSuccess) return e;
+    } else {
+        if ((e = launch_runs(s, sc, b, sorted_keys(), sorted_idx(), n)) != hipSuccess) return e;
+        if ((e = launch_bucket_scan(s, b, bucket_total)) != hipSuccess) return e;
+        if ((e = launch_place(s, sc, b, sorted_keys(), sorted_idx(), n, grid_total)) != hipSuccess) return e;
+    }
     if (ev) hipEventRecord(ev->ev[3], s);
     e = bricks ? launch_density_brick(s, sc, b, n, brick_total) : launch_density(s, sc, b, n, lane_cfg);
     if (e != hipSuccess) return e;
@@ -433,7 +475,11 @@ void sph_params_derive(sph_params *p, double frame_dt, uint32_t substeps, double
 uint64_t sph_world_footprint(uint32_t n, uint32_t table_cells) {
     const uint64_t T = table_cells ? table_cells : n;
     const uint64_t tiles = ((uint64_t)n + radix_tile_size() - 1) / radix_tile_size();
-    return (uint64_t)n * (16 * 2 + 4 + 16 + 16 * 3 + 8) + T * 8 + 4 * tiles * 256 * 4 + 4096;
+    // state, keys, 2 x radix ping-pong, 3 sorted float4 arrays, densities, brick and
+    // cell-order slot arrays; the cell table; buckets (at most 2n + 1024), the halo grid
+    // (at most 4n + 4096 cells) and look-back words
+    return (uint64_t)n * (16 * 2 + 4 + 16 + 16 * 3 + 8 + 8 + 12) + T * 8 + (2 * (uint64_t)n + 1024) * 4 +
+           (4 * (uint64_t)n + 4096) * 8 + 4 * tiles * 256 * 4 + 4096;
 }
 
 int sph_world_create(const sph_params *params, uint32_t n, sph_world **out) {
@@ -457,8 +503,9 @@ int sph_world_create(const sph_params *params, uint32_t n, sph_world **out) {
         delete w;
         return hip_fail(e, "stream creation");
     }
-    w->scene.c = make_scene_const(w->params, T, 0, 0, n, 0);
-    rc = w->chain.allocate(n, T, (uint32_t)scene_bricks(w->scene.c));
+    w->scene.c = make_scene_const(w->params, T, 0, 0, n, 0, 0, 0);
+    rc = w->chain.allocate(n, T, (uint32_t)scene_bricks(w->scene.c), scene_buckets(w->scene.c),
+                           w->scene.c.grid_cells);
     if (rc) {
         hipStreamDestroy(w->stream);
         delete w;
@@ -475,9 +522,10 @@ int sph_world_set_params(sph_world *w, const sph_params *params) {
     if (rc) return rc;
     const uint32_t T = params->table_cells ? params->table_cells : w->n;
     if (T != w->T) return fail(SPH_ERR_INVALID_ARG, "table_cells is fixed at creation");
-    SceneConst nc = make_scene_const(*params, T, 0, 0, w->n, 0);
-    if (scene_bricks(nc) != w->chain.brick_total)
-        return fail(SPH_ERR_INVALID_ARG, "the box and smoothing radius fix the brick grid at creation");
+    SceneConst nc = make_scene_const(*params, T, 0, 0, w->n, 0, 0, 0);
+    if (scene_bricks(nc) != w->chain.brick_total || scene_buckets(nc) != w->chain.bucket_total ||
+        nc.grid_cells != w->chain.grid_total)
+        return fail(SPH_ERR_INVALID_ARG, "the box and smoothing radius fix the cell grid at creation");
     w->params = *params;
     w->params.table_cells = T;
     w->scene.c = nc;
