@@ -148,6 +148,26 @@ typedef struct sph_pass_times {
 } sph_pass_times;
 int sph_world_pass_times(sph_world *w, uint32_t substeps, sph_pass_times *out);
 
+/* Density map for the volumetric renderer: replaces updateDensityMap.comp:114-148 and its
+ * dispatch after the substeps (commands.c:418-424; image created by initializeDensityMap,
+ * renderer.c:774-804).  For each voxel of edge voxel_size (DENSITY_CELL_SIZE, main.c:14)
+ * over the box: density / volume at the voxel centre, summed over the runs of the 27 cells
+ * around it in the positions and cell table the last substep left (so at least one
+ * substep must have run since the last upload; SPH_ERR_STATE otherwise).  Voxels are
+ * stored x fastest, dims[0] * dims[1] * dims[2] floats; dims are the image size,
+ * ceil((max - min) * (1 / voxel_size)) per axis.  The cell keys use the table size
+ * (table_cells - 1 as mask), which equals the shader's numParticles - 1 when T = N.  A
+ * voxel the shader would not store (the corner one, when its two voxel-count formulas
+ * differ) is written as 0. */
+int sph_world_density_map_dims(const sph_world *w, float voxel_size, uint32_t dims[3]);
+/* The same voxel counts for any scene's parameters (e.g. one scene of a batch). */
+int sph_density_map_dims(const sph_params *params, float voxel_size, uint32_t dims[3]);
+/* Into host memory; synchronous. */
+int sph_world_density_map(sph_world *w, float voxel_size, float *out, size_t count);
+/* Into device memory (e.g. memory a renderer imported); enqueued on the world's stream,
+ * complete after sph_world_sync. */
+int sph_world_density_map_device(sph_world *w, float voxel_size, float *dev_out, size_t count);
+
 /* Device footprint in bytes of a world of n particles and table_cells cells (0 = n). */
 uint64_t sph_world_footprint(uint32_t n, uint32_t table_cells);
 
@@ -176,6 +196,8 @@ int sph_batch_set_debug(sph_batch *b, int enable);
 int sph_batch_debug_dump(sph_batch *b, uint32_t scene, int stage, void *dst, size_t bytes);
 int sph_batch_scene_uses_bricks(sph_batch *b, uint32_t scene);  /* 1, 0, or < 0 on error */
 int sph_batch_pass_times(sph_batch *b, uint32_t substeps, sph_pass_times *out);
+/* sph_world_density_map for one scene of the batch (its own box and table range). */
+int sph_batch_density_map(sph_batch *b, uint32_t scene, float voxel_size, float *out, size_t count);
 uint32_t sph_batch_particles(const sph_batch *b);
 void sph_batch_destroy(sph_batch *b);
 

@@ -67,6 +67,8 @@ def load() -> C.CDLL:
         "sphref_substep": (C.c_int, [P(RefParams), u32, vp, vp, P(SubstepOut)]),
         "sphref_step": (C.c_int, [P(RefParams), u32, vp, vp, u32]),
         "sphref_density_bruteforce": (None, [P(RefParams), u32, vp, vp, vp, vp]),
+        "sphref_density_map_dims": (None, [P(RefParams), C.c_float, vp]),
+        "sphref_density_map": (None, [P(RefParams), u32, vp, vp, vp, C.c_float, vp]),
         "sphref64_substep": (C.c_int, [P(RefParams), u32, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                        vp, vp]),
     }
@@ -176,6 +178,30 @@ def density_bruteforce(params, pred_pos, sorted_keys):
                                   np.ascontiguousarray(sorted_keys).ctypes.data,
                                   d.ctypes.data, dn.ctypes.data)
     return d, dn
+
+
+def density_map_dims(params, voxel_size: float):
+    dims = np.empty(3, np.uint32)
+    rp = to_ref(params)
+    load().sphref_density_map_dims(C.byref(rp), voxel_size, dims.ctypes.data)
+    return tuple(int(d) for d in dims)
+
+
+def density_map(params, pos, start, end, voxel_size: float) -> np.ndarray:
+    """updateDensityMap over positions in sorted order and the start/end table of the
+    substep that produced them; returns a float32 array of shape (dz, dy, dx)."""
+    lib = load()
+    rp = to_ref(params)
+    n = pos.shape[0]
+    rp.table_cells = rp.table_cells or n
+    dx, dy, dz = density_map_dims(params, voxel_size)
+    out = np.empty((dz, dy, dx), np.float32)
+    pos = np.ascontiguousarray(pos, np.float32)
+    lib.sphref_density_map(C.byref(rp), n, pos.ctypes.data,
+                           np.ascontiguousarray(start, np.int32).ctypes.data,
+                           np.ascontiguousarray(end, np.int32).ctypes.data, voxel_size,
+                           out.ctypes.data)
+    return out
 
 
 def color(vel):

@@ -20,6 +20,7 @@
  *   multiRadixSortHistograms.comp + multiRadixSort.comp (semantics: stable LSD sort),
  *   scatterPass.comp, clearStartIndices.comp, updateStartIndices.comp,
  *   calculateDensities.comp, applyPressureAndViscosityForce.comp, updateVelocities.comp,
+ *   updateDensityMap.comp (the density map after the substeps),
  *   src/renderer/commands.c:106-161,193-402 (pass order), src/main.c:37-82,349 (params).
  *
  * Pins (tests/test_oracle_*.py): closed-form hash known answers, brute-force O(N^2)
@@ -111,6 +112,14 @@ void sphref_force(const sphref_params *p, uint32_t n, const float *pred_pos4,
                   float *vel_out4);
 void sphref_integrate(const sphref_params *p, uint32_t n, float *pos4, float *vel4);
 void sphref_color(uint32_t n, const float *vel4, float *color4);
+
+/* updateDensityMap.comp:114-148: density / volume on voxels of edge voxel_size over the
+ * box, x fastest, from positions in sorted order and the start/end table of the substep
+ * that produced them.  dims: the voxel counts (renderer.c:774-783). */
+void sphref_density_map_dims(const sphref_params *p, float voxel_size, uint32_t dims[3]);
+void sphref_density_map(const sphref_params *p, uint32_t n, const float *pos4,
+                        const int32_t *start, const int32_t *end, float voxel_size,
+                        float *out);
 
 /* One substep in place on pos4/vel4 (n vec4 each); the arrays end up in sorted order.
  * Returns 0, or -1 on allocation failure / invalid sizes. */

@@ -359,6 +359,79 @@ void sphref_color(uint32_t n, const float *vel4, float *color4) {
     }
 }
 
+/* Voxel counts of the density map: initializeDensityMap (renderer.c:774-783) scales the
+ * box extent by the float reciprocal of the voxel edge and takes ceil. */
+void sphref_density_map_dims(const sphref_params *p, float voxel_size, uint32_t dims[3]) {
+    const float inv = 1.0f / voxel_size;
+    for (int a = 0; a < 3; ++a) dims[a] = (uint32_t)(int)ceil((double)((p->max_xyz[a] - p->min_xyz[a]) * inv));
+}
+
+/* outOfBounds (updateDensityMap.comp:94-97): against the voxel count the shader derives,
+ * ceil((max - min) / voxel_size); used for voxels and, as the shader does, for cells. */
+static int map_out_of_bounds(const int32_t v[3], const int32_t mx[3]) {
+    if (v[0] < 0 || v[1] < 0 || v[2] < 0) return 1;
+    if (v[0] > mx[0] || v[1] > mx[1] || v[2] > mx[2]) return 1;
+    return v[0] == mx[0] && v[1] == mx[1] && v[2] == mx[2];
+}
+
+/* updateDensityMap.comp:114-148 over the state and table a substep left (positions in
+ * sorted order, startIndices/endIndices of that substep).  Voxels the shader does not
+ * store (only the corner voxel, and only when the two voxel-count formulas disagree) are
+ * written as 0. */
+void sphref_density_map(const sphref_params *p, uint32_t n, const float *pos4,
+                        const int32_t *start, const int32_t *end, float voxel_size,
+                        float *out) {
+    (void)n;
+    const float R = p->smoothing_radius;
+    const uint32_t mask = p->table_cells - 1u;
+    uint32_t dims[3];
+    sphref_density_map_dims(p, voxel_size, dims);
+    int32_t mx[3];
+    for (int a = 0; a < 3; ++a) mx[a] = (int32_t)ceilf((p->max_xyz[a] - p->min_xyz[a]) / voxel_size);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t vz = 0; vz < (int64_t)dims[2]; ++vz)
+        for (uint32_t vy = 0; vy < dims[1]; ++vy)
+            for (uint32_t vx = 0; vx < dims[0]; ++vx) {
+                const size_t o = (size_t)vx + (size_t)dims[0] * ((size_t)vy + (size_t)dims[1] * (size_t)vz);
+                const int32_t v[3] = {(int32_t)vx, (int32_t)vy, (int32_t)vz};
+                if (map_out_of_bounds(v, mx)) {
+                    out[o] = 0.0f;
+                    continue;
+                }
+                float c[3];
+                for (int a = 0; a < 3; ++a) c[a] = p->min_xyz[a] + ((float)v[a] + 0.5f) * voxel_size;
+                int32_t vc[3];
+                const int outside = c[0] < p->min_xyz[0] || c[1] < p->min_xyz[1] || c[2] < p->min_xyz[2] ||
+                                    c[0] > p->max_xyz[0] || c[1] > p->max_xyz[1] || c[2] > p->max_xyz[2];
+                for (int a = 0; a < 3; ++a)
+                    vc[a] = outside ? -1 : (int32_t)floorf((c[a] - p->min_xyz[a]) / R);
+                float density = 0.0f;
+                for (int dz = -1; dz <= 1; ++dz)
+                    for (int dy = -1; dy <= 1; ++dy)
+                        for (int dx = -1; dx <= 1; ++dx) {
+                            const int32_t cell[3] = {vc[0] + dx, vc[1] + dy, vc[2] + dz};
+                            if (map_out_of_bounds(cell, mx)) continue;
+                            const uint32_t key = sphref_cell_hash((uint32_t)cell[0], (uint32_t)cell[1],
+                                                                  (uint32_t)cell[2]) & mask;
+                            const int32_t s0 = start[key];
+                            if (s0 == -1) continue;
+                            const int32_t e0 = end[key];
+                            for (int32_t i = s0; i < e0; ++i) {
+                                const float *q = &pos4[4 * (size_t)i];
+                                const float ex = c[0] - q[0], ey = c[1] - q[1], ez = c[2] - q[2];
+                                const float d2 = ex * ex + ey * ey + ez * ez;
+                                if (d2 < R * R) {
+                                    const float dist = sqrtf(d2);
+                                    const float k = dist >= R ? 0.0f : (R - dist) * (R - dist);
+                                    density = density + p->particle_mass * k;
+                                }
+                            }
+                        }
+                density = density * 1.0f;
+                out[o] = density / p->volume;
+            }
+}
+
 static void gather4(uint32_t n, const uint32_t *idx, const float *src, float *dst) {
 #pragma omp parallel for schedule(static)
     for (int64_t j = 0; j < (int64_t)n; ++j) memcpy(&dst[4 * j], &src[4 * (size_t)idx[j]], 16);

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/sph_world.h"
+#include "sph_map.hpp"
 #include "sph_host.hpp"
 #include "sph_kernels.hpp"
 
@@ -40,6 +41,8 @@ struct sph_batch {
     uint16_t *d_scene_of = nullptr;
     GraphCache graph;
     bool stepped = false;
+    float *d_map = nullptr;  // density map staging for host copies
+    size_t map_cap = 0;
     ManyScenes policy() const { return ManyScenes{d_scenes, d_scene_of, count}; }
 };
 
@@ -236,6 +239,7 @@ static int batch_autotune(sph_batch *b, uint32_t substeps) {
     if (sp) (void)hipFree(sp);
     if (sv) (void)hipFree(sv);
     if (e != hipSuccess) return hip_fail(e, "batch autotune");
+    b->stepped = false;  // the table is the last trial's, not the restored state's
     b->tuned_variant = best < 0 ? SPH_KERNEL_BRICK : SPH_KERNEL_DIRECT;
     if (best >= 0) b->chain.lane_cfg = best;
     return batch_apply(b, b->tuned_variant, false);
@@ -372,9 +376,35 @@ int sph_batch_pass_times(sph_batch *b, uint32_t substeps, sph_pass_times *out) {
 
 uint32_t sph_batch_particles(const sph_batch *b) { return b ? b->n_total : 0u; }
 
+int sph_batch_density_map(sph_batch *b, uint32_t scene, float voxel_size, float *out, size_t count) {
+    if (!b || !out) return fail(SPH_ERR_INVALID_ARG, "NULL argument");
+    if (scene >= b->count) return fail(SPH_ERR_INVALID_ARG, "scene %u out of range", scene);
+    MapGeometry g{};
+    int rc = map_geometry(b->params[scene], voxel_size, g);
+    if (rc) return rc;
+    if (count < g.voxels) return fail(SPH_ERR_INVALID_ARG, "density map needs %llu floats, got %zu",
+                                      (unsigned long long)g.voxels, count);
+    if (!b->stepped) return fail(SPH_ERR_STATE, "no substep has run since the last upload");
+    if (b->map_cap < g.voxels) {
+        hipError_t e = hipStreamSynchronize(b->stream);
+        if (e == hipSuccess && b->d_map) e = hipFree(b->d_map);
+        b->d_map = nullptr;
+        b->map_cap = 0;
+        if (e == hipSuccess) e = hipMalloc((void **)&b->d_map, g.voxels * sizeof(float));
+        if (e != hipSuccess) return fail(SPH_ERR_OUT_OF_MEMORY, "density map buffer: %s", hipGetErrorString(e));
+        b->map_cap = g.voxels;
+    }
+    hipError_t e = launch_density_map(b->stream, b->scenes[scene], b->chain.b.pos, b->chain.b.table, g, b->d_map);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, b->d_map, g.voxels * sizeof(float), hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    return e == hipSuccess ? SPH_OK : hip_fail(e, "density map");
+}
+
 void sph_batch_destroy(sph_batch *b) {
     if (!b) return;
     if (b->stream) hipStreamSynchronize(b->stream);
+    if (b->d_map) hipFree(b->d_map);
     b->graph.drop();
     b->chain.release();
     if (b->d_scenes) hipFree(b->d_scenes);

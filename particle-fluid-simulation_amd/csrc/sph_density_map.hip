@@ -1,0 +1,109 @@
+// Density map for the volumetric renderer: updateDensityMap.comp:114-148, dispatched after
+// the substeps (commands.c:418-424) over a voxel grid of edge DENSITY_CELL_SIZE (main.c:14).
+//
+// One lane per voxel, voxels numbered x-fastest (the image's x, y, z).  Each voxel centre
+// walks the 27 cells around its own cell in (dz, dy, dx) order and sums
+// m * (R - d)^2 over the particles of each cell's run closer than R, reading the positions
+// and the start/end table the last substep left: positions in its sorted order, runs keyed
+// by the cells the particles were hashed in at the start of that substep.  As in the
+// shader, cells are bounds-checked against the voxel counts, not the cell counts, and the
+// corner voxel (M, M, M) is skipped.  Lanes of a wave hold neighbouring voxels, which share
+// most of their cells, so the runs they read are shared through the caches.
+#include "sph_device.hpp"
+#include "sph_map.hpp"
+
+namespace sph {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ bool map_out_of_bounds(int x, int y, int z, const MapGeometry &g) {
+    if (x < 0 || y < 0 || z < 0) return true;
+    if (x > g.mx || y > g.my || z > g.mz) return true;
+    return x == g.mx && y == g.my && z == g.mz;
+}
+
+__global__ __launch_bounds__(kBlock) void k_density_map(SceneConst c, const float4 *__restrict__ pos,
+                                                       const int2 *__restrict__ table, MapGeometry g,
+                                                       float *__restrict__ out) {
+    const uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (v >= g.voxels) return;
+    const int vx = (int)(v % g.dx), vy = (int)((v / g.dx) % g.dy), vz = (int)(v / ((uint64_t)g.dx * g.dy));
+    if (map_out_of_bounds(vx, vy, vz, g)) {
+        out[v] = 0.0f;
+        return;
+    }
+    const float cx = c.minx + ((float)vx + 0.5f) * g.voxel;
+    const float cy = c.miny + ((float)vy + 0.5f) * g.voxel;
+    const float cz = c.minz + ((float)vz + 0.5f) * g.voxel;
+    // samplePointToCellCoord: -1 on every axis when the centre lies outside the box
+    const bool outside = cx < c.minx || cy < c.miny || cz < c.minz || cx > c.maxx || cy > c.maxy ||
+                         cz > c.maxz;
+    const int ox = outside ? -1 : cell_of(cx, c.minx, c.R);
+    const int oy = outside ? -1 : cell_of(cy, c.miny, c.R);
+    const int oz = outside ? -1 : cell_of(cz, c.minz, c.R);
+    const float R = c.R, R2 = c.R2;
+    float density = 0.0f;
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int x = ox + dx, y = oy + dy, z = oz + dz;
+                if (map_out_of_bounds(x, y, z, g)) continue;
+                const int2 se = table[c.table_base + (cell_hash((uint32_t)x, (uint32_t)y, (uint32_t)z) & c.mask)];
+                if (se.x == -1) continue;
+                for (int i = se.x; i < se.y; ++i) {
+                    const float4 q = pos[i];
+                    const float ex = cx - q.x, ey = cy - q.y, ez = cz - q.z;
+                    const float d2 = ex * ex + ey * ey + ez * ez;
+                    if (d2 < R2) {
+                        const float dist = sqrtf(d2);
+                        const float k = dist >= R ? 0.0f : (R - dist) * (R - dist);
+                        density = density + c.m * k;
+                    }
+                }
+            }
+    density = density * 1.0f;
+    out[v] = density / c.volume;
+}
+
+}  // namespace
+
+int map_geometry(const sph_params &p, float voxel_size, MapGeometry &g) {
+    if (!std::isfinite(voxel_size) || !(voxel_size > 0.0f))
+        return fail(SPH_ERR_INVALID_ARG, "voxel_size must be finite and > 0");
+    // initializeDensityMap (renderer.c:774-783): extent times the float reciprocal, ceil
+    const float inv = 1.0f / voxel_size;
+    uint64_t d[3];
+    int32_t m[3];
+    for (int a = 0; a < 3; ++a) {
+        const double e = std::ceil((double)((p.max_xyz[a] - p.min_xyz[a]) * inv));
+        // the shader's own count: ceil((max - min) / voxel) (updateDensityMap.comp:95)
+        const double s = std::ceil((double)((p.max_xyz[a] - p.min_xyz[a]) / voxel_size));
+        if (!(e >= 1.0) || e > 1.0e9 || s > 1.0e9)
+            return fail(SPH_ERR_INVALID_ARG, "voxel grid of %g voxels per axis is out of range", e);
+        d[a] = (uint64_t)e;
+        m[a] = (int32_t)s;
+    }
+    const uint64_t total = d[0] * d[1] * d[2];
+    if (total >= (1ull << 34)) return fail(SPH_ERR_CAPACITY, "density map of %llu voxels is too large",
+                                           (unsigned long long)total);
+    g.voxel = voxel_size;
+    g.dx = (uint32_t)d[0];
+    g.dy = (uint32_t)d[1];
+    g.dz = (uint32_t)d[2];
+    g.mx = m[0];
+    g.my = m[1];
+    g.mz = m[2];
+    g.voxels = total;
+    return SPH_OK;
+}
+
+hipError_t launch_density_map(hipStream_t stream, const SceneConst &c, const float4 *pos,
+                              const int2 *table, const MapGeometry &g, float *out) {
+    const uint64_t blocks = (g.voxels + kBlock - 1u) / kBlock;
+    hipLaunchKernelGGL(k_density_map, dim3((uint32_t)blocks), dim3(kBlock), 0, stream, c, pos, table, g, out);
+    return hipGetLastError();
+}
+
+}  // namespace sph

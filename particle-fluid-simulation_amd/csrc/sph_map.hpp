@@ -1,0 +1,30 @@
+// Density map (updateDensityMap.comp) launcher shared by the world and batch entry points.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cmath>
+
+#include "sph_device.hpp"
+#include "sph_host.hpp"
+
+namespace sph {
+
+// Voxel grid of the density map over one scene's box.
+struct MapGeometry {
+    float voxel;             // voxel edge (DENSITY_CELL_SIZE)
+    uint32_t dx, dy, dz;     // voxels per axis: the image size (renderer.c:774-783)
+    int32_t mx, my, mz;      // the shader's bound, ceil((max - min) / voxel) (:95)
+    uint64_t voxels;         // dx * dy * dz
+};
+
+// Fills g for the scene's box; SPH_ERR_INVALID_ARG / SPH_ERR_CAPACITY on a bad voxel size.
+int map_geometry(const sph_params &p, float voxel_size, MapGeometry &g);
+
+// Writes g.voxels floats (x fastest) to device memory `out` on `stream`.  `pos` and
+// `table` are the chain's state and cell table after at least one substep.
+hipError_t launch_density_map(hipStream_t stream, const SceneConst &c, const float4 *pos,
+                              const int2 *table, const MapGeometry &g, float *out);
+
+}  // namespace sph

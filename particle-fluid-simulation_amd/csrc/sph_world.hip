@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/sph_world.h"
+#include "sph_map.hpp"
 #include "sph_host.hpp"
 #include "sph_kernels.hpp"
 
@@ -352,9 +353,11 @@ struct sph_world {
     OneScene scene{};
     GraphCache graph;
     bool debug = false;
-    bool stepped = false;
+    bool stepped = false;           // the table and intermediates match the state
     int variant = SPH_KERNEL_AUTO;  // requested neighbour-pass kernels
     bool tuned = false;             // AUTO resolved (by the tuning cache or a measurement)
+    float *d_map = nullptr;         // density map staging for host copies
+    size_t map_cap = 0;             // floats allocated at d_map
 };
 
 namespace {
@@ -429,6 +432,7 @@ int world_autotune(sph_world *w, uint32_t substeps, int *chosen, double *ms_out)
     (void)hipFree(save_p);
     (void)hipFree(save_v);
     if (e != hipSuccess) return hip_fail(e, "autotune");
+    w->stepped = false;  // the table is the last trial's, not the restored state's
     world_apply_code(w, best);
     *chosen = best;
     if (ms_out) *ms_out = best_ms / substeps;
@@ -683,9 +687,70 @@ int sph_world_pass_times(sph_world *w, uint32_t substeps, sph_pass_times *out) {
     return SPH_OK;
 }
 
+int sph_world_density_map_dims(const sph_world *w, float voxel_size, uint32_t dims[3]) {
+    if (!w || !dims) return fail(SPH_ERR_INVALID_ARG, "NULL argument");
+    MapGeometry g{};
+    int rc = map_geometry(w->params, voxel_size, g);
+    if (rc) return rc;
+    dims[0] = g.dx;
+    dims[1] = g.dy;
+    dims[2] = g.dz;
+    return SPH_OK;
+}
+
+int sph_density_map_dims(const sph_params *params, float voxel_size, uint32_t dims[3]) {
+    if (!params || !dims) return fail(SPH_ERR_INVALID_ARG, "NULL argument");
+    int rc = validate_params(params);
+    if (rc) return rc;
+    MapGeometry g{};
+    rc = map_geometry(*params, voxel_size, g);
+    if (rc) return rc;
+    dims[0] = g.dx;
+    dims[1] = g.dy;
+    dims[2] = g.dz;
+    return SPH_OK;
+}
+
+int sph_world_density_map_device(sph_world *w, float voxel_size, float *dev_out, size_t count) {
+    if (!w || !dev_out) return fail(SPH_ERR_INVALID_ARG, "NULL argument");
+    if (!w->stepped) return fail(SPH_ERR_STATE, "no substep has run since the last upload");
+    MapGeometry g{};
+    int rc = map_geometry(w->params, voxel_size, g);
+    if (rc) return rc;
+    if (count < g.voxels) return fail(SPH_ERR_INVALID_ARG, "density map needs %llu floats, got %zu",
+                                      (unsigned long long)g.voxels, count);
+    hipError_t e = launch_density_map(w->stream, w->scene.c, w->chain.b.pos, w->chain.b.table, g, dev_out);
+    return e == hipSuccess ? SPH_OK : hip_fail(e, "density map launch");
+}
+
+int sph_world_density_map(sph_world *w, float voxel_size, float *out, size_t count) {
+    if (!w || !out) return fail(SPH_ERR_INVALID_ARG, "NULL argument");
+    MapGeometry g{};
+    int rc = map_geometry(w->params, voxel_size, g);
+    if (rc) return rc;
+    if (count < g.voxels) return fail(SPH_ERR_INVALID_ARG, "density map needs %llu floats, got %zu",
+                                      (unsigned long long)g.voxels, count);
+    if (!w->stepped) return fail(SPH_ERR_STATE, "no substep has run since the last upload");
+    if (w->map_cap < g.voxels) {
+        hipError_t e = hipStreamSynchronize(w->stream);
+        if (e == hipSuccess && w->d_map) e = hipFree(w->d_map);
+        w->d_map = nullptr;
+        w->map_cap = 0;
+        if (e == hipSuccess) e = hipMalloc((void **)&w->d_map, g.voxels * sizeof(float));
+        if (e != hipSuccess) return fail(SPH_ERR_OUT_OF_MEMORY, "density map buffer: %s", hipGetErrorString(e));
+        w->map_cap = g.voxels;
+    }
+    hipError_t e = launch_density_map(w->stream, w->scene.c, w->chain.b.pos, w->chain.b.table, g, w->d_map);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(out, w->d_map, g.voxels * sizeof(float), hipMemcpyDeviceToHost, w->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
+    return e == hipSuccess ? SPH_OK : hip_fail(e, "density map");
+}
+
 void sph_world_destroy(sph_world *w) {
     if (!w) return;
     if (w->stream) hipStreamSynchronize(w->stream);
+    if (w->d_map) hipFree(w->d_map);
     w->graph.drop();
     w->chain.release();
     if (w->stream) hipStreamDestroy(w->stream);

@@ -96,6 +96,10 @@ def load_library() -> C.CDLL:
         "sph_world_debug_dump": (C.c_int, [vp, C.c_int, vp, C.c_size_t]),
         "sph_world_pass_times": (C.c_int, [vp, C.c_uint32, P(PassTimes)]),
         "sph_world_footprint": (C.c_uint64, [C.c_uint32, C.c_uint32]),
+        "sph_world_density_map_dims": (C.c_int, [vp, C.c_float, P(C.c_uint32 * 3)]),
+        "sph_density_map_dims": (C.c_int, [P(SphParams), C.c_float, P(C.c_uint32 * 3)]),
+        "sph_world_density_map": (C.c_int, [vp, C.c_float, vp, C.c_size_t]),
+        "sph_world_density_map_device": (C.c_int, [vp, C.c_float, vp, C.c_size_t]),
         "sph_world_destroy": (None, [vp]),
         "sph_last_error": (C.c_char_p, []),
         "sph_batch_create": (C.c_int, [C.c_uint32, P(SphParams), P(C.c_uint32), P(vp)]),
@@ -109,6 +113,7 @@ def load_library() -> C.CDLL:
         "sph_batch_scene_uses_bricks": (C.c_int, [vp, C.c_uint32]),
         "sph_batch_pass_times": (C.c_int, [vp, C.c_uint32, P(PassTimes)]),
         "sph_batch_particles": (C.c_uint32, [vp]),
+        "sph_batch_density_map": (C.c_int, [vp, C.c_uint32, C.c_float, vp, C.c_size_t]),
         "sph_batch_destroy": (None, [vp]),
         "sph_server_create": (C.c_int, [vp, P(vp)]),
         "sph_server_submit": (C.c_int, [vp, P(SphParams), C.c_uint32, vp, vp, C.c_uint32,
@@ -133,6 +138,13 @@ def check(rc: int) -> None:
     if rc != SPH_OK:
         msg = load_library().sph_last_error()
         raise SphError(rc, msg.decode() if msg else "")
+
+
+def density_map_dims(params: SphParams, voxel_size: float) -> tuple:
+    """Voxel counts (x, y, z) of the density map of a scene with these parameters."""
+    dims = (C.c_uint32 * 3)()
+    check(load_library().sph_density_map_dims(C.byref(params), voxel_size, C.byref(dims)))
+    return tuple(int(d) for d in dims)
 
 
 def params_for(*, particle_radius: float, smoothing_radius: float, g: float, box_min, box_max,

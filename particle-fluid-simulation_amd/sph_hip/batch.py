@@ -5,7 +5,7 @@ import ctypes as C
 
 import numpy as np
 
-from .abi import DUMP_STAGES, KERNELS, PassTimes, SphParams, check, load_library
+from .abi import DUMP_STAGES, KERNELS, PassTimes, SphParams, check, density_map_dims, load_library
 
 
 class Batch:
@@ -81,6 +81,13 @@ class Batch:
         else:
             out = np.empty((n, 4), np.float32)
         check(self._lib.sph_batch_debug_dump(self._h, scene, DUMP_STAGES[stage], out.ctypes.data, out.nbytes))
+        return out
+
+    def density_map(self, scene: int, voxel_size: float) -> np.ndarray:
+        """updateDensityMap of one scene after the last substep: shape (dz, dy, dx)."""
+        dx, dy, dz = density_map_dims(self.params[scene], voxel_size)
+        out = np.empty((dz, dy, dx), np.float32)
+        check(self._lib.sph_batch_density_map(self._h, scene, voxel_size, out.ctypes.data, out.size))
         return out
 
     def pass_times(self, substeps: int) -> dict:

@@ -52,6 +52,18 @@ class World:
     def sync(self) -> None:
         check(self._lib.sph_world_sync(self._h))
 
+    def density_map_dims(self, voxel_size: float) -> tuple:
+        dims = (C.c_uint32 * 3)()
+        check(self._lib.sph_world_density_map_dims(self._h, voxel_size, C.byref(dims)))
+        return tuple(int(d) for d in dims)
+
+    def density_map(self, voxel_size: float) -> np.ndarray:
+        """updateDensityMap after the last substep: float32 array of shape (dz, dy, dx)."""
+        dx, dy, dz = self.density_map_dims(voxel_size)
+        out = np.empty((dz, dy, dx), np.float32)
+        check(self._lib.sph_world_density_map(self._h, voxel_size, out.ctypes.data, out.size))
+        return out
+
     def download(self, color: bool = False):
         pos = np.empty((self.n, 4), np.float32)
         vel = np.empty((self.n, 4), np.float32)

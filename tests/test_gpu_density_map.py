@@ -1,0 +1,73 @@
+"""Density map (updateDensityMap.comp) on the GPU: bit for bit the FP32 oracle's map of the
+positions and cell table the last substep left, for a single world on either kernel
+family, for the 1M-particle scene, and for scenes of a batch."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _bitexact(name, got, ref):
+    assert got.shape == ref.shape, (got.shape, ref.shape)
+    bad = np.flatnonzero(got.view(np.uint32).ravel() != ref.view(np.uint32).ravel())
+    assert bad.size == 0, f"{name}: {bad.size} mismatches, first {bad[:5]}"
+
+
+def _oracle_map(orc, params, pos, vel, frames, voxel):
+    """frames * substeps oracle substeps; the map of the last one's positions and table."""
+    total = frames * params.substeps
+    if frames > 1:
+        pos, vel = orc.step(params, pos, vel, frames - 1)
+    for _ in range(params.substeps - 1):
+        pos, vel, _ = orc.substep(params, pos, vel)
+    pos, vel, inter = orc.substep(params, pos, vel)
+    assert total >= 1
+    return orc.density_map(params, pos, inter["start"], inter["end"], voxel)
+
+
+@pytest.mark.parametrize("kernel", ["direct", "brick"])
+def test_world_density_map_dam_break(gpu, orc, kernel):
+    from sph_hip.workloads import DAM_BREAK_16K as W
+    p = W.params()
+    pos, vel = W.state()
+    print(f"density map dam_break_16k kernel={kernel}", flush=True)
+    with gpu.World(p, W.n) as w:
+        w.set_kernel(kernel)
+        w.upload(pos, vel)
+        with pytest.raises(gpu.SphError):
+            w.density_map(0.1)  # no substep since the upload
+        w.step(2)
+        got = w.density_map(0.1)
+    ref = _oracle_map(orc, p, pos, vel, 2, 0.1)
+    assert got.shape == (60, 60, 100)
+    assert np.count_nonzero(ref) > 1000
+    _bitexact("density map", got, ref)
+
+
+def test_world_density_map_uniform_1m(gpu, orc):
+    from sph_hip.workloads import UNIFORM_1M as W
+    p = W.params(table_cells=1 << 21)
+    pos, vel = W.state()
+    print("density map uniform_1m T=2^21", flush=True)
+    with gpu.World(p, W.n) as w:
+        w.upload(pos, vel)
+        w.step(1)
+        got = w.density_map(0.5)
+    ref = _oracle_map(orc, p, pos, vel, 1, 0.5)
+    _bitexact("density map", got, ref)
+
+
+def test_batch_density_map_per_scene(gpu, orc):
+    from sph_hip.workloads import DAM_BREAK_16K, mixed_scenes
+    specs = mixed_scenes(count=3, seed=200, n_min=500, n_max=4000, extent=(2.0, 6.0))
+    params = [DAM_BREAK_16K.params()] + [s.params() for s in specs]
+    states = [DAM_BREAK_16K.state()] + [s.state() for s in specs]
+    counts = [DAM_BREAK_16K.n] + [s.n for s in specs]
+    print(f"density map batch counts={counts}", flush=True)
+    with gpu.Batch(params, counts) as b:
+        for i, (q, v) in enumerate(states):
+            b.upload(i, q, v)
+        b.step(1)
+        got = [b.density_map(i, 0.2) for i in range(len(counts))]
+    for i, (q, v) in enumerate(states):
+        _bitexact(f"scene {i}", got[i], _oracle_map(orc, params[i], q, v, 1, 0.2))
