@@ -113,10 +113,14 @@ typedef enum sph_dump_stage {
  *           6x6x6 halo's runs from LDS.
  *   AUTO:   the autotuner's choice (the default): before the first step of a size bucket
  *           (particle count, table size, brick count rounded to powers of two) each
- *           configuration (the brick kernels, and the per-lane kernels with 1, 2 or 4
- *           candidates per scan step and 64-, 128- or 256-lane workgroups) is timed on a
- *           saved copy of the world's state, and the fastest is cached for the process.  SPH_AUTOTUNE=0 in the environment disables the
- *           measurement (DIRECT is used). */
+ *           configuration (the brick kernels, and the per-lane kernels with 2 or 4
+ *           candidates per scan step, 64-, 128- or 256-lane workgroups, with or without
+ *           force neighbour lists recorded by the density pass) is timed on a
+ *           saved copy of the world's state, and the fastest is cached for the process.
+ *           SPH_AUTOTUNE=0 in the environment disables the measurement (DIRECT is used);
+ *           SPH_LANE_CONFIG=0..9 instead pins one per-lane configuration (for profiling:
+ *           candidates per scan step x workgroup lanes 0 = 2x256, 1 = 4x256, 2 = 2x128,
+ *           3 = 4x128, 4 = 2x64 with force neighbour lists, 5..9 the same without). */
 typedef enum sph_kernel_variant {
     SPH_KERNEL_AUTO = 0,
     SPH_KERNEL_DIRECT = 1,
@@ -131,6 +135,11 @@ int sph_world_kernel(const sph_world *w);
 /* Human-readable launch configuration the next step will use (the autotuner also chooses
  * the per-lane kernels' candidates per scan step and workgroup size). */
 int sph_world_kernel_config(const sph_world *w, char *buf, size_t bytes);
+/* Selects DIRECT with one fixed per-lane launch configuration, 0 <= config <
+ * sph_lane_configs() (the numbering of SPH_LANE_CONFIG); for reproducible timings and for
+ * testing every configuration.  All configurations give bit-identical results. */
+int sph_world_set_lane_config(sph_world *w, int config);
+int sph_lane_configs(void);
 
 int sph_world_set_debug(sph_world *w, int enable);
 int sph_world_debug_dump(sph_world *w, int stage, void *dst, size_t bytes);
@@ -205,7 +214,7 @@ void sph_batch_destroy(sph_batch *b);
  * Serving: a request loop over the batched step.  No reference counterpart (the reference
  * runs one interactive scene).  Requests carry their parameters, initial state and step
  * count; a worker thread admits them in arrival order while the sum of resident footprints
- * (136 B per particle + 8 B per table cell and per halo-grid cell) stays within
+ * (392 B per particle, 16 B per table cell, 8 B per halo-grid cell) stays within
  * hbm_budget_bytes and the capacities hold, keeps their state resident in device memory,
  * steps every resident scene together one frame at a time, and retires finished scenes
  * with their state copied back.

@@ -140,7 +140,7 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
 __device__ __forceinline__ GridRuns grid_runs(const SceneConst &c, const ChainBuffers &b) {
     return GridRuns{b.grid, c.grid_base, (uint32_t)c.gx + 2u, (uint32_t)c.gy + 2u,
                     c.grid_cells ? (uint32_t)c.gz + 2u : 0u, c.table_base, c.mask, b.table,
-                    b.bcount, b.run_off, b.run_bucket};
+                    b.bcount, b.run_info};
 }
 
 // Workgroups are handed to the 8 XCDs round-robin; this renumbering gives each XCD one
@@ -152,16 +152,28 @@ __device__ __forceinline__ uint32_t xcd_block() {
     return x * q + min(x, r) + b / kXcds;
 }
 
-template <class Scenes, int kU, int kB>
+template <class Scenes, int kU, int kB, bool kL>
 __global__ __launch_bounds__(kB) void k_density(Scenes sc, const ChainBuffers b, uint32_t n) {
+    __shared__ uint32_t s_list[kL ? kListLds * kB : 1];
     const uint32_t j = xcd_block() * kB + threadIdx.x;
     if (j >= n) return;
     const SceneConst c = sc.scene(j);
     const float4 s = b.pred_pos[j];
-    const DensityCellsOf<GridRuns> cells{grid_runs(c, b), sample_axis(s.x, c.minx, c.maxx, c.R),
-                                         sample_axis(s.y, c.miny, c.maxy, c.R),
-                                         sample_axis(s.z, c.minz, c.maxz, c.R)};
-    b.dens[j] = density_walk<false, kU>(c, s, b.pred_pos, nullptr, cells);
+    if constexpr (!kL) {
+        const DensityCellsOf<GridRuns> cells{grid_runs(c, b), sample_axis(s.x, c.minx, c.maxx, c.R),
+                                             sample_axis(s.y, c.miny, c.maxy, c.R),
+                                             sample_axis(s.z, c.minz, c.maxz, c.R)};
+        b.dens[j] = density_walk_rows<kU>(c, s, b.pred_pos, cells);
+        return;
+    }
+    uint32_t count;
+    b.dens[j] = density_with_list<kU>(c, j, s, b.pred_pos, grid_runs(c, b), s_list + threadIdx.x, kB,
+                                      b.nlist, n, count);
+    b.ncount[j] = (uint8_t)count;
+    // entry-major copy-out: every store of the loop writes one coalesced row
+    const uint32_t m = count == kListInvalid ? 0u : min(count, (uint32_t)kListLds);
+    for (uint32_t k = 0; __ballot(k < m); ++k)
+        if (k < m) b.nlist[(size_t)k * n + j] = s_list[k * kB + threadIdx.x];
 }
 
 // ---------------------------------------------------------------------------------------
@@ -169,7 +181,7 @@ __global__ __launch_bounds__(kB) void k_density(Scenes sc, const ChainBuffers b,
 // updateVelocities.comp:71-119.  Neighbour velocities come from vel_sorted, which no lane
 // writes (the reference races on that buffer; here every read sees the pre-force value).
 // Works in cell order; the new state of cell-ordered slot j goes to hash slot out_slot[j].
-template <class Scenes, int kU, int kB>
+template <class Scenes, int kU, int kB, bool kL>
 __global__ __launch_bounds__(kB) void k_force(Scenes sc, const ChainBuffers b, uint32_t n) {
     __shared__ uint32_t s_queue[kQueue * kB];
     const uint32_t j = xcd_block() * kB + threadIdx.x;
@@ -180,10 +192,17 @@ __global__ __launch_bounds__(kB) void k_force(Scenes sc, const ChainBuffers b, u
     const float4 V = b.vel_sorted[j];
     const float2 myd = b.dens[j];
     const ForceSelf me = force_self(c, j, s, P, V, myd);
-    const ForceCellsOf<GridRuns> cells{grid_runs(c, b), force_cell(c, s)};
     float3 Fp = make_float3(0.0f, 0.0f, 0.0f), Fv = make_float3(0.0f, 0.0f, 0.0f);
-    force_walk<false, kU>(c, me, b.pred_pos, b.vel_sorted, b.dens, nullptr, nullptr, cells,
-                          s_queue + threadIdx.x, kB, Fp, Fv);
+    const uint32_t cnt = kL ? b.ncount[j] : kListInvalid;
+    if (cnt != kListInvalid) {
+        // the recorded neighbours, in the walk's order
+        for (uint32_t i = 0; i < cnt; ++i)
+            force_term<false>(c, me, kGlobalBit | b.nlist[(size_t)i * n + j], b.pred_pos, b.vel_sorted,
+                              b.dens, nullptr, nullptr, Fp, Fv);
+    } else {
+        const ForceCellsOf<GridRuns> cells{grid_runs(c, b), force_cell(c, s)};
+        force_walk_rows<kU>(c, me, b.pred_pos, b.vel_sorted, b.dens, cells, s_queue + threadIdx.x, kB, Fp, Fv);
+    }
     float4 po, vo, vf;
     force_finish(c, P, V, myd.x, Fp, Fv, po, vo, vf);
     if (b.dbg_vel_force) b.dbg_vel_force[j] = vf;
@@ -228,44 +247,62 @@ hipError_t launch_reorder(hipStream_t stream, const Scenes &sc, const ChainBuffe
 }
 
 const LaneConfig kLaneConfigs[kNumLaneConfigs] = {
-    {1, 256}, {2, 256}, {4, 256}, {2, 128}, {4, 128}, {2, 64},
+    {2, 256, true}, {4, 256, true}, {2, 128, true}, {4, 128, true}, {2, 64, true},
+    {2, 256, false}, {4, 256, false}, {2, 128, false}, {4, 128, false}, {2, 64, false},
 };
 
-template <class Scenes, int kU, int kB>
+template <class Scenes, int kU, int kB, bool kL>
 hipError_t launch_density_t(hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n) {
-    hipLaunchKernelGGL((k_density<Scenes, kU, kB>), dim3((n + kB - 1u) / kB), dim3(kB), 0, stream, sc, b, n);
+    hipLaunchKernelGGL((k_density<Scenes, kU, kB, kL>), dim3((n + kB - 1u) / kB), dim3(kB), 0, stream, sc, b, n);
     return hipGetLastError();
 }
 
-template <class Scenes, int kU, int kB>
+template <class Scenes, int kU, int kB, bool kL>
 hipError_t launch_force_t(hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n) {
-    hipLaunchKernelGGL((k_force<Scenes, kU, kB>), dim3((n + kB - 1u) / kB), dim3(kB), 0, stream, sc, b, n);
+    hipLaunchKernelGGL((k_force<Scenes, kU, kB, kL>), dim3((n + kB - 1u) / kB), dim3(kB), 0, stream, sc, b, n);
     return hipGetLastError();
 }
+
+// Dispatches configuration `cfg` (an index into kLaneConfigs) to its instantiation.
+template <template <class, int, int, bool> class Launch, class Scenes>
+hipError_t launch_cfg(int cfg, hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n) {
+    switch (cfg) {
+        case 0: return Launch<Scenes, 2, 256, true>::run(stream, sc, b, n);
+        case 1: return Launch<Scenes, 4, 256, true>::run(stream, sc, b, n);
+        case 2: return Launch<Scenes, 2, 128, true>::run(stream, sc, b, n);
+        case 4: return Launch<Scenes, 2, 64, true>::run(stream, sc, b, n);
+        case 5: return Launch<Scenes, 2, 256, false>::run(stream, sc, b, n);
+        case 6: return Launch<Scenes, 4, 256, false>::run(stream, sc, b, n);
+        case 7: return Launch<Scenes, 2, 128, false>::run(stream, sc, b, n);
+        case 8: return Launch<Scenes, 4, 128, false>::run(stream, sc, b, n);
+        case 9: return Launch<Scenes, 2, 64, false>::run(stream, sc, b, n);
+        default: return Launch<Scenes, 4, 128, true>::run(stream, sc, b, n);
+    }
+}
+
+template <class Scenes, int kU, int kB, bool kL>
+struct DensityLaunch {
+    static hipError_t run(hipStream_t st, const Scenes &sc, const ChainBuffers &b, uint32_t n) {
+        return launch_density_t<Scenes, kU, kB, kL>(st, sc, b, n);
+    }
+};
+
+template <class Scenes, int kU, int kB, bool kL>
+struct ForceLaunch {
+    static hipError_t run(hipStream_t st, const Scenes &sc, const ChainBuffers &b, uint32_t n) {
+        return launch_force_t<Scenes, kU, kB, kL>(st, sc, b, n);
+    }
+};
 
 template <class Scenes>
 hipError_t launch_density(hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n,
                           int cfg) {
-    switch (cfg) {
-        case 0: return launch_density_t<Scenes, 1, 256>(stream, sc, b, n);
-        case 2: return launch_density_t<Scenes, 4, 256>(stream, sc, b, n);
-        case 3: return launch_density_t<Scenes, 2, 128>(stream, sc, b, n);
-        case 4: return launch_density_t<Scenes, 4, 128>(stream, sc, b, n);
-        case 5: return launch_density_t<Scenes, 2, 64>(stream, sc, b, n);
-        default: return launch_density_t<Scenes, 2, 256>(stream, sc, b, n);
-    }
+    return launch_cfg<DensityLaunch>(cfg, stream, sc, b, n);
 }
 
 template <class Scenes>
 hipError_t launch_force(hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n, int cfg) {
-    switch (cfg) {
-        case 0: return launch_force_t<Scenes, 1, 256>(stream, sc, b, n);
-        case 2: return launch_force_t<Scenes, 4, 256>(stream, sc, b, n);
-        case 3: return launch_force_t<Scenes, 2, 128>(stream, sc, b, n);
-        case 4: return launch_force_t<Scenes, 4, 128>(stream, sc, b, n);
-        case 5: return launch_force_t<Scenes, 2, 64>(stream, sc, b, n);
-        default: return launch_force_t<Scenes, 2, 256>(stream, sc, b, n);
-    }
+    return launch_cfg<ForceLaunch>(cfg, stream, sc, b, n);
 }
 
 #define SPH_INSTANTIATE(S)                                                                       \

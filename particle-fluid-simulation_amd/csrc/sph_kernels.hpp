@@ -44,10 +44,12 @@ struct ChainBuffers {
     // Cell-ordered layout (sph_layout.hip).
     uint32_t *bcount;       // [bucket_total] particles per bucket, then the bucket's first slot
     uint32_t *blookback;    // [bucket tiles] look-back words of the bucket scan
-    uint32_t *run_off;      // [n] at a run's first hash slot: the run's offset in its bucket
-    uint32_t *run_bucket;   // [n] at a run's first hash slot: the run's bucket
+    uint2 *run_info;        // [table_total] per occupied key: (bucket, offset in the bucket)
     uint32_t *hslot;        // [n] hash slot of each cell-ordered slot
     int2 *grid;             // [grid_total] (start, end) in cell order of each halo-grid cell's run
+    // Force neighbour lists recorded by the per-lane density pass (cell-ordered slots).
+    uint32_t *nlist;        // [kList][n]: entry k of slot j at nlist[k * n + j]
+    uint8_t *ncount;        // [n] entries, or 0xFF: the force pass walks the cells
 };
 
 uint32_t radix_tile_size();
@@ -89,13 +91,16 @@ hipError_t launch_place(hipStream_t stream, const Scenes &sc, const ChainBuffers
                         uint32_t grid_total);
 
 // Launch configurations of the one-lane-per-particle density / force kernels: candidates
-// loaded per scan iteration and workgroup size.  The autotuner picks one per size bucket.
+// loaded per scan step, workgroup size, and whether the density pass records force
+// neighbour lists (worth it unless lists overflow, i.e. in very dense scenes).  The
+// autotuner picks one per size bucket.
 struct LaneConfig {
     int unroll;
     int block;
+    bool lists;
 };
-constexpr int kNumLaneConfigs = 6;
-constexpr int kDefaultLaneConfig = 1;
+constexpr int kNumLaneConfigs = 10;
+constexpr int kDefaultLaneConfig = 3;
 extern const LaneConfig kLaneConfigs[kNumLaneConfigs];
 
 // Density over the cell-ordered layout.

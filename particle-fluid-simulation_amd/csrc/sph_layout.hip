@@ -47,7 +47,7 @@ template <class Scenes>
 __global__ __launch_bounds__(kBlock) void k_runs(
     Scenes sc, const uint32_t *__restrict__ skeys, const uint32_t *__restrict__ sidx, uint32_t n,
     const float4 *__restrict__ pos, const int2 *__restrict__ table, uint32_t *__restrict__ bcount,
-    uint32_t *__restrict__ run_off, uint32_t *__restrict__ run_bucket) {
+    uint2 *__restrict__ run_info) {
     const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
     if (j >= n) return;
     const uint32_t key = skeys[j];
@@ -57,8 +57,7 @@ __global__ __launch_bounds__(kBlock) void k_runs(
     const uint32_t bucket = bucket_of(c, cell_of(P.x, c.minx, c.R), cell_of(P.y, c.miny, c.R),
                                       cell_of(P.z, c.minz, c.R));
     const uint32_t len = (uint32_t)table[key].y - j;
-    run_off[j] = atomicAdd(&bcount[bucket], len);
-    run_bucket[j] = bucket;
+    run_info[key] = make_uint2(bucket, atomicAdd(&bcount[bucket], len));
 }
 
 // Exclusive scan of bcount[0, count) in place.  Tiles are taken in launch order from an
@@ -130,8 +129,8 @@ __global__ __launch_bounds__(kBlock) void k_place(
     Scenes sc, const uint32_t *__restrict__ skeys, const uint32_t *__restrict__ sidx, uint32_t n,
     uint32_t place_blocks, uint32_t grid_total, const float4 *__restrict__ pos,
     const float4 *__restrict__ vel, const int2 *__restrict__ table,
-    const uint32_t *__restrict__ bstart, const uint32_t *__restrict__ run_off,
-    const uint32_t *__restrict__ run_bucket, float4 *__restrict__ pos_sorted,
+    const uint32_t *__restrict__ bstart, const uint2 *__restrict__ run_info,
+    float4 *__restrict__ pos_sorted,
     float4 *__restrict__ vel_sorted, float4 *__restrict__ pred_pos,
     float4 *__restrict__ dbg_pred_vel, uint32_t *__restrict__ hslot, int2 *__restrict__ grid) {
     if (blockIdx.x >= place_blocks) {
@@ -144,10 +143,10 @@ __global__ __launch_bounds__(kBlock) void k_place(
         // cell (lx-1, ly-1, lz-1); -1 wraps to 2^32-1 as in the reference's u32 hash
         const uint32_t key = c.table_base + (cell_hash(lx - 1u, ly - 1u, lz - 1u) & c.mask);
         const int2 t = table[key];
+        const uint2 ri = run_info[key];
         int2 out = make_int2(-1, -1);
         if (t.x >= 0) {
-            const uint32_t h = (uint32_t)t.x;
-            const int first = (int)(bstart[run_bucket[h]] + run_off[h]);
+            const int first = (int)(bstart[ri.x] + ri.y);
             out = make_int2(first, first + (t.y - t.x));
         }
         grid[g] = out;
@@ -157,7 +156,8 @@ __global__ __launch_bounds__(kBlock) void k_place(
     if (j >= n) return;
     const uint32_t key = skeys[j];
     const uint32_t head = (uint32_t)table[key].x;
-    const uint32_t j2 = bstart[run_bucket[head]] + run_off[head] + (j - head);
+    const uint2 ri = run_info[key];
+    const uint32_t j2 = bstart[ri.x] + ri.y + (j - head);
     const uint32_t idx = sidx ? sidx[j] : j;
     const float4 P = pos[idx];
     const float4 V = vel[idx];
@@ -179,7 +179,7 @@ template <class Scenes>
 hipError_t launch_runs(hipStream_t stream, const Scenes &sc, const ChainBuffers &b,
                        const uint32_t *sorted_keys, const uint32_t *sorted_idx, uint32_t n) {
     hipLaunchKernelGGL(k_runs<Scenes>, dim3(blocks_for(n)), dim3(kBlock), 0, stream, sc, sorted_keys,
-                       sorted_idx, n, b.pos, b.table, b.bcount, b.run_off, b.run_bucket);
+                       sorted_idx, n, b.pos, b.table, b.bcount, b.run_info);
     return hipGetLastError();
 }
 
@@ -197,7 +197,7 @@ hipError_t launch_place(hipStream_t stream, const Scenes &sc, const ChainBuffers
     const uint32_t place_blocks = blocks_for(n);
     hipLaunchKernelGGL(k_place<Scenes>, dim3(place_blocks + blocks_for(grid_total)), dim3(kBlock), 0,
                        stream, sc, sorted_keys, sorted_idx, n, place_blocks, grid_total, b.pos, b.vel,
-                       b.table, b.bcount, b.run_off, b.run_bucket, b.pos_sorted, b.vel_sorted,
+                       b.table, b.bcount, b.run_info, b.pos_sorted, b.vel_sorted,
                        b.pred_pos, b.dbg_pred_vel, b.hslot, b.grid);
     return hipGetLastError();
 }

@@ -100,8 +100,7 @@ struct GridRuns {
     uint32_t table_base, mask;
     const int2 *__restrict__ table;
     const uint32_t *__restrict__ bstart;
-    const uint32_t *__restrict__ run_off;
-    const uint32_t *__restrict__ run_bucket;
+    const uint2 *__restrict__ run_info;
     __device__ __forceinline__ RunRef get(int cx, int cy, int cz) const {
         const uint32_t lx = (uint32_t)(cx + 1), ly = (uint32_t)(cy + 1), lz = (uint32_t)(cz + 1);
         if (lx < hx && ly < hy && lz < hz) {
@@ -109,10 +108,11 @@ struct GridRuns {
             if (se.x < 0) return RunRef{0, 0, 0, true};
             return RunRef{se.x, se.y - se.x, se.x, true};
         }
-        const int2 se = table[table_base + (cell_hash((uint32_t)cx, (uint32_t)cy, (uint32_t)cz) & mask)];
+        const uint32_t key = table_base + (cell_hash((uint32_t)cx, (uint32_t)cy, (uint32_t)cz) & mask);
+        const int2 se = table[key];
+        const uint2 ri = run_info[key];  // independent of se: both loads are in flight together
         if (se.x == -1) return RunRef{0, 0, 0, true};
-        const uint32_t h = (uint32_t)se.x;
-        const int first = (int)(bstart[run_bucket[h]] + run_off[h]);
+        const int first = (int)(bstart[ri.x] + ri.y);
         return RunRef{first, se.y - se.x, first, true};
     }
 };
@@ -297,6 +297,85 @@ __device__ __forceinline__ void force_walk(const SceneConst &c, const ForceSelf 
     drain();
 }
 
+// ---------------------------------------------------------------------------------------
+// Walks of the per-lane kernels (runs in global memory).  The walks are bound by the
+// latency of dependent loads, so they are arranged for loads in flight: the three run
+// lookups of a (dz, dy) row are issued together, and every scan step loads kU candidates
+// (indices clamped to the run, extra lanes masked) so that a run of up to kU candidates
+// costs one round trip.  Sums still take the candidates one by one in run order.
+
+__device__ __forceinline__ float4 load_xyz(const float4 *p) {
+    const float3 v = *reinterpret_cast<const float3 *>(p);
+    return make_float4(v.x, v.y, v.z, 0.0f);
+}
+
+template <int kU, class Cells>
+__device__ __forceinline__ float2 density_walk_rows(const SceneConst &c, const float4 &s,
+                                                    const float4 *__restrict__ pred_pos,
+                                                    const Cells &cells) {
+    float M = 0.0f, Mn = 0.0f;
+    for (int dz = 0; dz < 3; ++dz)
+        for (int dy = 0; dy < 3; ++dy) {
+            RunRef r[3];
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) r[dx] = cells(dz, dy, dx);
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                const float4 *src = pred_pos + r[dx].base;
+                const int cnt = r[dx].cnt;
+                for (int k = 0; k < cnt; k += kU) {
+                    float4 p[kU];
+#pragma unroll
+                    for (int u = 0; u < kU; ++u) p[u] = load_xyz(src + min(k + u, cnt - 1));
+#pragma unroll
+                    for (int u = 0; u < kU; ++u)
+                        if (k + u < cnt) density_accumulate(c, dist2(p[u], s), M, Mn);
+                }
+            }
+        }
+    return density_finish(c, M, Mn);
+}
+
+template <int kU, class Cells>
+__device__ __forceinline__ void force_walk_rows(const SceneConst &c, const ForceSelf &me,
+                                                const float4 *__restrict__ pred_pos,
+                                                const float4 *__restrict__ vel_sorted,
+                                                const float2 *__restrict__ dens, const Cells &cells,
+                                                uint32_t *q, int stride, float3 &Fp, float3 &Fv) {
+    int qc = 0;
+    auto drain = [&]() {
+        for (int i = 0; i < qc; ++i)
+            force_term<false>(c, me, q[i * stride], pred_pos, vel_sorted, dens, nullptr, nullptr, Fp, Fv);
+        qc = 0;
+    };
+    for (int dz = 0; dz < 3; ++dz)
+        for (int dy = 0; dy < 3; ++dy) {
+            RunRef r[3];
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) r[dx] = cells(dz, dy, dx);
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                const float4 *src = pred_pos + r[dx].base;
+                const int cnt = r[dx].cnt;
+                for (int k = 0; k < cnt; k += kU) {
+                    float4 p[kU];
+#pragma unroll
+                    for (int u = 0; u < kU; ++u) p[u] = load_xyz(src + min(k + u, cnt - 1));
+#pragma unroll
+                    for (int u = 0; u < kU; ++u) {
+                        if (k + u >= cnt) continue;
+                        if ((uint32_t)(r[dx].first + k + u) == me.j) continue;
+                        if (dist2(p[u], me.s) > c.R2) continue;
+                        q[qc * stride] = kGlobalBit | (uint32_t)(r[dx].base + k + u);
+                        if (++qc == kQueue) drain();
+                    }
+                }
+                if (__ballot(qc >= kFlushAt)) drain();
+            }
+        }
+    drain();
+}
+
 // Self terms of the force pass for slot j (applyPressureAndViscosityForce.comp:276-286).
 __device__ __forceinline__ ForceSelf force_self(const SceneConst &c, uint32_t j, const float4 &s,
                                                 const float4 &P, const float4 &V,
@@ -335,5 +414,81 @@ __device__ __forceinline__ void force_finish(const SceneConst &c, const float4 &
     vel_out = make_float4(v.x, v.y, v.z, 1.0f);
 }
 
+
+// ---------------------------------------------------------------------------------------
+// Force neighbour lists, recorded by the density pass.
+//
+// The force pass visits the cells c0 + (dx, dy, dz) of the predicted position's cell c0
+// (applyPressure...comp:283-292); the density pass visits the cells of the sample points
+// s + R*(dx, dy, dz), skipping sample points outside the box (calculateDensities.comp:
+// 114-118).  Per axis the middle sample point is s itself, whose cell is c0, and the other
+// two normally fall in c0 -/+ 1.  When that holds on every axis for every sample point
+// inside the box, the density's cells are exactly the force cells whose sample points are
+// inside, in the same (dz, dy, dx) order, so one walk over the force cells yields both:
+// density terms from the member cells, and the force candidates (slot != j, d2 <= R*R) in
+// the force pass's visit order.  The first kListLds candidates are appended to a per-lane
+// list in LDS and copied out entry-major (entry k of slot j at list[k * n + j]: one
+// coalesced row per entry), the few beyond are stored directly; the force pass then reads
+// its list instead of scanning 27 cells.  A particle for
+// which the cell relation fails (a rounded sample cell that is not c0 -/+ 1, a predicted
+// position outside the box), or whose list would exceed kList entries, gets the density's
+// own walk and an invalid count, and the force pass walks its cells.
+constexpr int kList = 64;      // entries per particle in the global list
+constexpr int kListLds = 16;   // entries staged in LDS (the rest are stored directly)
+constexpr uint8_t kListInvalid = 0xFF;
+
+template <int kU>
+__device__ __forceinline__ float2 density_with_list(const SceneConst &c, uint32_t j, const float4 &s,
+                                                    const float4 *__restrict__ pred_pos,
+                                                    const GridRuns &runs, uint32_t *lane_list,
+                                                    int stride, uint32_t *__restrict__ nlist,
+                                                    uint32_t n, uint32_t &count) {
+    const AxisCells ax = sample_axis(s.x, c.minx, c.maxx, c.R);
+    const AxisCells ay = sample_axis(s.y, c.miny, c.maxy, c.R);
+    const AxisCells az = sample_axis(s.z, c.minz, c.maxz, c.R);
+    const int3 c0 = force_cell(c, s);
+    auto same = [](const AxisCells &a) {
+        return a.in(1) && (!a.in(0) || a.lo_off == 0xFFFFFFFFu) && (!a.in(2) || a.hi_off == 1u);
+    };
+    if (!(same(ax) && same(ay) && same(az) && (int32_t)ax.mid == c0.x && (int32_t)ay.mid == c0.y &&
+          (int32_t)az.mid == c0.z)) {
+        count = kListInvalid;
+        return density_walk_rows<kU>(c, s, pred_pos, DensityCellsOf<GridRuns>{runs, ax, ay, az});
+    }
+    const float R2 = c.R2;
+    float M = 0.0f, Mn = 0.0f;
+    uint32_t nl = 0;
+    for (int dz = 0; dz < 3; ++dz)
+        for (int dy = 0; dy < 3; ++dy) {
+            RunRef r[3];
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) r[dx] = runs.get(c0.x + dx - 1, c0.y + dy - 1, c0.z + dz - 1);
+#pragma unroll
+            for (int dx = 0; dx < 3; ++dx) {
+                const bool member = ax.in(dx) && ay.in(dy) && az.in(dz);
+                const float4 *src = pred_pos + r[dx].base;
+                const int cnt = r[dx].cnt;
+                for (int k = 0; k < cnt; k += kU) {
+                    float4 p[kU];
+#pragma unroll
+                    for (int u = 0; u < kU; ++u) p[u] = load_xyz(src + min(k + u, cnt - 1));
+#pragma unroll
+                    for (int u = 0; u < kU; ++u) {
+                        if (k + u >= cnt) continue;
+                        const float d2 = dist2(p[u], s);
+                        if (member) density_accumulate(c, d2, M, Mn);
+                        const uint32_t slot = (uint32_t)(r[dx].base + k + u);
+                        if (slot != j && !(d2 > R2)) {
+                            if (nl < (uint32_t)kListLds) lane_list[nl * stride] = slot;
+                            else if (nl < (uint32_t)kList) nlist[(size_t)nl * n + j] = slot;
+                            ++nl;
+                        }
+                    }
+                }
+            }
+        }
+    count = nl <= (uint32_t)kList ? nl : kListInvalid;
+    return density_finish(c, M, Mn);
+}
 
 }  // namespace sph

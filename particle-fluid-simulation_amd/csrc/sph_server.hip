@@ -96,7 +96,7 @@ namespace {
 
 // Device bytes a resident scene occupies (the chain's per-slot buffers and its table).
 uint64_t scene_footprint(uint32_t n, uint32_t T, uint32_t grid_cells) {
-    return (uint64_t)n * 136u + (uint64_t)T * 8u + (uint64_t)grid_cells * 8u;
+    return (uint64_t)n * 392u + (uint64_t)T * 16u + (uint64_t)grid_cells * 8u;
 }
 
 bool fits(const sph_server *s, const Request *r) {
@@ -117,8 +117,10 @@ int repack(sph_server *s, const std::vector<uint32_t> &old_begin) {
     for (size_t i = 0; i < count; ++i) {
         Request *r = s->resident[i];
         SceneConst c = make_scene_const(r->params, r->T, tb, pb, pb + r->n, bb, ub, gb);
-        // <= 2n + 1024 buckets and 4n + 4096 grid cells per scene: within the capacity
+        // <= 2n + 1024 buckets per scene: within the capacity; a halo grid that does not
+        // fit what is left of the grid capacity is dropped (hash-table lookups instead)
         ub += scene_buckets(c);
+        if ((uint64_t)gb + c.grid_cells > s->chain.g_cap) c.grid_cells = 0u;
         gb += c.grid_cells;
         if (s->cfg_bricks && r->bricks) bb += (uint32_t)r->bricks;
         else c.use_bricks = 0u;
@@ -299,8 +301,9 @@ int sph_server_create(const sph_server_config *cfg, sph_server **out) {
         return hip_fail(e, "stream creation");
     }
     const uint64_t bucket_cap = 2ull * cfg->max_particles + 1024ull * cfg->max_scenes;
-    const uint64_t grid_cap = 4ull * cfg->max_particles + 4096ull * cfg->max_scenes;
-    if (grid_cap >= (1ull << 31)) {
+    const uint64_t grid_want = 16ull * cfg->max_particles + 4096ull * cfg->max_scenes;
+    const uint64_t grid_cap = grid_want < (1ull << 30) ? grid_want : (1ull << 30);
+    if (bucket_cap >= (1ull << 32)) {
         hipStreamDestroy(s->stream);
         delete s;
         return fail(SPH_ERR_CAPACITY, "max_particles / max_scenes too large");

@@ -18,6 +18,7 @@
 #include "sph_map.hpp"
 #include "sph_host.hpp"
 #include "sph_kernels.hpp"
+#include "sph_pairs.hpp"
 
 namespace sph {
 
@@ -97,10 +98,11 @@ SceneConst make_scene_const(const sph_params &p, uint32_t table_cells, uint32_t 
     c.bgx = (((uint32_t)c.gx - 1u) >> sh) + 1u;
     c.bgy = (((uint32_t)c.gy - 1u) >> sh) + 1u;
     c.bucket_base = bucket_base;
-    // Halo grid when it has at most 4n + 4096 cells.
+    // Halo grid when it has at most 16n + 4096 cells (filling a cell costs about what a
+    // particle's 27 hashed lookups would, so a grid pays up to ~20 cells per particle).
     const double halo = ((double)c.gx + 2.0) * ((double)c.gy + 2.0) * ((double)c.gz + 2.0);
     c.grid_base = grid_base;
-    c.grid_cells = halo <= 4.0 * (double)(p_end - p_begin) + 4096.0 ? (uint32_t)halo : 0u;
+    c.grid_cells = halo <= 16.0 * (double)(p_end - p_begin) + 4096.0 ? (uint32_t)halo : 0u;
     return c;
 }
 
@@ -192,8 +194,9 @@ int Chain::allocate(uint32_t n_, uint32_t table_total_, uint32_t brick_total_,
         {(void **)&b.tile_ctr, kTileCounters * 4}, {(void **)&b.hfull, N * 4}, {(void **)&b.stray, N * 4},
         {(void **)&b.bcount, (size_t)bucket_total * 4},
         {(void **)&b.blookback, (size_t)((bucket_total + kBucketTile - 1u) / kBucketTile) * 4},
-        {(void **)&b.run_off, N * 4}, {(void **)&b.run_bucket, N * 4}, {(void **)&b.hslot, N * 4},
+        {(void **)&b.run_info, (size_t)table_total * 8}, {(void **)&b.hslot, N * 4},
         {(void **)&b.grid, (size_t)(grid_total ? grid_total : 1) * 8},
+        {(void **)&b.nlist, N * kList * 4}, {(void **)&b.ncount, N},
         {(void **)&b.counters, 8 * 4},
         {(void **)&b.brick_flag, (size_t)(brick_total ? brick_total : 1) * 4},
         {(void **)&b.queue, (size_t)(brick_total ? brick_total : 1) * 4},
@@ -236,7 +239,7 @@ void Chain::release() {
                     b.pred_pos, b.pos_sorted, b.vel_sorted, b.dens, b.table, b.ghist,
                     b.lookback, b.tile_ctr, b.dbg_pred_vel, b.dbg_vel_force, b.hfull,
                     b.brick_flag, b.queue, b.stray, b.counters, b.bcount, b.blookback,
-                    b.run_off, b.run_bucket, b.hslot, b.grid};
+                    b.run_info, b.hslot, b.grid, b.nlist, b.ncount};
     for (void *p : ptrs)
         if (p) hipFree(p);
     b = ChainBuffers{};
@@ -451,7 +454,11 @@ int world_resolve_variant(sph_world *w) {
     }
     if (code < 0) {
         const char *off = std::getenv("SPH_AUTOTUNE");
-        if (off && std::strcmp(off, "0") == 0) {
+        const char *pin = std::getenv("SPH_LANE_CONFIG");
+        const int pinned = pin ? std::atoi(pin) : -1;
+        if (pin && *pin >= '0' && *pin <= '9' && pinned >= 0 && pinned < kNumLaneConfigs) {
+            code = pinned;  // a fixed per-lane launch configuration (profiling)
+        } else if (off && std::strcmp(off, "0") == 0) {
             code = kDefaultLaneConfig;
         } else {
             int rc = world_autotune(w, 3, &code, nullptr);
@@ -481,9 +488,9 @@ uint64_t sph_world_footprint(uint32_t n, uint32_t table_cells) {
     const uint64_t tiles = ((uint64_t)n + radix_tile_size() - 1) / radix_tile_size();
     // state, keys, 2 x radix ping-pong, 3 sorted float4 arrays, densities, brick and
     // cell-order slot arrays; the cell table; buckets (at most 2n + 1024), the halo grid
-    // (at most 4n + 4096 cells) and look-back words
-    return (uint64_t)n * (16 * 2 + 4 + 16 + 16 * 3 + 8 + 8 + 12) + T * 8 + (2 * (uint64_t)n + 1024) * 4 +
-           (4 * (uint64_t)n + 4096) * 8 + 4 * tiles * 256 * 4 + 4096;
+    // (at most 16n + 4096 cells) and look-back words
+    return (uint64_t)n * (16 * 2 + 4 + 16 + 16 * 3 + 8 + 8 + 4 + 4 * kList + 1) + T * 16 +
+           (2 * (uint64_t)n + 1024) * 4 + (16 * (uint64_t)n + 4096) * 8 + 4 * tiles * 256 * 4 + 4096;
 }
 
 int sph_world_create(const sph_params *params, uint32_t n, sph_world **out) {
@@ -625,6 +632,20 @@ int sph_world_autotune(sph_world *w, uint32_t substeps, int *chosen, double *ms_
     return SPH_OK;
 }
 
+int sph_world_set_lane_config(sph_world *w, int config) {
+    if (!w) return fail(SPH_ERR_INVALID_ARG, "world is NULL");
+    if (config < 0 || config >= kNumLaneConfigs)
+        return fail(SPH_ERR_INVALID_ARG, "lane configuration %d outside [0, %d)", config, kNumLaneConfigs);
+    hipError_t e = hipStreamSynchronize(w->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    w->variant = SPH_KERNEL_DIRECT;
+    w->tuned = true;
+    world_apply_code(w, config);
+    return SPH_OK;
+}
+
+int sph_lane_configs(void) { return kNumLaneConfigs; }
+
 int sph_world_kernel(const sph_world *w) {
     if (!w) return fail(SPH_ERR_INVALID_ARG, "world is NULL");
     return w->chain.bricks ? SPH_KERNEL_BRICK : SPH_KERNEL_DIRECT;
@@ -635,8 +656,9 @@ int sph_world_kernel_config(const sph_world *w, char *buf, size_t bytes) {
     if (w->chain.bricks)
         snprintf(buf, bytes, "brick %dx%dx%d cells, persistent queue", kBrick, kBrick, kBrick);
     else
-        snprintf(buf, bytes, "per-lane, %d candidates per scan step, %d-lane workgroups",
-                 kLaneConfigs[w->chain.lane_cfg].unroll, kLaneConfigs[w->chain.lane_cfg].block);
+        snprintf(buf, bytes, "per-lane, %d candidates per scan step, %d-lane workgroups, %s",
+                 kLaneConfigs[w->chain.lane_cfg].unroll, kLaneConfigs[w->chain.lane_cfg].block,
+                 kLaneConfigs[w->chain.lane_cfg].lists ? "force neighbour lists" : "force cell walk");
     return SPH_OK;
 }
 

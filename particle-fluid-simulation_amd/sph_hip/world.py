@@ -97,6 +97,14 @@ class World:
         check(self._lib.sph_world_kernel_config(self._h, buf, 256))
         return buf.value.decode()
 
+    def set_lane_config(self, config: int) -> None:
+        """DIRECT with one fixed per-lane launch configuration (see lane_configs())."""
+        check(self._lib.sph_world_set_lane_config(self._h, config))
+
+    @staticmethod
+    def lane_configs() -> int:
+        return int(load_library().sph_lane_configs())
+
     def set_debug(self, on: bool) -> None:
         check(self._lib.sph_world_set_debug(self._h, 1 if on else 0))
 

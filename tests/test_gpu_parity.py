@@ -105,6 +105,25 @@ def test_dam_break_multi_frame_bitexact(gpu, orc, kernel):
     _assert_bitexact("vel", gv, rv)
 
 
+@pytest.mark.parametrize("config", range(10))
+def test_dam_break_every_lane_config(gpu, orc, config):
+    """Every per-lane launch configuration (scan unroll x workgroup size, with and without
+    force neighbour lists) gives the oracle's trajectory bit for bit over 3 frames."""
+    from sph_hip.workloads import DAM_BREAK_16K as W
+    assert gpu.World.lane_configs() == 10
+    params = W.params()
+    pos, vel = W.state()
+    with gpu.World(params, W.n) as w:
+        w.set_lane_config(config)
+        print(f"lane config {config}: {w.kernel_config}", flush=True)
+        w.upload(pos, vel)
+        w.step(3)
+        gp, gv = w.download()
+    rp, rv = orc.step(params, pos, vel, 3)
+    _assert_bitexact("pos", gp, rp)
+    _assert_bitexact("vel", gv, rv)
+
+
 def test_tiny_box_strays(gpu, orc):
     """A box narrower than the integrate margin (min + r + 0.5 > max) pushes particles
     outside the cell grid on the first substep; the second substep must still process
