@@ -35,11 +35,19 @@ def alg_bytes_per_particle_substep(n: int, table_cells: int) -> float:
     return 200.0 + 16.0 * passes + 8.0 * table_cells / n
 
 
-# Algorithmic bytes per particle of the force + integrate kernel: own predicted position,
-# position and velocity (3 x 16 B float4), own (density, near density) (8 B), new position
-# and velocity (2 x 16 B).  Neighbour reads are reuse of data another particle already
-# counts, not algorithmic traffic.
-FORCE_BYTES_PER_PARTICLE = 3 * 16 + 8 + 2 * 16
+# Algorithmic bytes per particle of the two neighbour kernels, from the whole-chain
+# formula's per-pass terms: density reads the sorted position and velocity (24 B) and
+# writes (density, near density) (8 B); force + integrate reads the same 24 B and the
+# densities (8 B) and writes the new position and velocity (32 B).  Neighbour reads are
+# reuse of data another particle already counts, not algorithmic traffic.
+KERNEL_BYTES_PER_PARTICLE = {"density": 24 + 8, "force": 24 + 8 + 32}
+KERNEL_NAMES = {"density": "k_density (calculateDensities + force neighbour lists)",
+                "force": "k_force (applyPressureAndViscosityForce + updateVelocities)"}
+
+# HBM bytes per launch of the neighbour kernels on uniform_1m (T = 2^21) measured with
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (profiles/pmc_uniform_1m_neighbour_passes.md),
+# each counter scaled by what it reads for a 64 MiB device-to-device copy in the same run.
+TRAFFIC_BYTES_PER_LAUNCH = {"density": None, "force": None}
 
 
 def cpu_baseline(workload, table_cells: int, budget_s: float = 12.0) -> dict:
@@ -237,11 +245,15 @@ def main() -> int:
     world = sph_hip.World(params, W.n)
     world.upload(pos, vel)
     print(f"[bench] {W.name}: N={W.n} T={T} substeps/step={W.substeps}", file=sys.stderr, flush=True)
+    # W untimed warmup steps; the autotuner runs on the state after the first W - 1 (it
+    # restores that state afterwards), and the last warmup step builds the tuned graph.
+    world.step(max(0, args.warmup - 1))
+    world.sync()
     chosen, tuned_ms = world.autotune(5)
     chosen = world.kernel_config
     print(f"[bench] autotune: {chosen}, {tuned_ms:.3f} ms/substep", file=sys.stderr, flush=True)
-
-    world.step(args.warmup)
+    if args.warmup > 0:
+        world.step(1)
     world.sync()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -253,14 +265,21 @@ def main() -> int:
     value = W.n * args.steps / elapsed
     print(f"[bench] timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
 
-    # Dominant kernel, timed with HIP events on the library's stream.
+    # Dominant kernel, timed with HIP events recorded on the library's stream around every
+    # launch of 30 more substeps continuing the timed run.
     pt = world.pass_times(30)
     variants = {}
     for v in ("brick", "direct"):  # direct last: keeps the tuned per-lane configuration
         world.set_kernel(v)
         variants[v] = world.pass_times(10)
-    force_s = pt["force_ms"] * 1e-3
-    force_gbs = W.n * FORCE_BYTES_PER_PARTICLE / force_s / 1e9
+    dom = "density" if pt["density_ms"] > pt["force_ms"] else "force"
+    dom_s = pt[f"{dom}_ms"] * 1e-3
+    dom_bytes = W.n * KERNEL_BYTES_PER_PARTICLE[dom]
+    dom_gbs = dom_bytes / dom_s / 1e9
+    # a 64 MiB device-to-device copy: the calibration the HBM byte counters are scaled by
+    calib_src = torch.ones(16 << 20, dtype=torch.float32, device="cuda")
+    torch.empty_like(calib_src).copy_(calib_src)
+    torch.cuda.synchronize()
     b_alg = alg_bytes_per_particle_substep(W.n, T)
     substep_s = ms_per_step * 1e-3 / W.substeps
     chain_gbs = W.n * b_alg / substep_s / 1e9
@@ -298,11 +317,11 @@ def main() -> int:
         "config": {"workload": W.name, "particles": W.n, "table_cells": T,
                    "substeps_per_step": W.substeps, "box": list(W.box_max),
                    "smoothing_radius": W.smoothing_radius, "path": "plain (one scene per launch)"},
-        "roofline": {"bound": "hbm", "achieved": force_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": force_gbs / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_force (force + integrate)",
-                     "avg_launch_ms": pt["force_ms"],
-                     "alg_bytes_per_launch": W.n * FORCE_BYTES_PER_PARTICLE},
+        "roofline": {"bound": "hbm", "achieved": dom_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": dom_gbs / HBM_PEAK_GBS,
+                     "traffic": TRAFFIC_BYTES_PER_LAUNCH[dom] if T == 1 << 21 and W is wl.UNIFORM_1M else None,
+                     "kernel": KERNEL_NAMES[dom], "avg_launch_ms": pt[f"{dom}_ms"],
+                     "alg_bytes_per_launch": dom_bytes},
         "chain_roofline": {"alg_bytes_per_particle_substep": b_alg, "achieved_gbs": chain_gbs,
                            "frac": chain_gbs / HBM_PEAK_GBS},
         "kernels": chosen,

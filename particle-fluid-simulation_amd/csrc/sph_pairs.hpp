@@ -434,7 +434,7 @@ __device__ __forceinline__ void force_finish(const SceneConst &c, const float4 &
 // position outside the box), or whose list would exceed kList entries, gets the density's
 // own walk and an invalid count, and the force pass walks its cells.
 constexpr int kList = 64;      // entries per particle in the global list
-constexpr int kListLds = 16;   // entries staged in LDS (the rest are stored directly)
+constexpr int kListLds = 24;   // entries staged in LDS (the rest are stored directly)
 constexpr uint8_t kListInvalid = 0xFF;
 
 template <int kU>

@@ -103,3 +103,10 @@ def test_server_rejects_bad_config_without_a_device():
     assert lib.sph_server_create(C.byref(cfg), C.byref(h)) == -1
     cfg = ServerConfig(1 << 30, 4, 1024, 1024, 0, 0)
     assert lib.sph_server_create(C.byref(cfg), C.byref(h)) == -1
+
+
+def test_lane_configs_without_a_device():
+    import sph_hip
+    lib = sph_hip.load_library()
+    assert lib.sph_lane_configs() == 10
+    assert lib.sph_world_set_lane_config(None, 0) == -1
