@@ -219,9 +219,9 @@ template <class Scenes>
 hipError_t launch_hash(hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n,
                        uint32_t table_total, uint32_t passes, uint32_t lookback_words,
                        uint32_t bucket_total) {
-    // ~8K keys per hashing workgroup keeps the global histogram atomics few; the clearing
-    // workgroups run beside them.
-    const uint32_t hash_blocks = (n + 8191u) / 8192u;
+    // 2K keys per hashing workgroup: enough workgroups to fill the chip, few enough global
+    // histogram atomics; the clearing workgroups run beside them.
+    const uint32_t hash_blocks = (n + 2047u) / 2048u;
     const uint32_t btiles = (bucket_total + kBucketTile - 1u) / kBucketTile;
     uint32_t clear_work = table_total > lookback_words ? table_total : lookback_words;
     if (bucket_total > clear_work) clear_work = bucket_total;
