@@ -433,7 +433,7 @@ __device__ __forceinline__ void force_finish(const SceneConst &c, const float4 &
 // which the cell relation fails (a rounded sample cell that is not c0 -/+ 1, a predicted
 // position outside the box), or whose list would exceed kList entries, gets the density's
 // own walk and an invalid count, and the force pass walks its cells.
-constexpr int kList = 64;      // entries per particle in the global list
+constexpr int kList = 128;     // entries per particle in the global list
 constexpr int kListLds = 24;   // entries staged in LDS (the rest are stored directly)
 constexpr uint8_t kListInvalid = 0xFF;
 
