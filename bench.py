@@ -80,7 +80,9 @@ def cpu_baseline(workload, table_cells: int, budget_s: float = 12.0) -> dict:
 def mixed_batch(steps: int, warmup: int, count: int = 256) -> dict:
     """Batched vs plain on the mixed workload: `count` scenes (log-uniform 2,048-65,536
     particles, boxes U[4,16]^3, target density U[40,70], seeds 3+s) stepped together in one
-    batch, then the same scenes as one world each, stepped one after another."""
+    batch, then the same scenes as one world each, stepped one after another (each world's
+    steps are enqueued on its own stream, so the runtime may overlap them).  Both are tuned
+    after `warmup` + 1 untimed frames."""
     import torch
     import sph_hip
     from sph_hip.workloads import mixed_scenes
@@ -95,6 +97,8 @@ def mixed_batch(steps: int, warmup: int, count: int = 256) -> dict:
     for i, (p, v) in enumerate(states):
         b.upload(i, p, v)
     b.step(warmup)
+    b.set_kernel("auto")  # re-tune on the warmed-up state, like the single worlds below
+    b.step(1)
     b.sync()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -110,6 +114,8 @@ def mixed_batch(steps: int, warmup: int, count: int = 256) -> dict:
         w = sph_hip.World(params[i], counts[i])
         w.upload(p, v)
         w.step(warmup)
+        w.autotune(3)  # tuned on the warmed-up state (restored afterwards)
+        w.step(1)
         worlds.append(w)
     for w in worlds:
         w.sync()

@@ -196,9 +196,7 @@ __global__ __launch_bounds__(kB) void k_force(Scenes sc, const ChainBuffers b, u
     const uint32_t cnt = kL ? b.ncount[j] : kListInvalid;
     if (cnt != kListInvalid) {
         // the recorded neighbours, in the walk's order
-        for (uint32_t i = 0; i < cnt; ++i)
-            force_term<false>(c, me, kGlobalBit | b.nlist[(size_t)i * n + j], b.pred_pos, b.vel_sorted,
-                              b.dens, nullptr, nullptr, Fp, Fv);
+        force_list<8>(c, me, cnt, b.nlist, n, b.pred_pos, b.vel_sorted, b.dens, Fp, Fv);
     } else {
         const ForceCellsOf<GridRuns> cells{grid_runs(c, b), force_cell(c, s)};
         force_walk_rows<kU>(c, me, b.pred_pos, b.vel_sorted, b.dens, cells, s_queue + threadIdx.x, kB, Fp, Fv);

@@ -199,29 +199,10 @@ __device__ __forceinline__ float2 density_walk(const SceneConst &c, const float4
     return density_finish(c, M, Mn);
 }
 
-// Full arithmetic of one force candidate (applyPressureAndViscosityForce.comp:303-321).
-// Staged neighbours are packed as s_a = (predicted position xyz, near density) and
-// s_b = (velocity xyz, density).
-template <bool kLds>
-__device__ __forceinline__ void force_term(const SceneConst &c, const ForceSelf &me, uint32_t e,
-                                           const float4 *__restrict__ pred_pos,
-                                           const float4 *__restrict__ vel_sorted,
-                                           const float2 *__restrict__ dens, const float4 *s_a,
-                                           const float4 *s_b, float3 &Fp, float3 &Fv) {
-    float3 pi, vi;
-    float2 di;
-    if (!kLds || (e & kGlobalBit)) {
-        const uint32_t g = e & ~kGlobalBit;
-        const float4 a = pred_pos[g], b = vel_sorted[g];
-        pi = make_float3(a.x, a.y, a.z);
-        vi = make_float3(b.x, b.y, b.z);
-        di = dens[g];
-    } else {
-        const float4 a = s_a[e], b = s_b[e];
-        pi = make_float3(a.x, a.y, a.z);
-        vi = make_float3(b.x, b.y, b.z);
-        di = make_float2(b.w, a.w);
-    }
+// Full arithmetic of one force candidate (applyPressureAndViscosityForce.comp:303-321) at
+// predicted position pi with velocity vi and (density, near density) di.
+__device__ __forceinline__ void force_pair(const SceneConst &c, const ForceSelf &me, const float3 &pi,
+                                           const float3 &vi, const float2 &di, float3 &Fp, float3 &Fv) {
     const float ex = pi.x - me.s.x, ey = pi.y - me.s.y, ez = pi.z - me.s.z;
     const float d2 = ex * ex + ey * ey + ez * ez;
     const float dist = sqrtf(d2);
@@ -245,6 +226,54 @@ __device__ __forceinline__ void force_term(const SceneConst &c, const ForceSelf 
     Fp.x = Fp.x + cn * dirx;
     Fp.y = Fp.y + cn * diry;
     Fp.z = Fp.z + cn * dirz;
+}
+
+// One force candidate by queue entry e: a global slot (kGlobalBit) or, on the brick path,
+// an LDS entry packed as s_a = (predicted position xyz, near density) and s_b = (velocity
+// xyz, density).
+template <bool kLds>
+__device__ __forceinline__ void force_term(const SceneConst &c, const ForceSelf &me, uint32_t e,
+                                           const float4 *__restrict__ pred_pos,
+                                           const float4 *__restrict__ vel_sorted,
+                                           const float2 *__restrict__ dens, const float4 *s_a,
+                                           const float4 *s_b, float3 &Fp, float3 &Fv) {
+    if (!kLds || (e & kGlobalBit)) {
+        const uint32_t g = e & ~kGlobalBit;
+        const float4 a = pred_pos[g], b = vel_sorted[g];
+        force_pair(c, me, make_float3(a.x, a.y, a.z), make_float3(b.x, b.y, b.z), dens[g], Fp, Fv);
+    } else {
+        const float4 a = s_a[e], b = s_b[e];
+        force_pair(c, me, make_float3(a.x, a.y, a.z), make_float3(b.x, b.y, b.z), make_float2(b.w, a.w), Fp, Fv);
+    }
+}
+
+// The force terms of a recorded neighbour list (entry i at list[i * n + j]), in order.
+// kU entries and their neighbours' data are loaded before any term is evaluated (the
+// pass waits on these dependent loads, not on the arithmetic).
+template <int kU>
+__device__ __forceinline__ void force_list(const SceneConst &c, const ForceSelf &me, uint32_t cnt,
+                                           const uint32_t *__restrict__ list, uint32_t n,
+                                           const float4 *__restrict__ pred_pos,
+                                           const float4 *__restrict__ vel_sorted,
+                                           const float2 *__restrict__ dens, float3 &Fp, float3 &Fv) {
+    for (uint32_t i = 0; i < cnt; i += kU) {
+        uint32_t g[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) g[u] = list[(size_t)min(i + (uint32_t)u, cnt - 1u) * n + me.j];
+        float4 a[kU], v[kU];
+        float2 d[kU];
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            a[u] = pred_pos[g[u]];
+            v[u] = vel_sorted[g[u]];
+            d[u] = dens[g[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u)
+            if (i + (uint32_t)u < cnt)
+                force_pair(c, me, make_float3(a[u].x, a[u].y, a[u].z), make_float3(v[u].x, v[u].y, v[u].z), d[u],
+                           Fp, Fv);
+    }
 }
 
 // applyPressureAndViscosityForce.comp:288-325: accumulates Fp and Fv for `me`.  The scan
