@@ -148,6 +148,15 @@ using ForceCells = ForceCellsOf<CellRuns<kHaloLds>>;
 // One density candidate at distance^2 d2 (calculateDensities.comp:123-127).  d2 > R2_pre
 // implies sqrt(d2) >= R: both kernels are 0 and M + m*0 == M for M >= +0, so such a
 // candidate changes nothing and its sqrt is skipped.
+// The terms of a candidate that passed the d2 <= R2_pre pre-filter.
+__device__ __forceinline__ void density_terms(const SceneConst &c, float d2, float &M, float &Mn) {
+    const float dist = sqrtf(d2);
+    if (dist >= c.R) return;
+    const float t = c.R - dist;
+    M = M + c.m * (t * t);
+    Mn = Mn + c.m * (t * t * t);
+}
+
 __device__ __forceinline__ void density_accumulate(const SceneConst &c, float d2, float &M, float &Mn) {
     if (d2 > c.R2_pre) return;
     const float dist = sqrtf(d2);
@@ -333,10 +342,8 @@ __device__ __forceinline__ void force_walk(const SceneConst &c, const ForceSelf 
 // (indices clamped to the run, extra lanes masked) so that a run of up to kU candidates
 // costs one round trip.  Sums still take the candidates one by one in run order.
 
-__device__ __forceinline__ float4 load_xyz(const float4 *p) {
-    const float3 v = *reinterpret_cast<const float3 *>(p);
-    return make_float4(v.x, v.y, v.z, 0.0f);
-}
+// A candidate's predicted position (whole float4: 16-byte loads gather faster than 12).
+__device__ __forceinline__ float4 load_xyz(const float4 *p) { return *p; }
 
 template <int kU, class Cells>
 __device__ __forceinline__ float2 density_walk_rows(const SceneConst &c, const float4 &s,
@@ -505,12 +512,15 @@ __device__ __forceinline__ float2 density_with_list(const SceneConst &c, uint32_
                     for (int u = 0; u < kU; ++u) {
                         if (k + u >= cnt) continue;
                         const float d2 = dist2(p[u], s);
-                        if (member) density_accumulate(c, d2, M, Mn);
-                        const uint32_t slot = (uint32_t)(r[dx].base + k + u);
-                        if (slot != j && !(d2 > R2)) {
-                            if (nl < (uint32_t)kListLds) lane_list[nl * stride] = slot;
-                            else if (nl < (uint32_t)kList) nlist[(size_t)nl * n + j] = slot;
-                            ++nl;
+                        // one branch for both uses: R2 <= R2_pre, and most candidates fail
+                        if (!(d2 > c.R2_pre)) {
+                            if (member) density_terms(c, d2, M, Mn);
+                            const uint32_t slot = (uint32_t)(r[dx].base + k + u);
+                            if (slot != j && !(d2 > R2)) {
+                                if (nl < (uint32_t)kListLds) lane_list[nl * stride] = slot;
+                                else if (nl < (uint32_t)kList) nlist[(size_t)nl * n + j] = slot;
+                                ++nl;
+                            }
                         }
                     }
                 }
