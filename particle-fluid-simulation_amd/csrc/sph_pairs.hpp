@@ -101,20 +101,39 @@ struct GridRuns {
     const int2 *__restrict__ table;
     const uint32_t *__restrict__ bstart;
     const uint2 *__restrict__ run_info;
-    __device__ __forceinline__ RunRef get(int cx, int cy, int cz) const {
-        const uint32_t lx = (uint32_t)(cx + 1), ly = (uint32_t)(cy + 1), lz = (uint32_t)(cz + 1);
-        if (lx < hx && ly < hy && lz < hz) {
-            const int2 se = grid[grid_base + lx + hx * (ly + hy * lz)];
-            if (se.x < 0) return RunRef{0, 0, 0, true};
-            return RunRef{se.x, se.y - se.x, se.x, true};
-        }
+    // Raw (start, end) in cell order; start < 0: no run.  Loads are left unconsumed so that
+    // several lookups can be in flight before any is used.
+    __device__ __forceinline__ int2 slow_raw(int cx, int cy, int cz) const {
         const uint32_t key = table_base + (cell_hash((uint32_t)cx, (uint32_t)cy, (uint32_t)cz) & mask);
         const int2 se = table[key];
         const uint2 ri = run_info[key];  // independent of se: both loads are in flight together
-        if (se.x == -1) return RunRef{0, 0, 0, true};
+        if (se.x == -1) return make_int2(-1, -1);
         const int first = (int)(bstart[ri.x] + ri.y);
-        return RunRef{first, se.y - se.x, first, true};
+        return make_int2(first, first + (se.y - se.x));
     }
+    __device__ __forceinline__ int2 raw(int cx, int cy, int cz) const {
+        const uint32_t lx = (uint32_t)(cx + 1), ly = (uint32_t)(cy + 1), lz = (uint32_t)(cz + 1);
+        if (lx < hx && ly < hy && lz < hz) return grid[grid_base + lx + hx * (ly + hy * lz)];
+        return slow_raw(cx, cy, cz);
+    }
+    // The three cells (cx - 1 .. cx + 1, cy, cz) of a row: consecutive grid entries.
+    __device__ __forceinline__ void row3(int cx, int cy, int cz, int2 out[3]) const {
+        const uint32_t lx = (uint32_t)cx, ly = (uint32_t)(cy + 1), lz = (uint32_t)(cz + 1);
+        if (lx + 2u < hx && ly < hy && lz < hz) {
+            const int2 *g = grid + grid_base + lx + hx * (ly + hy * lz);
+            out[0] = g[0];
+            out[1] = g[1];
+            out[2] = g[2];
+        } else {
+            for (int d = 0; d < 3; ++d) out[d] = raw(cx + d - 1, cy, cz);
+        }
+    }
+    static __device__ __forceinline__ RunRef ref(const int2 &se) {
+        const bool empty = se.x < 0;
+        const int base = empty ? 0 : se.x;
+        return RunRef{base, empty ? 0 : se.y - se.x, base, true};
+    }
+    __device__ __forceinline__ RunRef get(int cx, int cy, int cz) const { return ref(raw(cx, cy, cz)); }
 };
 
 // The density pass's 27 cells (calculateDensities.comp:114-118): sample points
@@ -496,9 +515,11 @@ __device__ __forceinline__ float2 density_with_list(const SceneConst &c, uint32_
     uint32_t nl = 0;
     for (int dz = 0; dz < 3; ++dz)
         for (int dy = 0; dy < 3; ++dy) {
+            int2 raw[3];
+            runs.row3(c0.x, c0.y + dy - 1, c0.z + dz - 1, raw);
             RunRef r[3];
 #pragma unroll
-            for (int dx = 0; dx < 3; ++dx) r[dx] = runs.get(c0.x + dx - 1, c0.y + dy - 1, c0.z + dz - 1);
+            for (int dx = 0; dx < 3; ++dx) r[dx] = GridRuns::ref(raw[dx]);
 #pragma unroll
             for (int dx = 0; dx < 3; ++dx) {
                 const bool member = ax.in(dx) && ay.in(dy) && az.in(dz);
