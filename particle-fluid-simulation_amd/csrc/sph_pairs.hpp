@@ -3,6 +3,7 @@
 #pragma once
 
 #include "sph_device.hpp"
+#include <type_traits>
 
 namespace sph {
 
@@ -146,6 +147,23 @@ struct DensityCellsOf {
         if (!(ax.in(dx) && ay.in(dy) && az.in(dz))) return RunRef{0, 0, 0, true};
         return runs.get(ax.c(dx), ay.c(dy), az.c(dz));
     }
+    // The three cells of row (dz, dy); with the grid lookup, consecutive cells are loaded
+    // together and non-member cells masked afterwards.
+    __device__ __forceinline__ void row(int dz, int dy, RunRef r[3]) const {
+        if constexpr (std::is_same_v<Runs, GridRuns>) {
+            if (ax.lo_off == 0xFFFFFFFFu && ax.hi_off == 1u) {
+                int2 raw[3];
+                runs.row3((int32_t)ax.mid, ay.c(dy), az.c(dz), raw);
+                const bool row_in = ay.in(dy) && az.in(dz);
+                for (int dx = 0; dx < 3; ++dx) {
+                    const RunRef q = GridRuns::ref(raw[dx]);
+                    r[dx] = (row_in && ax.in(dx)) ? q : RunRef{0, 0, 0, true};
+                }
+                return;
+            }
+        }
+        for (int dx = 0; dx < 3; ++dx) r[dx] = (*this)(dz, dy, dx);
+    }
 };
 
 // The force pass's 27 cells (applyPressure...comp:283-292): the sample point's cell plus
@@ -156,6 +174,15 @@ struct ForceCellsOf {
     int3 c0;
     __device__ __forceinline__ RunRef operator()(int dz, int dy, int dx) const {
         return runs.get(c0.x + dx - 1, c0.y + dy - 1, c0.z + dz - 1);
+    }
+    __device__ __forceinline__ void row(int dz, int dy, RunRef r[3]) const {
+        if constexpr (std::is_same_v<Runs, GridRuns>) {
+            int2 raw[3];
+            runs.row3(c0.x, c0.y + dy - 1, c0.z + dz - 1, raw);
+            for (int dx = 0; dx < 3; ++dx) r[dx] = GridRuns::ref(raw[dx]);
+        } else {
+            for (int dx = 0; dx < 3; ++dx) r[dx] = (*this)(dz, dy, dx);
+        }
     }
 };
 
@@ -372,8 +399,7 @@ __device__ __forceinline__ float2 density_walk_rows(const SceneConst &c, const f
     for (int dz = 0; dz < 3; ++dz)
         for (int dy = 0; dy < 3; ++dy) {
             RunRef r[3];
-#pragma unroll
-            for (int dx = 0; dx < 3; ++dx) r[dx] = cells(dz, dy, dx);
+            cells.row(dz, dy, r);
 #pragma unroll
             for (int dx = 0; dx < 3; ++dx) {
                 const float4 *src = pred_pos + r[dx].base;
@@ -406,8 +432,7 @@ __device__ __forceinline__ void force_walk_rows(const SceneConst &c, const Force
     for (int dz = 0; dz < 3; ++dz)
         for (int dy = 0; dy < 3; ++dy) {
             RunRef r[3];
-#pragma unroll
-            for (int dx = 0; dx < 3; ++dx) r[dx] = cells(dz, dy, dx);
+            cells.row(dz, dy, r);
 #pragma unroll
             for (int dx = 0; dx < 3; ++dx) {
                 const float4 *src = pred_pos + r[dx].base;
