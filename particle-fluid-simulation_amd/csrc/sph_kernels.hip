@@ -72,7 +72,8 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
     float4 *__restrict__ pos_sorted, float4 *__restrict__ vel_sorted,
     float4 *__restrict__ pred_pos, float4 *__restrict__ dbg_pred_vel, int2 *__restrict__ table,
     uint32_t *__restrict__ ghist, uint32_t *__restrict__ hfull, uint32_t *__restrict__ brick_flag,
-    uint32_t *__restrict__ stray, uint32_t *__restrict__ counters) {
+    uint32_t *__restrict__ stray, uint32_t *__restrict__ counters, uint32_t *__restrict__ bcount,
+    uint2 *__restrict__ run_info) {
     const uint32_t tid = threadIdx.x;
     if (blockIdx.x == 0)
         for (uint32_t i = tid; i < 4u * 256u; i += kBlock) ghist[i] = 0u;
@@ -91,6 +92,18 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
             }
         }
         if (j == n - 1u) tbl[2u * key + 1u] = (int)n;
+        if (bcount && (j == 0 || skeys[j - 1u] != key)) {
+            // Cell-ordered layout: the run's bucket (its first particle's cell) and its
+            // offset in the bucket.  The run's length comes from scanning its keys (runs
+            // are a few slots long; the table's end is written by another lane).
+            uint32_t e = j + 1u;
+            while (e < n && skeys[e] == key) ++e;
+            const SceneConst c = sc.scene(j);
+            const float4 P = pos[sidx ? sidx[j] : j];
+            const uint32_t bucket = bucket_of(c, cell_of(P.x, c.minx, c.R), cell_of(P.y, c.miny, c.R),
+                                              cell_of(P.z, c.minz, c.R));
+            run_info[key] = make_uint2(bucket, atomicAdd(&bcount[bucket], e - j));
+        }
         if (brick_flag) {
             const uint32_t idx = sidx ? sidx[j] : j;
             const float4 P = pos[idx];
@@ -240,7 +253,8 @@ hipError_t launch_reorder(hipStream_t stream, const Scenes &sc, const ChainBuffe
     hipLaunchKernelGGL(k_reorder<Scenes>, dim3(blocks_for(n)), dim3(kBlock), 0, stream, sc,
                        sorted_keys, sorted_idx, n, b.pos, b.vel, b.pos_sorted, b.vel_sorted,
                        b.pred_pos, b.dbg_pred_vel, b.table, b.ghist, b.hfull,
-                       bricks ? b.brick_flag : nullptr, b.stray, b.counters);
+                       bricks ? b.brick_flag : nullptr, b.stray, b.counters, bricks ? nullptr : b.bcount,
+                       b.run_info);
     return hipGetLastError();
 }
 

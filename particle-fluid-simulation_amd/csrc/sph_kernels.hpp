@@ -71,19 +71,16 @@ hipError_t launch_hash(hipStream_t stream, const Scenes &sc, const ChainBuffers 
 
 // Cell start/end table from the sorted keys; clears histograms.  With `bricks`, also
 // gathers the state into hash order with predicted positions, marks non-empty bricks and
-// lists stray particles; without, the state is gathered later by launch_place.
+// lists stray particles; without, places every run in its bucket of the cell-ordered
+// layout (the state is gathered later by launch_place).
 template <class Scenes>
 hipError_t launch_reorder(hipStream_t stream, const Scenes &sc, const ChainBuffers &b,
                           const uint32_t *sorted_keys, const uint32_t *sorted_idx, uint32_t n,
                           bool bricks);
 
-// Cell-ordered layout (sph_layout.hip): bucket and in-bucket offset of every run, the
-// bucket scan, then the gather of the state into cell order with predicted positions and
-// the halo grids' run table.
+// Cell-ordered layout (sph_layout.hip): the bucket scan, then the gather of the state into
+// cell order with predicted positions and the halo grids' run table.
 constexpr uint32_t kBucketTile = 2048;
-template <class Scenes>
-hipError_t launch_runs(hipStream_t stream, const Scenes &sc, const ChainBuffers &b,
-                       const uint32_t *sorted_keys, const uint32_t *sorted_idx, uint32_t n);
 hipError_t launch_bucket_scan(hipStream_t stream, const ChainBuffers &b, uint32_t bucket_total);
 template <class Scenes>
 hipError_t launch_place(hipStream_t stream, const Scenes &sc, const ChainBuffers &b,

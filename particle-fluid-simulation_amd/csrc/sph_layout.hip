@@ -4,13 +4,13 @@
 // neighbouring cells over the whole array: a wave of 64 consecutive slots touches ~600
 // unrelated runs in each neighbour pass and nothing it loads is reused by the next wave.
 // The neighbour sums only require that each key's run is visited whole and in its own
-// (hash) order, so the runs themselves may be placed anywhere.  These kernels place every
-// run by the grid bucket of its first particle's cell (buckets numbered x-fastest), which
-// puts the runs of neighbouring cells next to each other:
+// (hash) order, so the runs themselves may be placed anywhere.  Every run goes to the grid
+// bucket of its first particle's cell (buckets numbered x-fastest), which puts the runs of
+// neighbouring cells next to each other:
 //
-//   k_runs        one lane per hash slot; at each run's first slot: the run's bucket and,
-//                 by an atomic add of the run length to the bucket's count, its offset in
-//                 the bucket (runs sharing a bucket land in any order);
+//   (k_reorder)   at each run's first slot: the run's bucket and, by an atomic add of the
+//                 run length to the bucket's count, its offset in the bucket (runs sharing
+//                 a bucket land in any order), stored per key;
 //   k_bucket_scan exclusive scan of the bucket counts (single pass, decoupled look-back);
 //   k_place       one lane per hash slot: its cell-ordered slot, the gather of the state
 //                 with the prediction (predictPositions.comp) and the hash slot of each
@@ -41,23 +41,6 @@ __device__ __forceinline__ uint32_t load_relaxed(const uint32_t *p) {
 }
 __device__ __forceinline__ void store_relaxed(uint32_t *p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-template <class Scenes>
-__global__ __launch_bounds__(kBlock) void k_runs(
-    Scenes sc, const uint32_t *__restrict__ skeys, const uint32_t *__restrict__ sidx, uint32_t n,
-    const float4 *__restrict__ pos, const int2 *__restrict__ table, uint32_t *__restrict__ bcount,
-    uint2 *__restrict__ run_info) {
-    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-    if (j >= n) return;
-    const uint32_t key = skeys[j];
-    if (j > 0u && skeys[j - 1u] == key) return;
-    const SceneConst c = sc.scene(j);
-    const float4 P = pos[sidx ? sidx[j] : j];
-    const uint32_t bucket = bucket_of(c, cell_of(P.x, c.minx, c.R), cell_of(P.y, c.miny, c.R),
-                                      cell_of(P.z, c.minz, c.R));
-    const uint32_t len = (uint32_t)table[key].y - j;
-    run_info[key] = make_uint2(bucket, atomicAdd(&bcount[bucket], len));
 }
 
 // Exclusive scan of bcount[0, count) in place.  Tiles are taken in launch order from an
@@ -92,28 +75,38 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scan(uint32_t *__restrict__ b
         if (i < w) wbase += s_wave[i];
         total += s_wave[i];
     }
-    if (tid == 0) {
-        uint32_t *mine = lookback + tile;
+    if (w == 0) {
+        // Decoupled look-back by the first wave: lane l reads tile - 1 - l, so 64
+        // predecessors are examined per round; the nearest one with an inclusive prefix
+        // ends the walk (tiles before tile 0 count as a prefix of 0).
         uint32_t excl = 0;
         if (tile == 0) {
-            store_relaxed(mine, kFlagPrefix | total);
+            if (lane == 0) store_relaxed(lookback, kFlagPrefix | total);
         } else {
-            store_relaxed(mine, kFlagAggregate | total);
-            int32_t t = (int32_t)tile - 1;
+            if (lane == 0) store_relaxed(lookback + tile, kFlagAggregate | total);
+            int32_t top = (int32_t)tile - 1;
             while (true) {
-                const uint32_t lv = load_relaxed(lookback + t);
+                const int32_t t = top - lane;
+                const uint32_t lv = t >= 0 ? load_relaxed(lookback + t) : kFlagPrefix;
                 const uint32_t f = lv & kFlagMask;
-                if (f == 0u) {
+                const uint64_t not_ready = __ballot(f == 0u);
+                const uint64_t prefix = __ballot(f == kFlagPrefix);
+                const int pl = prefix ? __ffsll((long long)prefix) - 1 : 64;
+                const uint64_t need = pl >= 63 ? ~0ull : ((1ull << (pl + 1)) - 1ull);
+                if (not_ready & need) {
                     __builtin_amdgcn_s_sleep(1);
                     continue;
                 }
-                excl += lv & kValueMask;
-                if (f == kFlagPrefix) break;
-                --t;
+                uint32_t v2 = lane <= pl ? (lv & kValueMask) : 0u;
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) v2 += __shfl_xor(v2, o, 64);
+                excl += v2;
+                if (prefix) break;
+                top -= 64;
             }
-            store_relaxed(mine, kFlagPrefix | (excl + total));
+            if (lane == 0) store_relaxed(lookback + tile, kFlagPrefix | (excl + total));
         }
-        s_excl = excl;
+        if (lane == 0) s_excl = excl;
     }
     __syncthreads();
     uint32_t run = s_excl + wbase + x - sum;
@@ -175,14 +168,6 @@ inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1u) / kBlock; }
 
 }  // namespace
 
-template <class Scenes>
-hipError_t launch_runs(hipStream_t stream, const Scenes &sc, const ChainBuffers &b,
-                       const uint32_t *sorted_keys, const uint32_t *sorted_idx, uint32_t n) {
-    hipLaunchKernelGGL(k_runs<Scenes>, dim3(blocks_for(n)), dim3(kBlock), 0, stream, sc, sorted_keys,
-                       sorted_idx, n, b.pos, b.table, b.bcount, b.run_info);
-    return hipGetLastError();
-}
-
 hipError_t launch_bucket_scan(hipStream_t stream, const ChainBuffers &b, uint32_t bucket_total) {
     const uint32_t tiles = (bucket_total + kBucketTile - 1u) / kBucketTile;
     hipLaunchKernelGGL(k_bucket_scan, dim3(tiles), dim3(kBlock), 0, stream, b.bcount, bucket_total,
@@ -203,8 +188,6 @@ hipError_t launch_place(hipStream_t stream, const Scenes &sc, const ChainBuffers
 }
 
 #define SPH_LAYOUT_INSTANTIATE(S)                                                                \
-    template hipError_t launch_runs<S>(hipStream_t, const S &, const ChainBuffers &,             \
-                                       const uint32_t *, const uint32_t *, uint32_t);            \
     template hipError_t launch_place<S>(hipStream_t, const S &, const ChainBuffers &,            \
                                         const uint32_t *, const uint32_t *, uint32_t, uint32_t);
 

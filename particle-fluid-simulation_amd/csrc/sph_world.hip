@@ -269,7 +269,6 @@ hipError_t Chain::enqueue(hipStream_t s, const Scenes &sc, PassEvents *ev) {
     if (bricks) {
         if ((e = launch_queue(s, b, brick_total)) != hipSuccess) return e;
     } else {
-        if ((e = launch_runs(s, sc, b, sorted_keys(), sorted_idx(), n)) != hipSuccess) return e;
         if ((e = launch_bucket_scan(s, b, bucket_total)) != hipSuccess) return e;
         if ((e = launch_place(s, sc, b, sorted_keys(), sorted_idx(), n, grid_total)) != hipSuccess) return e;
     }
