@@ -222,6 +222,9 @@ def main() -> int:
     ap.add_argument("--workload", default="uniform_1m", choices=["uniform_1m", "dam_break_16k"])
     ap.add_argument("--table-cells", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--lane-config", type=int, default=None,
+                    help="pin one per-lane launch configuration (0-9) instead of autotuning; "
+                         "for hardware-counter runs, where counters distort the tuner's timings")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--batch-steps", type=int, default=100,
                     help="steps of the mixed 256-scene batched-vs-plain comparison (0 = skip)")
@@ -255,9 +258,14 @@ def main() -> int:
     # restores that state afterwards), and the last warmup step builds the tuned graph.
     world.step(max(0, args.warmup - 1))
     world.sync()
-    chosen, tuned_ms = world.autotune(5)
-    chosen = world.kernel_config
-    print(f"[bench] autotune: {chosen}, {tuned_ms:.3f} ms/substep", file=sys.stderr, flush=True)
+    if args.lane_config is None:
+        chosen, tuned_ms = world.autotune(5)
+        chosen = world.kernel_config
+        print(f"[bench] autotune: {chosen}, {tuned_ms:.3f} ms/substep", file=sys.stderr, flush=True)
+    else:
+        world.set_lane_config(args.lane_config)
+        chosen = world.kernel_config
+        print(f"[bench] pinned: {chosen}", file=sys.stderr, flush=True)
     if args.warmup > 0:
         world.step(1)
     world.sync()

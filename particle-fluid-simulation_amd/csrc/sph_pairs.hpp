@@ -388,8 +388,18 @@ __device__ __forceinline__ void force_walk(const SceneConst &c, const ForceSelf 
 // (indices clamped to the run, extra lanes masked) so that a run of up to kU candidates
 // costs one round trip.  Sums still take the candidates one by one in run order.
 
-// A candidate's predicted position (whole float4: 16-byte loads gather faster than 12).
+// A candidate's predicted position.
 __device__ __forceinline__ float4 load_xyz(const float4 *p) { return *p; }
+
+// The kU candidates of a run loaded together at run offset k: only lanes whose run still
+// has candidate u issue its load (the others would only repeat a load, and every load costs
+// the texture path its lanes' bytes); candidate 0 always exists inside the loop.
+template <int kU>
+__device__ __forceinline__ void load_candidates(const float4 *src, int k, int cnt, float4 (&p)[kU]) {
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+        p[u] = (u == 0 || k + u < cnt) ? load_xyz(src + k + u) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
 
 template <int kU, class Cells>
 __device__ __forceinline__ float2 density_walk_rows(const SceneConst &c, const float4 &s,
@@ -406,8 +416,7 @@ __device__ __forceinline__ float2 density_walk_rows(const SceneConst &c, const f
                 const int cnt = r[dx].cnt;
                 for (int k = 0; k < cnt; k += kU) {
                     float4 p[kU];
-#pragma unroll
-                    for (int u = 0; u < kU; ++u) p[u] = load_xyz(src + min(k + u, cnt - 1));
+                    load_candidates<kU>(src, k, cnt, p);
 #pragma unroll
                     for (int u = 0; u < kU; ++u)
                         if (k + u < cnt) density_accumulate(c, dist2(p[u], s), M, Mn);
@@ -439,8 +448,7 @@ __device__ __forceinline__ void force_walk_rows(const SceneConst &c, const Force
                 const int cnt = r[dx].cnt;
                 for (int k = 0; k < cnt; k += kU) {
                     float4 p[kU];
-#pragma unroll
-                    for (int u = 0; u < kU; ++u) p[u] = load_xyz(src + min(k + u, cnt - 1));
+                    load_candidates<kU>(src, k, cnt, p);
 #pragma unroll
                     for (int u = 0; u < kU; ++u) {
                         if (k + u >= cnt) continue;
@@ -514,7 +522,7 @@ __device__ __forceinline__ void force_finish(const SceneConst &c, const float4 &
 // position outside the box), or whose list would exceed kList entries, gets the density's
 // own walk and an invalid count, and the force pass walks its cells.
 constexpr int kList = 128;     // entries per particle in the global list
-constexpr int kListLds = 24;   // entries staged in LDS (the rest are stored directly)
+constexpr int kListLds = 20;   // entries staged in LDS (the rest are stored directly)
 constexpr uint8_t kListInvalid = 0xFF;
 
 template <int kU>
@@ -552,8 +560,7 @@ __device__ __forceinline__ float2 density_with_list(const SceneConst &c, uint32_
                 const int cnt = r[dx].cnt;
                 for (int k = 0; k < cnt; k += kU) {
                     float4 p[kU];
-#pragma unroll
-                    for (int u = 0; u < kU; ++u) p[u] = load_xyz(src + min(k + u, cnt - 1));
+                    load_candidates<kU>(src, k, cnt, p);
 #pragma unroll
                     for (int u = 0; u < kU; ++u) {
                         if (k + u >= cnt) continue;
