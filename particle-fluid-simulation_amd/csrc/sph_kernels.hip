@@ -22,8 +22,9 @@ template <class Scenes>
 __global__ __launch_bounds__(kBlock) void k_hash(
     Scenes sc, const float4 *__restrict__ pos, uint32_t *__restrict__ keys, uint32_t n,
     uint32_t passes, uint32_t *__restrict__ ghist, uint32_t hash_blocks,
-    int2 *__restrict__ table, uint32_t table_total, uint32_t *__restrict__ lookback,
-    uint32_t lookback_words, uint32_t *__restrict__ tile_ctr, uint32_t *__restrict__ counters,
+    int2 *__restrict__ table, int2 *__restrict__ crun, uint32_t table_total,
+    uint32_t *__restrict__ lookback, uint32_t lookback_words, uint32_t *__restrict__ tile_ctr,
+    uint32_t *__restrict__ counters,
     uint32_t *__restrict__ bcount, uint32_t bucket_total, uint32_t *__restrict__ blookback,
     uint32_t blookback_words) {
     __shared__ uint32_t s_hist[4][256];
@@ -54,7 +55,10 @@ __global__ __launch_bounds__(kBlock) void k_hash(
         const uint32_t nb = gridDim.x - hash_blocks;
         if (b == 0 && tid < kTileCounters) tile_ctr[tid] = 0u;
         if (b == 0 && tid < 2 && counters) counters[tid] = 0u;
-        for (uint32_t i = b * kBlock + tid; i < table_total; i += nb * kBlock) table[i] = make_int2(-1, -1);
+        for (uint32_t i = b * kBlock + tid; i < table_total; i += nb * kBlock) {
+            table[i] = make_int2(-1, -1);
+            if (crun) crun[i] = make_int2(-1, -1);
+        }
         for (uint32_t i = b * kBlock + tid; i < lookback_words; i += nb * kBlock) lookback[i] = 0u;
         for (uint32_t i = b * kBlock + tid; i < bucket_total; i += nb * kBlock) bcount[i] = 0u;
         for (uint32_t i = b * kBlock + tid; i < blookback_words; i += nb * kBlock) blookback[i] = 0u;
@@ -152,8 +156,7 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
 // Run lookup of the cell-ordered layout for scene c.
 __device__ __forceinline__ GridRuns grid_runs(const SceneConst &c, const ChainBuffers &b) {
     return GridRuns{b.grid, c.grid_base, (uint32_t)c.gx + 2u, (uint32_t)c.gy + 2u,
-                    c.grid_cells ? (uint32_t)c.gz + 2u : 0u, c.table_base, c.mask, b.table,
-                    b.bcount, b.run_info};
+                    c.grid_cells ? (uint32_t)c.gz + 2u : 0u, c.table_base, c.mask, b.crun};
 }
 
 // Workgroups are handed to the 8 XCDs round-robin; this renumbering gives each XCD one
@@ -240,7 +243,8 @@ hipError_t launch_hash(hipStream_t stream, const Scenes &sc, const ChainBuffers 
     if (clear_blocks < 1u) clear_blocks = 1u;
     if (clear_blocks > 2048u) clear_blocks = 2048u;
     hipLaunchKernelGGL(k_hash<Scenes>, dim3(hash_blocks + clear_blocks), dim3(kBlock), 0, stream,
-                       sc, b.pos, b.keys, n, passes, b.ghist, hash_blocks, b.table, table_total,
+                       sc, b.pos, b.keys, n, passes, b.ghist, hash_blocks, b.table,
+                       bucket_total ? b.crun : nullptr, table_total,
                        b.lookback, lookback_words, b.tile_ctr, b.counters, b.bcount, bucket_total,
                        b.blookback, btiles);
     return hipGetLastError();

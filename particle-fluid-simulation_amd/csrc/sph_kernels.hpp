@@ -45,6 +45,7 @@ struct ChainBuffers {
     uint32_t *bcount;       // [bucket_total] particles per bucket, then the bucket's first slot
     uint32_t *blookback;    // [bucket tiles] look-back words of the bucket scan
     uint2 *run_info;        // [table_total] per occupied key: (bucket, offset in the bucket)
+    int2 *crun;             // [table_total] per key: its run's (start, end) in cell order, -1: empty
     uint32_t *hslot;        // [n] hash slot of each cell-ordered slot
     int2 *grid;             // [grid_total] (start, end) in cell order of each halo-grid cell's run
     // Force neighbour lists recorded by the per-lane density pass (cell-ordered slots).
@@ -79,7 +80,7 @@ hipError_t launch_reorder(hipStream_t stream, const Scenes &sc, const ChainBuffe
                           bool bricks);
 
 // Cell-ordered layout (sph_layout.hip): the bucket scan, then the gather of the state into
-// cell order with predicted positions and the halo grids' run table.
+// cell order with predicted positions and each key's cell-ordered run, then the halo grids.
 constexpr uint32_t kBucketTile = 2048;
 hipError_t launch_bucket_scan(hipStream_t stream, const ChainBuffers &b, uint32_t bucket_total);
 template <class Scenes>

@@ -14,10 +14,12 @@
 //   k_bucket_scan exclusive scan of the bucket counts (single pass, decoupled look-back);
 //   k_place       one lane per hash slot: its cell-ordered slot, the gather of the state
 //                 with the prediction (predictPositions.comp) and the hash slot of each
-//                 cell-ordered slot; further workgroups fill each scene's halo grid: the
-//                 cell-ordered (start, end) of the run of the key of every cell of the box
-//                 grid grown by one cell per side, which the neighbour walks read instead
-//                 of hashing the cell and reading the hash-order table.
+//                 cell-ordered slot; each run's first lane stores the run's cell-ordered
+//                 (start, end) for its key (keys without particles keep the -1 the hash
+//                 pass wrote);
+//   k_grid        each scene's halo grid: the cell-ordered run of the key of every cell
+//                 of the box grid grown by one cell per side, which the neighbour walks
+//                 read instead of hashing the cell.
 //
 // Placement never changes a result: every particle's sums visit the same runs in the
 // same order whatever the layout, so only the memory order (and speed) depends on the
@@ -117,40 +119,23 @@ __global__ __launch_bounds__(kBlock) void k_bucket_scan(uint32_t *__restrict__ b
     }
 }
 
+// Gathers every particle into its cell-ordered slot; the first particle of each run also
+// records the run's cell-ordered (start, end) for its key.
 template <class Scenes>
 __global__ __launch_bounds__(kBlock) void k_place(
     Scenes sc, const uint32_t *__restrict__ skeys, const uint32_t *__restrict__ sidx, uint32_t n,
-    uint32_t place_blocks, uint32_t grid_total, const float4 *__restrict__ pos,
-    const float4 *__restrict__ vel, const int2 *__restrict__ table,
+    const float4 *__restrict__ pos, const float4 *__restrict__ vel, const int2 *__restrict__ table,
     const uint32_t *__restrict__ bstart, const uint2 *__restrict__ run_info,
-    float4 *__restrict__ pos_sorted,
-    float4 *__restrict__ vel_sorted, float4 *__restrict__ pred_pos,
-    float4 *__restrict__ dbg_pred_vel, uint32_t *__restrict__ hslot, int2 *__restrict__ grid) {
-    if (blockIdx.x >= place_blocks) {
-        const uint32_t g = (blockIdx.x - place_blocks) * kBlock + threadIdx.x;
-        if (g >= grid_total) return;
-        const SceneConst c = sc.scene_of_grid(g);
-        const uint32_t hx = (uint32_t)c.gx + 2u, hy = (uint32_t)c.gy + 2u;
-        const uint32_t l = g - c.grid_base;
-        const uint32_t lx = l % hx, ly = (l / hx) % hy, lz = l / (hx * hy);
-        // cell (lx-1, ly-1, lz-1); -1 wraps to 2^32-1 as in the reference's u32 hash
-        const uint32_t key = c.table_base + (cell_hash(lx - 1u, ly - 1u, lz - 1u) & c.mask);
-        const int2 t = table[key];
-        const uint2 ri = run_info[key];
-        int2 out = make_int2(-1, -1);
-        if (t.x >= 0) {
-            const int first = (int)(bstart[ri.x] + ri.y);
-            out = make_int2(first, first + (t.y - t.x));
-        }
-        grid[g] = out;
-        return;
-    }
+    float4 *__restrict__ pos_sorted, float4 *__restrict__ vel_sorted, float4 *__restrict__ pred_pos,
+    float4 *__restrict__ dbg_pred_vel, uint32_t *__restrict__ hslot, int2 *__restrict__ crun) {
     const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
     if (j >= n) return;
     const uint32_t key = skeys[j];
-    const uint32_t head = (uint32_t)table[key].x;
+    const int2 t = table[key];
+    const uint32_t head = (uint32_t)t.x;
     const uint2 ri = run_info[key];
     const uint32_t j2 = bstart[ri.x] + ri.y + (j - head);
+    if (j == head) crun[key] = make_int2((int)j2, (int)j2 + (t.y - t.x));
     const uint32_t idx = sidx ? sidx[j] : j;
     const float4 P = pos[idx];
     const float4 V = vel[idx];
@@ -162,6 +147,20 @@ __global__ __launch_bounds__(kBlock) void k_place(
     pred_pos[j2] = make_float4(p.x, p.y, p.z, 1.0f);
     if (dbg_pred_vel) dbg_pred_vel[j2] = make_float4(v.x, v.y, v.z, 1.0f);
     hslot[j2] = j;
+}
+
+// The halo grid: each cell's key's cell-ordered run (one lookup per cell).
+template <class Scenes>
+__global__ __launch_bounds__(kBlock) void k_grid(Scenes sc, uint32_t grid_total,
+                                                 const int2 *__restrict__ crun, int2 *__restrict__ grid) {
+    const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= grid_total) return;
+    const SceneConst c = sc.scene_of_grid(g);
+    const uint32_t hx = (uint32_t)c.gx + 2u, hy = (uint32_t)c.gy + 2u;
+    const uint32_t l = g - c.grid_base;
+    const uint32_t lx = l % hx, ly = (l / hx) % hy, lz = l / (hx * hy);
+    // cell (lx-1, ly-1, lz-1); -1 wraps to 2^32-1 as in the reference's u32 hash
+    grid[g] = crun[c.table_base + (cell_hash(lx - 1u, ly - 1u, lz - 1u) & c.mask)];
 }
 
 inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1u) / kBlock; }
@@ -179,11 +178,13 @@ template <class Scenes>
 hipError_t launch_place(hipStream_t stream, const Scenes &sc, const ChainBuffers &b,
                         const uint32_t *sorted_keys, const uint32_t *sorted_idx, uint32_t n,
                         uint32_t grid_total) {
-    const uint32_t place_blocks = blocks_for(n);
-    hipLaunchKernelGGL(k_place<Scenes>, dim3(place_blocks + blocks_for(grid_total)), dim3(kBlock), 0,
-                       stream, sc, sorted_keys, sorted_idx, n, place_blocks, grid_total, b.pos, b.vel,
-                       b.table, b.bcount, b.run_info, b.pos_sorted, b.vel_sorted,
-                       b.pred_pos, b.dbg_pred_vel, b.hslot, b.grid);
+    hipLaunchKernelGGL(k_place<Scenes>, dim3(blocks_for(n)), dim3(kBlock), 0, stream, sc, sorted_keys,
+                       sorted_idx, n, b.pos, b.vel, b.table, b.bcount, b.run_info, b.pos_sorted,
+                       b.vel_sorted, b.pred_pos, b.dbg_pred_vel, b.hslot, b.crun);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || grid_total == 0) return e;
+    hipLaunchKernelGGL(k_grid<Scenes>, dim3(blocks_for(grid_total)), dim3(kBlock), 0, stream, sc,
+                       grid_total, b.crun, b.grid);
     return hipGetLastError();
 }
 

@@ -93,24 +93,16 @@ struct CellRuns {
 };
 
 // Finds cell runs in the cell-ordered layout: through the scene's halo grid when the cell
-// lies in it (hz = 0: the scene has none), else through the hash-order table and the
-// run's placement (first cell-ordered slot = bucket start + offset in the bucket).
+// lies in it (hz = 0: the scene has none), else through the key's cell-ordered run.
 struct GridRuns {
     const int2 *__restrict__ grid;
     uint32_t grid_base, hx, hy, hz;
     uint32_t table_base, mask;
-    const int2 *__restrict__ table;
-    const uint32_t *__restrict__ bstart;
-    const uint2 *__restrict__ run_info;
+    const int2 *__restrict__ crun;
     // Raw (start, end) in cell order; start < 0: no run.  Loads are left unconsumed so that
     // several lookups can be in flight before any is used.
     __device__ __forceinline__ int2 slow_raw(int cx, int cy, int cz) const {
-        const uint32_t key = table_base + (cell_hash((uint32_t)cx, (uint32_t)cy, (uint32_t)cz) & mask);
-        const int2 se = table[key];
-        const uint2 ri = run_info[key];  // independent of se: both loads are in flight together
-        if (se.x == -1) return make_int2(-1, -1);
-        const int first = (int)(bstart[ri.x] + ri.y);
-        return make_int2(first, first + (se.y - se.x));
+        return crun[table_base + (cell_hash((uint32_t)cx, (uint32_t)cy, (uint32_t)cz) & mask)];
     }
     __device__ __forceinline__ int2 raw(int cx, int cy, int cz) const {
         const uint32_t lx = (uint32_t)(cx + 1), ly = (uint32_t)(cy + 1), lz = (uint32_t)(cz + 1);

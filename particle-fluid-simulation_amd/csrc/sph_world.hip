@@ -194,7 +194,8 @@ int Chain::allocate(uint32_t n_, uint32_t table_total_, uint32_t brick_total_,
         {(void **)&b.tile_ctr, kTileCounters * 4}, {(void **)&b.hfull, N * 4}, {(void **)&b.stray, N * 4},
         {(void **)&b.bcount, (size_t)bucket_total * 4},
         {(void **)&b.blookback, (size_t)((bucket_total + kBucketTile - 1u) / kBucketTile) * 4},
-        {(void **)&b.run_info, (size_t)table_total * 8}, {(void **)&b.hslot, N * 4},
+        {(void **)&b.run_info, (size_t)table_total * 8}, {(void **)&b.crun, (size_t)table_total * 8},
+        {(void **)&b.hslot, N * 4},
         {(void **)&b.grid, (size_t)(grid_total ? grid_total : 1) * 8},
         {(void **)&b.nlist, N * kList * 4}, {(void **)&b.ncount, N},
         {(void **)&b.counters, 8 * 4},
@@ -239,7 +240,7 @@ void Chain::release() {
                     b.pred_pos, b.pos_sorted, b.vel_sorted, b.dens, b.table, b.ghist,
                     b.lookback, b.tile_ctr, b.dbg_pred_vel, b.dbg_vel_force, b.hfull,
                     b.brick_flag, b.queue, b.stray, b.counters, b.bcount, b.blookback,
-                    b.run_info, b.hslot, b.grid, b.nlist, b.ncount};
+                    b.run_info, b.crun, b.hslot, b.grid, b.nlist, b.ncount};
     for (void *p : ptrs)
         if (p) hipFree(p);
     b = ChainBuffers{};
@@ -486,9 +487,10 @@ uint64_t sph_world_footprint(uint32_t n, uint32_t table_cells) {
     const uint64_t T = table_cells ? table_cells : n;
     const uint64_t tiles = ((uint64_t)n + radix_tile_size() - 1) / radix_tile_size();
     // state, keys, 2 x radix ping-pong, 3 sorted float4 arrays, densities, brick and
-    // cell-order slot arrays; the cell table; buckets (at most 2n + 1024), the halo grid
-    // (at most 16n + 4096 cells) and look-back words
-    return (uint64_t)n * (16 * 2 + 4 + 16 + 16 * 3 + 8 + 8 + 4 + 4 * kList + 1) + T * 16 +
+    // cell-order slot arrays; per table cell its (start, end), run placement and cell-ordered
+    // run; buckets (at most 2n + 1024), the halo grid (at most 16n + 4096 cells) and
+    // look-back words
+    return (uint64_t)n * (16 * 2 + 4 + 16 + 16 * 3 + 8 + 8 + 4 + 4 * kList + 1) + T * 24 +
            (2 * (uint64_t)n + 1024) * 4 + (16 * (uint64_t)n + 4096) * 8 + 4 * tiles * 256 * 4 + 4096;
 }
 

@@ -88,7 +88,7 @@ def test_footprint_is_monotone():
     lib = sph_hip.load_library()
     f1 = lib.sph_world_footprint(1 << 20, 1 << 20)
     f2 = lib.sph_world_footprint(1 << 20, 1 << 21)
-    assert f2 - f1 == 16 * (1 << 20)
+    assert f2 - f1 == 24 * (1 << 20)
     assert lib.sph_world_footprint(1 << 20, 0) == f1
 
 
