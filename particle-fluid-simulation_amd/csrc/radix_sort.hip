@@ -174,7 +174,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_pass(
     }
 }
 
-constexpr int kItemsDefault = 8;
+constexpr int kItemsDefault = 16;  // 4096-key tiles: half the look-back chain of 2048-key tiles
 
 }  // namespace
 
