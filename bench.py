@@ -47,7 +47,7 @@ KERNEL_NAMES = {"density": "k_density (calculateDensities + force neighbour list
 # HBM bytes per launch of the neighbour kernels on uniform_1m (T = 2^21) measured with
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (profiles/pmc_uniform_1m_neighbour_passes.md),
 # each counter scaled by what it reads for a 64 MiB device-to-device copy in the same run.
-TRAFFIC_BYTES_PER_LAUNCH = {"density": (2 * 55782 + 76854) * 1024, "force": (2 * 146786 + 42611) * 1024}
+TRAFFIC_BYTES_PER_LAUNCH = {"density": (2 * 56080 + 77389) * 1024, "force": (2 * 161667 + 42456) * 1024}
 
 
 def cpu_baseline(workload, table_cells: int, budget_s: float = 12.0) -> dict:
