@@ -25,6 +25,41 @@ for _p in (ROOT, PKG):
 
 METRIC = "particle-steps/s batched vs plain (1M pts, 1 GPU); p50/p99 scene latency"
 HBM_PEAK_GBS = 8000.0
+FP32_VECTOR_PEAK_TFLOPS = 157.3
+
+
+def flops_per_particle_substep(workload) -> dict:
+    """Floating-point work per particle per substep of the uniform fill, in closed form:
+    density ~10 flops per candidate and 8 per in-radius neighbour, force ~9 per candidate
+    and 50 per neighbour, with C = n 27 R^3 candidates in the 27 cells and K = n 4/3 pi R^3
+    neighbours at the initial fill's number density n = N / volume of the filled block."""
+    import math
+    vol = 1.0
+    for lo, hi in zip(workload.block_lo, workload.block_hi):
+        vol *= hi - lo
+    dens = workload.n / vol
+    R = workload.smoothing_radius
+    C = dens * 27.0 * R ** 3
+    K = dens * 4.0 / 3.0 * math.pi * R ** 3
+    return {"candidates": C, "neighbours": K, "flops": 10 * C + 8 * K + 9 * C + 50 * K}
+
+
+def copy_bandwidth_gbs(mib: int = 512, reps: int = 5) -> float:
+    """Device-to-device copy rate (bytes read + written per second), best of `reps`: the
+    measured counterpart of the HBM peak."""
+    import torch
+    src = torch.ones(mib << 18, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    best = float("inf")
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        e1.synchronize()
+        best = min(best, e0.elapsed_time(e1) * 1e-3)
+    del src, dst
+    return 2.0 * (mib << 20) / best / 1e9
 
 
 def alg_bytes_per_particle_substep(n: int, table_cells: int) -> float:
@@ -297,6 +332,9 @@ def main() -> int:
     b_alg = alg_bytes_per_particle_substep(W.n, T)
     substep_s = ms_per_step * 1e-3 / W.substeps
     chain_gbs = W.n * b_alg / substep_s / 1e9
+    copy_gbs = copy_bandwidth_gbs()
+    work = flops_per_particle_substep(W)
+    tflops = W.n * work["flops"] / substep_s / 1e12
 
     # Sanity: the state stays finite and in the box.
     p_out, v_out = world.download()
@@ -337,7 +375,12 @@ def main() -> int:
                      "kernel": KERNEL_NAMES[dom], "avg_launch_ms": pt[f"{dom}_ms"],
                      "alg_bytes_per_launch": dom_bytes},
         "chain_roofline": {"alg_bytes_per_particle_substep": b_alg, "achieved_gbs": chain_gbs,
-                           "frac": chain_gbs / HBM_PEAK_GBS},
+                           "frac": chain_gbs / HBM_PEAK_GBS, "copy_gbs": copy_gbs,
+                           "frac_of_copy": chain_gbs / copy_gbs},
+        "chain_flops": {"flops_per_particle_substep": work["flops"],
+                        "candidates": work["candidates"], "neighbours": work["neighbours"],
+                        "achieved_tflops": tflops, "peak_tflops": FP32_VECTOR_PEAK_TFLOPS,
+                        "frac": tflops / FP32_VECTOR_PEAK_TFLOPS},
         "kernels": chosen,
         "pass_ms": pt,
         "pass_ms_by_kernel": variants,
