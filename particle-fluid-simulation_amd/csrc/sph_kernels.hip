@@ -1,6 +1,7 @@
 // SPH substep kernels for gfx950: hash, reorder + cell table, density, force + integrate.
 //
-// One lane per particle slot; 256-lane workgroups (4 wave64).  Neighbour sums visit the
+// One lane per particle slot; 256-lane workgroups (4 wave64) for the hash and reorder, an
+// autotuned workgroup size (64, 128 or 256 lanes) for the neighbour passes.  Neighbour sums visit the
 // 27 cells in the reference's (dz, dy, dx) order and each cell's run in ascending slot
 // order, so every particle's floating-point sum has the reference's association order and
 // does not depend on launch geometry or on which other scenes share the launch.
