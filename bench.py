@@ -112,17 +112,21 @@ def cpu_baseline(workload, table_cells: int, budget_s: float = 12.0) -> dict:
     }
 
 
-def mixed_batch(steps: int, warmup: int, count: int = 256) -> dict:
+def mixed_batch(steps: int, warmup: int, count: int = 256, fitting_only: bool = False) -> dict:
     """Batched vs plain on the mixed workload: `count` scenes (log-uniform 2,048-65,536
     particles, boxes U[4,16]^3, target density U[40,70], seeds 3+s) stepped together in one
     batch, then the same scenes as one world each, stepped one after another (each world's
     steps are enqueued on its own stream, so the runtime may overlap them).  Both are tuned
-    after `warmup` + 1 untimed frames."""
+    after `warmup` + 1 untimed frames.  With `fitting_only`, only the scenes whose resting
+    block fits the box at its target density (the others start over-dense)."""
     import torch
     import sph_hip
     from sph_hip.workloads import mixed_scenes
 
     specs = mixed_scenes(count=count, seed=3)
+    if fitting_only:
+        specs = [sp for sp in specs if sp.block_fits]
+    count = len(specs)
     params = [sp.params() for sp in specs]
     counts = [sp.n for sp in specs]
     states = [sp.state() for sp in specs]
@@ -166,7 +170,8 @@ def mixed_batch(steps: int, warmup: int, count: int = 256) -> dict:
         w.close()
     batched = total * steps / t_batch
     plain = total * steps / t_plain
-    return {"scenes": count, "particles": total, "steps": steps,
+    return {"scenes": count, "subset": "blocks that fit at the target density" if fitting_only else "all",
+            "particles": total, "steps": steps,
             "batched_particle_steps_per_s": batched, "plain_particle_steps_per_s": plain,
             "batched_over_plain": batched / plain, "batched_ms_per_step": t_batch * 1e3 / steps,
             "plain_ms_per_step": t_plain * 1e3 / steps, "scenes_on_brick_kernels": bricks}
@@ -342,11 +347,14 @@ def main() -> int:
     ok = bool(np.isfinite(p_out).all() and np.isfinite(v_out).all())
     world.close()
 
-    mixed = None
+    mixed = mixed_fit = None
     if args.batch_steps > 0:
         print("[bench] mixed 256-scene batch vs plain", file=sys.stderr, flush=True)
         mixed = mixed_batch(args.batch_steps, 3)
         print(f"[bench] batched/plain = {mixed['batched_over_plain']:.2f}", file=sys.stderr, flush=True)
+        mixed_fit = mixed_batch(args.batch_steps, 3, fitting_only=True)
+        print(f"[bench] scenes whose block fits: batched/plain = {mixed_fit['batched_over_plain']:.2f}",
+              file=sys.stderr, flush=True)
 
     serve = None
     if args.serve_requests > 0:
@@ -385,6 +393,7 @@ def main() -> int:
         "pass_ms": pt,
         "pass_ms_by_kernel": variants,
         "mixed_batch": mixed,
+        "mixed_batch_fitting_scenes": mixed_fit,
         "serving": serve,
         "state_finite": ok,
         "cpu_baseline": cpu,

@@ -78,6 +78,13 @@ class SceneSpec:
         e.g. 2077 particles would have only 32 distinct keys."""
         return 1 << max(0, (self.n - 1).bit_length())
 
+    @property
+    def block_fits(self) -> bool:
+        """Whether the resting block at the target density fits under the box height (when
+        it does not, state() caps it at the box height and the scene starts over-dense)."""
+        bx, by, bz = self.box_max
+        return self.n / (self.target_density * bx * by) <= bz
+
     def state(self):
         """A resting block on the floor of the box at the scene's target density: number
         density n / volume = rho0 (the density estimate of a uniform fill equals its number
