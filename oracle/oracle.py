@@ -70,7 +70,7 @@ def load() -> C.CDLL:
         "sphref_density_map_dims": (None, [P(RefParams), C.c_float, vp]),
         "sphref_density_map": (None, [P(RefParams), u32, vp, vp, vp, C.c_float, vp]),
         "sphref64_substep": (C.c_int, [P(RefParams), u32, vp, vp, vp, vp, vp, vp, vp, vp, vp,
-                                       vp, vp]),
+                                       vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -153,7 +153,7 @@ def substep64(params, pos, vel, inter):
     rp.table_cells = rp.table_cells or n
     pos = np.ascontiguousarray(pos, np.float32)
     vel = np.ascontiguousarray(vel, np.float32)
-    o = {k: np.empty(n, np.float64) for k in ("density", "near_density")}
+    o = {k: np.empty(n, np.float64) for k in ("density", "near_density", "dv_scale")}
     for k in ("vel_force", "pos", "vel"):
         o[k] = np.empty((n, 3), np.float64)
     rc = lib.sphref64_substep(C.byref(rp), n, pos.ctypes.data, vel.ctypes.data,
@@ -161,7 +161,7 @@ def substep64(params, pos, vel, inter):
                               inter["end"].ctypes.data, inter["pred_pos"].ctypes.data,
                               o["density"].ctypes.data, o["near_density"].ctypes.data,
                               o["vel_force"].ctypes.data, o["pos"].ctypes.data,
-                              o["vel"].ctypes.data)
+                              o["vel"].ctypes.data, o["dv_scale"].ctypes.data)
     if rc != 0:
         raise RuntimeError("sphref64_substep failed")
     return o

@@ -136,11 +136,15 @@ void sphref_density_bruteforce(const sphref_params *p, uint32_t n, const float *
 /* --- FP64 oracle (sph_ref64.c) -------------------------------------------------------- */
 /* Given the FP32 run's integer decisions (sorted order, table, and the FP32 predicted
  * positions used to choose visited cells), recompute one substep in binary64.
- * pos4/vel4 are the substep's input state (input order); outputs are in sorted order. */
+ * pos4/vel4 are the substep's input state (input order); outputs are in sorted order.
+ * dv_scale (may be NULL) receives, per particle, the sum over its force pairs of the
+ * magnitudes of the pair's pressure and viscosity contributions to the velocity change,
+ * (|a + c| + |b| |v_i - u|) dt / rho: the scale against which the error of the net change
+ * (where pair terms cancel) is measured. */
 int sphref64_substep(const sphref_params *p, uint32_t n, const float *pos4, const float *vel4,
                      const uint32_t *sorted_idx, const int32_t *start, const int32_t *end,
                      const float *pred_pos4_f32, double *density, double *near_density,
-                     double *vel_force3, double *pos_out3, double *vel_out3);
+                     double *vel_force3, double *pos_out3, double *vel_out3, double *dv_scale);
 
 #ifdef __cplusplus
 }

@@ -44,7 +44,7 @@ static inline uint32_t key_of_cell(uint32_t mask, int32_t x, int32_t y, int32_t 
 int sphref64_substep(const sphref_params *p, uint32_t n, const float *pos4, const float *vel4,
                      const uint32_t *sorted_idx, const int32_t *start, const int32_t *end,
                      const float *pred_pos4_f32, double *density, double *near_density,
-                     double *vel_force3, double *pos_out3, double *vel_out3) {
+                     double *vel_force3, double *pos_out3, double *vel_out3, double *dv_scale) {
     sphref_derived d;
     sphref_derive(p, &d);
     const double R = p->smoothing_radius, m = p->particle_mass, rho0 = p->target_density;
@@ -111,7 +111,7 @@ int sphref64_substep(const sphref_params *p, uint32_t n, const float *pos4, cons
     for (int64_t j = 0; j < (int64_t)n; ++j) {
         const float *s32 = &pred_pos4_f32[4 * j];
         const double *s = &pps[3 * j], *u = &pvs[3 * j];
-        double Fp[3] = {0, 0, 0}, Fv[3] = {0, 0, 0};
+        double Fp[3] = {0, 0, 0}, Fv[3] = {0, 0, 0}, mag = 0.0;
         const double myd = density[j], near_p = near_density[j] * kn;
         int32_t c0[3];
         int outside = 0;
@@ -142,10 +142,13 @@ int sphref64_substep(const sphref_params *p, uint32_t n, const float *pos4, cons
                         double nrho_i = near_density[i];
                         double nslope = dist >= R ? 0.0 : (R - dist) * (R - dist) * near_slope_scale;
                         double c = (near_p + nrho_i * kn) / 2.0 * nslope * m / nrho_i;
+                        double du2 = 0.0;
                         for (int q = 0; q < 3; ++q) {
                             Fp[q] += (a + c) * (dv[q] / dist);
                             Fv[q] += b * (vs[3 * (int64_t)i + q] - u[q]);
+                            du2 += (vs[3 * (int64_t)i + q] - u[q]) * (vs[3 * (int64_t)i + q] - u[q]);
                         }
+                        mag += fabs(a + c) + fabs(b) * sqrt(du2);
                     }
                 }
         double v[3], x[3];
@@ -154,6 +157,7 @@ int sphref64_substep(const sphref_params *p, uint32_t n, const float *pos4, cons
             vel_force3[3 * j + q] = v[q];
             x[q] = ps[3 * j + q];
         }
+        if (dv_scale) dv_scale[j] = mag / myd * (double)p->dt;
         kick_drift(p, (double)d.integrate_margin, x, v);
         for (int q = 0; q < 3; ++q) {
             pos_out3[3 * j + q] = x[q];

@@ -60,8 +60,34 @@ def _check_substep_one(sph_hip, orc, params, pos, vel, kernel):
     return got, ref
 
 
-def _rel_err(x32, x64, scale):
-    return np.max(np.abs(x32.astype(np.float64) - x64) / scale)
+def _fp64_errors(gpu, orc, params, pos, vel, got, ref):
+    """Max and p99 error per quantity of the GPU substep against the FP64 oracle, each
+    relative to the scale its error is bounded by: densities to themselves; the force's
+    velocity change to the sum of the magnitudes of its pair terms (net forces cancel);
+    positions to their norm; final velocities to the sum of the magnitudes that make them
+    up (input velocity, pair terms, gravity).  Prints max and p99 of each, with final
+    velocities against their own norm for information, and returns the errors."""
+    p1 = gpu.SphParams.from_buffer_copy(params)
+    p1.substeps = 1
+    o64 = orc.substep64(p1, pos, vel, ref)
+    v0 = vel[ref["sorted_index"], :3].astype(np.float64)
+    dv_gpu = got["vel_force"][:, :3].astype(np.float64) - v0
+    dv_64 = o64["vel_force"] - v0
+    gdt = float(np.float32(p1.g)) * float(np.float32(p1.dt))
+    norm = lambda x: np.linalg.norm(x, axis=1)
+    vel_scale = norm(v0) + o64["dv_scale"] + gdt
+    err = {
+        "density": np.abs(got["density"] - o64["density"]) / o64["density"],
+        "near_density": np.abs(got["near_density"] - o64["near_density"]) / o64["near_density"],
+        "force dv": norm(dv_gpu - dv_64) / np.maximum(o64["dv_scale"], 1e-30),
+        "pos": norm(got["pos"][:, :3] - o64["pos"]) / norm(o64["pos"]),
+        "vel": norm(got["vel"][:, :3] - o64["vel"]) / vel_scale,
+    }
+    info = norm(got["vel"][:, :3] - o64["vel"]) / np.maximum(norm(o64["vel"]), 1e-30)
+    for name, e in err.items():
+        print(f"fp64 {name:13s} max {e.max():.3e}  p99 {np.percentile(e, 99):.3e}")
+    print(f"fp64 vel/|vel|      max {info.max():.3e}  p99 {np.percentile(info, 99):.3e} (information)")
+    return err
 
 
 def test_dam_break_16k_one_substep(gpu, orc):
@@ -69,10 +95,9 @@ def test_dam_break_16k_one_substep(gpu, orc):
     params = W.params()
     pos, vel = W.state()
     got, ref = _check_substep(gpu, orc, params, pos, vel)
-    o64 = orc.substep64(params, pos, vel, ref)
-    e_rho = _rel_err(got["density"], o64["density"], np.abs(o64["density"]))
-    assert e_rho <= REL_TOL, e_rho
-    print(f"density max rel err vs fp64: {e_rho:.3e}")
+    err = _fp64_errors(gpu, orc, params, pos, vel, got, ref)
+    for name, e in err.items():
+        assert e.max() <= REL_TOL, (name, e.max())
 
 
 def test_uniform_1m_one_substep_table_n(gpu, orc):
@@ -86,7 +111,17 @@ def test_uniform_1m_one_substep_table_2m(gpu, orc):
     from sph_hip.workloads import UNIFORM_1M as W
     params = W.params(table_cells=1 << 21)
     pos, vel = W.state()
-    _check_substep(gpu, orc, params, pos, vel)
+    got, ref = _check_substep(gpu, orc, params, pos, vel)
+    err = _fp64_errors(gpu, orc, params, pos, vel, got, ref)
+    # In a 40-unit box a float32 coordinate is quantised to ~3.8e-6, 1e-5 of R: the tolerance
+    # holds for the densities at the 99th percentile (their maxima reach ~5e-5, particles
+    # whose neighbours sit near distance R) and for every position.  The force's error is
+    # further scaled by the cancellation in (rho - rho0) of the pressure term, and a few
+    # particles at a wall take the reflection in one precision and not in the other, so
+    # those two are reported, not bounded, here (the dam break bounds them).
+    for name in ("density", "near_density"):
+        assert np.percentile(err[name], 99) <= REL_TOL, (name, np.percentile(err[name], 99))
+    assert err["pos"].max() <= REL_TOL, err["pos"].max()
 
 
 @pytest.mark.parametrize("kernel", ["direct", "brick"])
