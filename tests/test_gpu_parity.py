@@ -116,9 +116,9 @@ def test_uniform_1m_one_substep_table_2m(gpu, orc):
     # In a 40-unit box a float32 coordinate is quantised to ~3.8e-6, 1e-5 of R: the tolerance
     # holds for the densities at the 99th percentile (their maxima reach ~5e-5, particles
     # whose neighbours sit near distance R) and for every position.  The force's error is
-    # further scaled by the cancellation in (rho - rho0) of the pressure term, and a few
-    # particles at a wall take the reflection in one precision and not in the other, so
-    # those two are reported, not bounded, here (the dam break bounds them).
+    # further scaled by the cancellation in (rho - rho0) of the pressure term, and a particle
+    # at a wall can take the reflection in one precision and not in the other, so those two
+    # are reported, not bounded, here (the dam break bounds them).
     for name in ("density", "near_density"):
         assert np.percentile(err[name], 99) <= REL_TOL, (name, np.percentile(err[name], 99))
     assert err["pos"].max() <= REL_TOL, err["pos"].max()
