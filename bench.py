@@ -257,7 +257,7 @@ def serving(count: int, rate_hz: float, seed: int = 4, resident: int = 0,
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=100)  # the 1M configuration runs 100 steps
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="uniform_1m", choices=["uniform_1m", "dam_break_16k"])
     ap.add_argument("--table-cells", type=int, default=None)
