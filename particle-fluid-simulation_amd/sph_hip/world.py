@@ -64,6 +64,11 @@ class World:
         check(self._lib.sph_world_density_map(self._h, voxel_size, out.ctypes.data, out.size))
         return out
 
+    def density_map_device(self, voxel_size: float, dev_ptr: int, count: int) -> None:
+        """updateDensityMap into device memory at dev_ptr (count floats, x fastest), enqueued
+        on the world's stream: complete after sync()."""
+        check(self._lib.sph_world_density_map_device(self._h, voxel_size, C.c_void_p(dev_ptr), count))
+
     def download(self, color: bool = False):
         pos = np.empty((self.n, 4), np.float32)
         vel = np.empty((self.n, 4), np.float32)

@@ -44,6 +44,38 @@ def test_world_density_map_dam_break(gpu, orc, kernel):
     _bitexact("density map", got, ref)
 
 
+def test_world_density_map_device_matches_host(gpu):
+    """The device-memory entry point writes the same map, and nothing past it, into memory
+    the caller owns (allocated here through the HIP runtime the library runs on), and
+    refuses a buffer that is too small."""
+    import ctypes as C
+    from sph_hip.workloads import DAM_BREAK_16K as W
+    hip = C.CDLL("libamdhip64.so.7")
+    p = W.params()
+    pos, vel = W.state()
+    print("density map device dam_break_16k", flush=True)
+    with gpu.World(p, W.n) as w:
+        w.upload(pos, vel)
+        w.step(2)
+        host = w.density_map(0.1)
+        count = host.size + 64
+        dev = C.c_void_p()
+        assert hip.hipMalloc(C.byref(dev), C.c_size_t(count * 4)) == 0
+        try:
+            assert hip.hipMemset(dev, 0xFF, C.c_size_t(count * 4)) == 0  # NaN pattern
+            assert hip.hipDeviceSynchronize() == 0
+            w.density_map_device(0.1, dev.value, count)
+            w.sync()
+            got = np.empty(count, np.float32)
+            assert hip.hipMemcpy(C.c_void_p(got.ctypes.data), dev, C.c_size_t(count * 4), 2) == 0
+            with pytest.raises(gpu.SphError):
+                w.density_map_device(0.1, dev.value, host.size - 1)
+        finally:
+            hip.hipFree(dev)
+    _bitexact("device map", got[:host.size].reshape(host.shape), host)
+    assert np.all(got[host.size:].view(np.uint32) == 0xFFFFFFFF)
+
+
 def test_world_density_map_uniform_1m(gpu, orc):
     from sph_hip.workloads import UNIFORM_1M as W
     p = W.params(table_cells=1 << 21)
