@@ -110,6 +110,7 @@ struct OneScene {
     __device__ __forceinline__ const SceneConst &scene(uint32_t) const { return c; }
     __device__ __forceinline__ const SceneConst &scene_of_brick(uint32_t) const { return c; }
     __device__ __forceinline__ const SceneConst &scene_of_grid(uint32_t) const { return c; }
+    __device__ __forceinline__ const SceneConst &scene_of_grid_near(uint32_t, uint32_t) const { return c; }
 };
 
 struct ManyScenes {
@@ -134,6 +135,18 @@ struct ManyScenes {
             const uint32_t mid = (lo + hi) >> 1;
             if (table[mid].grid_base <= g) lo = mid; else hi = mid;
         }
+        return table[lo];
+    }
+    // The same for a grid entry `g` at or after `g0`, an entry the whole wave shares: the
+    // search runs once per wave on g0 (scalar loads), and a lane steps forward only across
+    // the scene boundaries between g0 and g.
+    __device__ __forceinline__ SceneConst scene_of_grid_near(uint32_t g0, uint32_t g) const {
+        uint32_t lo = 0, hi = count;
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (table[mid].grid_base <= g0) lo = mid; else hi = mid;
+        }
+        while (lo + 1u < count && table[lo + 1u].grid_base <= g) ++lo;
         return table[lo];
     }
 };

@@ -154,8 +154,9 @@ template <class Scenes>
 __global__ __launch_bounds__(kBlock) void k_grid(Scenes sc, uint32_t grid_total,
                                                  const int2 *__restrict__ crun, int2 *__restrict__ grid) {
     const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t g0 = __builtin_amdgcn_readfirstlane(g);  // the wave's first entry
     if (g >= grid_total) return;
-    const SceneConst c = sc.scene_of_grid(g);
+    const SceneConst c = sc.scene_of_grid_near(g0, g);
     const uint32_t hx = (uint32_t)c.gx + 2u, hy = (uint32_t)c.gy + 2u;
     const uint32_t l = g - c.grid_base;
     const uint32_t lx = l % hx, ly = (l / hx) % hy, lz = l / (hx * hy);
