@@ -44,6 +44,32 @@ def flops_per_particle_substep(workload) -> dict:
     return {"candidates": C, "neighbours": K, "flops": 10 * C + 8 * K + 9 * C + 50 * K}
 
 
+def density_map_timing(world, voxel_size: float, reps: int = 5) -> dict:
+    """updateDensityMap of the world's last substep into device memory (allocated through
+    the HIP runtime the library runs on), best of `reps` enqueue + sync; each voxel walks
+    the 27 cells around it."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")
+    dims = world.density_map_dims(voxel_size)
+    voxels = dims[0] * dims[1] * dims[2]
+    dev = C.c_void_p()
+    if hip.hipMalloc(C.byref(dev), C.c_size_t(voxels * 4)) != 0:
+        raise RuntimeError("hipMalloc of the density map failed")
+    try:
+        world.density_map_device(voxel_size, dev.value, voxels)
+        world.sync()
+        best = float("inf")
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            world.density_map_device(voxel_size, dev.value, voxels)
+            world.sync()
+            best = min(best, time.perf_counter() - t0)
+    finally:
+        hip.hipFree(dev)
+    return {"voxel_size": voxel_size, "dims": list(dims), "voxels": voxels, "ms": best * 1e3,
+            "voxels_per_s": voxels / best}
+
+
 def copy_bandwidth_gbs(mib: int = 512, reps: int = 5) -> float:
     """Device-to-device copy rate (bytes read + written per second), best of `reps`: the
     measured counterpart of the HBM peak."""
@@ -341,6 +367,10 @@ def main() -> int:
     work = flops_per_particle_substep(W)
     tflops = W.n * work["flops"] / substep_s / 1e12
 
+    # updateDensityMap of the last substep at a voxel of R/3.5 (16M voxels for uniform_1m,
+    # the size of the reference's own 560 x 160 x 160 image)
+    dmap = density_map_timing(world, 0.1)
+
     # Sanity: the state stays finite and in the box.
     p_out, v_out = world.download()
     import numpy as np
@@ -392,6 +422,7 @@ def main() -> int:
         "kernels": chosen,
         "pass_ms": pt,
         "pass_ms_by_kernel": variants,
+        "density_map": dmap,
         "mixed_batch": mixed,
         "mixed_batch_fitting_scenes": mixed_fit,
         "serving": serve,
