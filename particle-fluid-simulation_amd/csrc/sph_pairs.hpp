@@ -109,10 +109,12 @@ struct GridRuns {
         if (lx < hx && ly < hy && lz < hz) return grid[grid_base + lx + hx * (ly + hy * lz)];
         return slow_raw(cx, cy, cz);
     }
-    // The three cells (cx - 1 .. cx + 1, cy, cz) of a row: consecutive grid entries.
+    // The three cells (cx - 1 .. cx + 1, cy, cz) of a row: consecutive grid entries.  The x
+    // test is lx < hx - 2 (hx >= 2), not lx + 2 < hx, which wraps for cx = -1 and -2 (the
+    // cell of a point outside the box) and would admit an index 2^32 - 1 past the row.
     __device__ __forceinline__ void row3(int cx, int cy, int cz, int2 out[3]) const {
         const uint32_t lx = (uint32_t)cx, ly = (uint32_t)(cy + 1), lz = (uint32_t)(cz + 1);
-        if (lx + 2u < hx && ly < hy && lz < hz) {
+        if (lx < hx - 2u && ly < hy && lz < hz) {
             const int2 *g = grid + grid_base + lx + hx * (ly + hy * lz);
             out[0] = g[0];
             out[1] = g[1];
