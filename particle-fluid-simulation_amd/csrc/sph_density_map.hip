@@ -8,9 +8,16 @@
 // by the cells the particles were hashed in at the start of that substep.  As in the
 // shader, cells are bounds-checked against the voxel counts, not the cell counts, and the
 // corner voxel (M, M, M) is skipped.  Lanes of a wave hold neighbouring voxels, which share
-// most of their cells, so the runs they read are shared through the caches.
+// most of their cells.
+//
+// After a substep of the per-lane kernels the same runs, in the same order, are also laid
+// out in cell order (the force pass stores the new positions there too), where the runs of
+// neighbouring cells sit next to each other; the map then reads them through the halo grid
+// and the cell-ordered run table instead of the hash-order table.  Either source gives the
+// same sums bit for bit.
 #include "sph_device.hpp"
 #include "sph_map.hpp"
+#include "sph_pairs.hpp"
 
 namespace sph {
 
@@ -24,8 +31,21 @@ __device__ __forceinline__ bool map_out_of_bounds(int x, int y, int z, const Map
     return x == g.mx && y == g.my && z == g.mz;
 }
 
+// Runs of the hash-order table: (start, end) of the cell's key, start -1 when empty.
+struct TableRuns {
+    const int2 *__restrict__ table;
+    uint32_t table_base, mask;
+    __device__ __forceinline__ int2 raw(int x, int y, int z) const {
+        return table[table_base + (cell_hash((uint32_t)x, (uint32_t)y, (uint32_t)z) & mask)];
+    }
+    __device__ __forceinline__ void row3(int x, int y, int z, int2 out[3]) const {
+        for (int d = 0; d < 3; ++d) out[d] = raw(x + d - 1, y, z);
+    }
+};
+
+template <class Runs>
 __global__ __launch_bounds__(kBlock) void k_density_map(SceneConst c, const float4 *__restrict__ pos,
-                                                       const int2 *__restrict__ table, MapGeometry g,
+                                                       Runs runs, MapGeometry g,
                                                        float *__restrict__ out) {
     const uint64_t v = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
     if (v >= g.voxels) return;
@@ -45,24 +65,35 @@ __global__ __launch_bounds__(kBlock) void k_density_map(SceneConst c, const floa
     const int oz = outside ? -1 : cell_of(cz, c.minz, c.R);
     const float R = c.R, R2 = c.R2;
     float density = 0.0f;
+    constexpr int kU = 4;  // candidates of a run loaded together
     for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int x = ox + dx, y = oy + dy, z = oz + dz;
-                if (map_out_of_bounds(x, y, z, g)) continue;
-                const int2 se = table[c.table_base + (cell_hash((uint32_t)x, (uint32_t)y, (uint32_t)z) & c.mask)];
-                if (se.x == -1) continue;
-                for (int i = se.x; i < se.y; ++i) {
-                    const float4 q = pos[i];
-                    const float ex = cx - q.x, ey = cy - q.y, ez = cz - q.z;
-                    const float d2 = ex * ex + ey * ey + ez * ez;
-                    if (d2 < R2) {
-                        const float dist = sqrtf(d2);
-                        const float k = dist >= R ? 0.0f : (R - dist) * (R - dist);
-                        density = density + c.m * k;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int y = oy + dy, z = oz + dz;
+            if (y < 0 || z < 0 || y > g.my || z > g.mz) continue;  // the whole row is out
+            int2 se[3];
+            runs.row3(ox, y, z, se);  // the row's three cells, loaded together
+            for (int dx = 0; dx < 3; ++dx) {
+                if (map_out_of_bounds(ox + dx - 1, y, z, g)) continue;
+                if (se[dx].x == -1) continue;
+                const float4 *src = pos + se[dx].x;
+                const int cnt = se[dx].y - se[dx].x;
+                for (int k = 0; k < cnt; k += kU) {
+                    float4 p[kU];
+                    load_candidates<kU>(src, k, cnt, p);
+#pragma unroll
+                    for (int u = 0; u < kU; ++u) {
+                        if (k + u >= cnt) continue;
+                        const float ex = cx - p[u].x, ey = cy - p[u].y, ez = cz - p[u].z;
+                        const float d2 = ex * ex + ey * ey + ez * ez;
+                        if (d2 < R2) {
+                            const float dist = sqrtf(d2);
+                            const float t = dist >= R ? 0.0f : (R - dist) * (R - dist);
+                            density = density + c.m * t;
+                        }
                     }
                 }
             }
+        }
     density = density * 1.0f;
     out[v] = density / c.volume;
 }
@@ -99,10 +130,20 @@ int map_geometry(const sph_params &p, float voxel_size, MapGeometry &g) {
     return SPH_OK;
 }
 
-hipError_t launch_density_map(hipStream_t stream, const SceneConst &c, const float4 *pos,
-                              const int2 *table, const MapGeometry &g, float *out) {
+hipError_t launch_density_map(hipStream_t stream, const SceneConst &c, const Chain &chain,
+                              const MapGeometry &g, float *out) {
     const uint64_t blocks = (g.voxels + kBlock - 1u) / kBlock;
-    hipLaunchKernelGGL(k_density_map, dim3((uint32_t)blocks), dim3(kBlock), 0, stream, c, pos, table, g, out);
+    const ChainBuffers &b = chain.b;
+    if (chain.cell_order) {
+        const GridRuns runs{b.grid, c.grid_base, (uint32_t)c.gx + 2u, (uint32_t)c.gy + 2u,
+                            c.grid_cells ? (uint32_t)c.gz + 2u : 0u, c.table_base, c.mask, b.crun};
+        hipLaunchKernelGGL(k_density_map<GridRuns>, dim3((uint32_t)blocks), dim3(kBlock), 0, stream, c,
+                           (const float4 *)b.pos_sorted, runs, g, out);
+    } else {
+        const TableRuns runs{b.table, c.table_base, c.mask};
+        hipLaunchKernelGGL(k_density_map<TableRuns>, dim3((uint32_t)blocks), dim3(kBlock), 0, stream, c,
+                           (const float4 *)b.pos, runs, g, out);
+    }
     return hipGetLastError();
 }
 

@@ -224,6 +224,9 @@ __global__ __launch_bounds__(kB) void k_force(Scenes sc, const ChainBuffers b, u
     const uint32_t o = b.hslot[j];
     b.pos[o] = po;
     b.vel[o] = vo;
+    // the new position also in cell order, for a density map of this substep (no lane reads
+    // pos_sorted but its own, before this store; the next placement overwrites it)
+    b.pos_sorted[j] = po;
 }
 
 inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1u) / kBlock; }

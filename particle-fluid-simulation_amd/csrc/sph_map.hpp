@@ -22,9 +22,9 @@ struct MapGeometry {
 // Fills g for the scene's box; SPH_ERR_INVALID_ARG / SPH_ERR_CAPACITY on a bad voxel size.
 int map_geometry(const sph_params &p, float voxel_size, MapGeometry &g);
 
-// Writes g.voxels floats (x fastest) to device memory `out` on `stream`.  `pos` and
-// `table` are the chain's state and cell table after at least one substep.
-hipError_t launch_density_map(hipStream_t stream, const SceneConst &c, const float4 *pos,
-                              const int2 *table, const MapGeometry &g, float *out);
+// Writes g.voxels floats (x fastest) to device memory `out` on `stream`, from the state
+// and cell table `chain` holds after at least one substep of the scene `c`.
+hipError_t launch_density_map(hipStream_t stream, const SceneConst &c, const Chain &chain,
+                              const MapGeometry &g, float *out);
 
 }  // namespace sph

@@ -742,7 +742,7 @@ int sph_world_density_map_device(sph_world *w, float voxel_size, float *dev_out,
     if (rc) return rc;
     if (count < g.voxels) return fail(SPH_ERR_INVALID_ARG, "density map needs %llu floats, got %zu",
                                       (unsigned long long)g.voxels, count);
-    hipError_t e = launch_density_map(w->stream, w->scene.c, w->chain.b.pos, w->chain.b.table, g, dev_out);
+    hipError_t e = launch_density_map(w->stream, w->scene.c, w->chain, g, dev_out);
     return e == hipSuccess ? SPH_OK : hip_fail(e, "density map launch");
 }
 
@@ -763,7 +763,7 @@ int sph_world_density_map(sph_world *w, float voxel_size, float *out, size_t cou
         if (e != hipSuccess) return fail(SPH_ERR_OUT_OF_MEMORY, "density map buffer: %s", hipGetErrorString(e));
         w->map_cap = g.voxels;
     }
-    hipError_t e = launch_density_map(w->stream, w->scene.c, w->chain.b.pos, w->chain.b.table, g, w->d_map);
+    hipError_t e = launch_density_map(w->stream, w->scene.c, w->chain, g, w->d_map);
     if (e == hipSuccess)
         e = hipMemcpyAsync(out, w->d_map, g.voxels * sizeof(float), hipMemcpyDeviceToHost, w->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
