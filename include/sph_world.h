@@ -214,7 +214,7 @@ void sph_batch_destroy(sph_batch *b);
  * Serving: a request loop over the batched step.  No reference counterpart (the reference
  * runs one interactive scene).  Requests carry their parameters, initial state and step
  * count; a worker thread admits them in arrival order while the sum of resident footprints
- * (648 B per particle, 24 B per table cell, 8 B per halo-grid cell) stays within
+ * (1152 B per particle, 24 B per table cell, 8 B per halo-grid cell) stays within
  * hbm_budget_bytes and the capacities hold, keeps their state resident in device memory,
  * steps every resident scene together one frame at a time, and retires finished scenes
  * with their state copied back.

@@ -155,6 +155,12 @@ struct ManyScenes {
 // in LDS and evaluates every particle whose current cell lies in the brick.
 constexpr int kBrick = 4;
 
+// Force neighbour lists: entries per particle (a count fits one byte, 0xFF marks "no
+// list"), so that the densest packed scenes (a few hundred candidates, up to ~200
+// neighbours) still read a list rather than walk their 27 cells.
+constexpr int kList = 254;
+constexpr uint8_t kListInvalid = 0xFF;
+
 // The neighbour walk of the density pass (calculateDensities.comp:114-131) for sample
 // point s.  Per axis only three sample coordinates exist (s + R*{-1,0,1}), so their cell
 // coordinates and in-box flags are computed once; the 27 cells are then visited in the

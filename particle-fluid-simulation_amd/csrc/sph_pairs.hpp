@@ -515,9 +515,7 @@ __device__ __forceinline__ void force_finish(const SceneConst &c, const float4 &
 // which the cell relation fails (a rounded sample cell that is not c0 -/+ 1, a predicted
 // position outside the box), or whose list would exceed kList entries, gets the density's
 // own walk and an invalid count, and the force pass walks its cells.
-constexpr int kList = 128;     // entries per particle in the global list
 constexpr int kListLds = 20;   // entries staged in LDS (the rest are stored directly)
-constexpr uint8_t kListInvalid = 0xFF;
 
 template <int kU>
 __device__ __forceinline__ float2 density_with_list(const SceneConst &c, uint32_t j, const float4 &s,

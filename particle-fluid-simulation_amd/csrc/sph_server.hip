@@ -96,7 +96,7 @@ namespace {
 
 // Device bytes a resident scene occupies (the chain's per-slot buffers and its table).
 uint64_t scene_footprint(uint32_t n, uint32_t T, uint32_t grid_cells) {
-    return (uint64_t)n * 648u + (uint64_t)T * 24u + (uint64_t)grid_cells * 8u;
+    return (uint64_t)n * (136u + 4u * kList) + (uint64_t)T * 24u + (uint64_t)grid_cells * 8u;
 }
 
 bool fits(const sph_server *s, const Request *r) {
