@@ -105,10 +105,14 @@ KERNEL_BYTES_PER_PARTICLE = {"density": 24 + 8, "force": 24 + 8 + 32}
 KERNEL_NAMES = {"density": "k_density (calculateDensities + force neighbour lists)",
                 "force": "k_force (applyPressureAndViscosityForce + updateVelocities)"}
 
-# HBM bytes per launch of the neighbour kernels on uniform_1m (T = 2^21) measured with
-# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (profiles/pmc_uniform_1m_neighbour_passes.md),
-# each counter scaled by what it reads for a 64 MiB device-to-device copy in the same run.
-TRAFFIC_BYTES_PER_LAUNCH = {"density": (2 * 56080 + 77389) * 1024, "force": (2 * 161667 + 42456) * 1024}
+# HBM bytes per launch of the neighbour kernels on uniform_1m (T = 2^21) after the default
+# 5 warmup + 100 timed steps (the fill settles and grows denser as it runs, so the figure
+# holds for that point only), measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE over
+# the launches that follow the timed steps (profiles/pmc_uniform_1m_neighbour_passes.md),
+# each counter scaled by what it reads for device-to-device copies of known size in the
+# same run (FETCH x 2, WRITE x 1).
+TRAFFIC_BYTES_PER_LAUNCH = {"density": (2 * 59429 + 147236) * 1024, "force": (2 * 179891 + 58743) * 1024}
+TRAFFIC_STEPS = (5, 100)  # (warmup, steps) the traffic figures were measured after
 
 
 def cpu_baseline(workload, table_cells: int, budget_s: float = 12.0) -> dict:
@@ -409,7 +413,8 @@ def main() -> int:
                    "smoothing_radius": W.smoothing_radius, "path": "plain (one scene per launch)"},
         "roofline": {"bound": "hbm", "achieved": dom_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": dom_gbs / HBM_PEAK_GBS,
-                     "traffic": TRAFFIC_BYTES_PER_LAUNCH[dom] if T == 1 << 21 and W is wl.UNIFORM_1M else None,
+                     "traffic": (TRAFFIC_BYTES_PER_LAUNCH[dom] if T == 1 << 21 and W is wl.UNIFORM_1M and
+                                 (args.warmup, args.steps) == TRAFFIC_STEPS else None),
                      "kernel": KERNEL_NAMES[dom], "avg_launch_ms": pt[f"{dom}_ms"],
                      "alg_bytes_per_launch": dom_bytes},
         "chain_roofline": {"alg_bytes_per_particle_substep": b_alg, "achieved_gbs": chain_gbs,
