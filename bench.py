@@ -137,8 +137,9 @@ def cpu_baseline(workload, table_cells: int, budget_s: float = 12.0) -> dict:
         "value": workload.n * frames / dt, "unit": "particle-steps/s", "cores": threads,
         "kind": "port",
         "sample": f"{substeps} substeps ({frames:.2f} steps) of {workload.name} "
-                  f"(N={workload.n}, T={table_cells}) in {dt:.1f} s; FP32 C restatement of the "
-                  f"reference shaders, OpenMP over particles",
+                  f"(N={workload.n}, T={table_cells}) in {dt:.1f} s; reference-semantics CPU "
+                  f"restatement (FP32 C of the reference shaders, OpenMP over particles), not "
+                  f"lavapipe, which this image lacks",
     }
 
 
