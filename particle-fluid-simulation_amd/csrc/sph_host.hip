@@ -90,6 +90,10 @@ int dump_stage(const Chain &c, hipStream_t s, int stage, uint32_t p_begin, uint3
             if (!src) return fail(SPH_ERR_STATE, "stage %d needs debug capture (sph_world_set_debug)", stage);
             std::vector<float> v;
             if ((rc = copy_back(s, v, src + p_begin, 4 * n))) return rc;
+            // the predicted position's w is 1 (predictPositions.comp); the buffer's w lane
+            // carries the near density once the per-lane density pass has run
+            if (stage == SPH_DUMP_PRED_POS)
+                for (size_t i = 0; i < n; ++i) v[4 * i + 3] = 1.0f;
             for (size_t i = 0; i < n; ++i) std::memcpy((float *)dst + 4 * at(i), &v[4 * i], 16);
             break;
         }

@@ -184,8 +184,14 @@ __global__ __launch_bounds__(kB) void k_density(Scenes sc, const ChainBuffers b,
         return;
     }
     uint32_t count;
-    b.dens[j] = density_with_list<kU>(c, j, s, b.pred_pos, grid_runs(c, b), s_list + threadIdx.x, kB,
-                                      b.nlist, n, count);
+    const float2 d = density_with_list<kU>(c, j, s, b.pred_pos, grid_runs(c, b), s_list + threadIdx.x, kB,
+                                           b.nlist, n, count);
+    b.dens[j] = d;
+    // The densities also ride in the w lanes of the particle's predicted velocity (density)
+    // and predicted position (near density), so that the force pass gathers a listed
+    // neighbour with two loads.  Every reader of these arrays uses their xyz only.
+    reinterpret_cast<float *>(b.vel_sorted + j)[3] = d.x;
+    reinterpret_cast<float *>(b.pred_pos + j)[3] = d.y;
     b.ncount[j] = (uint8_t)count;
     // entry-major copy-out: every store of the loop writes one coalesced row
     const uint32_t m = count == kListInvalid ? 0u : min(count, (uint32_t)kListLds);
@@ -213,15 +219,19 @@ __global__ __launch_bounds__(kB) void k_force(Scenes sc, const ChainBuffers b, u
     const uint32_t cnt = kL ? b.ncount[j] : kListInvalid;
     if (cnt != kListInvalid) {
         // the recorded neighbours, in the walk's order
-        force_list<8>(c, me, cnt, b.nlist, n, b.pred_pos, b.vel_sorted, b.dens, Fp, Fv);
+        force_list<8>(c, me, cnt, b.nlist, n, b.pred_pos, b.vel_sorted, Fp, Fv);
     } else {
         const ForceCellsOf<GridRuns> cells{grid_runs(c, b), force_cell(c, s)};
         force_walk_rows<kU>(c, me, b.pred_pos, b.vel_sorted, b.dens, cells, s_queue + threadIdx.x, kB, Fp, Fv);
     }
     float4 po, vo, vf;
     force_finish(c, P, V, myd.x, Fp, Fv, po, vo, vf);
-    if (b.dbg_vel_force) b.dbg_vel_force[j] = vf;
     const uint32_t o = b.hslot[j];
+    if (b.dbg_vel_force) {
+        // V.w holds the density when lists are recorded; the velocity's own w is the state's
+        if (kL) vf.w = b.vel[o].w + 1.0f;
+        b.dbg_vel_force[j] = vf;
+    }
     b.pos[o] = po;
     b.vel[o] = vo;
     // the new position also in cell order, for a density map of this substep (no lane reads

@@ -298,30 +298,29 @@ __device__ __forceinline__ void force_term(const SceneConst &c, const ForceSelf 
 
 // The force terms of a recorded neighbour list (entry i at list[i * n + j]), in order.
 // kU entries and their neighbours' data are loaded before any term is evaluated (the
-// pass waits on these dependent loads, not on the arithmetic).
+// pass waits on these dependent loads, not on the arithmetic).  The density pass that
+// recorded the list left each particle's density in vel_sorted[].w and its near density
+// in pred_pos[].w, so a neighbour is two 16-byte loads.
 template <int kU>
 __device__ __forceinline__ void force_list(const SceneConst &c, const ForceSelf &me, uint32_t cnt,
                                            const uint32_t *__restrict__ list, uint32_t n,
                                            const float4 *__restrict__ pred_pos,
-                                           const float4 *__restrict__ vel_sorted,
-                                           const float2 *__restrict__ dens, float3 &Fp, float3 &Fv) {
+                                           const float4 *__restrict__ vel_sorted, float3 &Fp, float3 &Fv) {
     for (uint32_t i = 0; i < cnt; i += kU) {
         uint32_t g[kU];
 #pragma unroll
         for (int u = 0; u < kU; ++u) g[u] = list[(size_t)min(i + (uint32_t)u, cnt - 1u) * n + me.j];
         float4 a[kU], v[kU];
-        float2 d[kU];
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
             a[u] = pred_pos[g[u]];
             v[u] = vel_sorted[g[u]];
-            d[u] = dens[g[u]];
         }
 #pragma unroll
         for (int u = 0; u < kU; ++u)
             if (i + (uint32_t)u < cnt)
-                force_pair(c, me, make_float3(a[u].x, a[u].y, a[u].z), make_float3(v[u].x, v[u].y, v[u].z), d[u],
-                           Fp, Fv);
+                force_pair(c, me, make_float3(a[u].x, a[u].y, a[u].z), make_float3(v[u].x, v[u].y, v[u].z),
+                           make_float2(v[u].w, a[u].w), Fp, Fv);
     }
 }
 
