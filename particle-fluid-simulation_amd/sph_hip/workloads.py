@@ -7,6 +7,9 @@
   uniform_1m    : 1,048,576 particles uniform over a 40x20x20 box, velocities N(0, 0.1);
                   same kernel parameters; T = 2^21 (throughput) or 2^20 (= N, the
                   reference's table size).
+  uniform_32m   : 33,554,432 particles uniform over a 160x40x80 box, velocities N(0, 0.1);
+                  same kernel parameters; T = N.  The resident scene of the mixed serving
+                  workload.
 """
 from __future__ import annotations
 
@@ -52,6 +55,8 @@ DAM_BREAK_16K = Workload("dam_break_16k", 16384, (10.0, 6.0, 6.0), (0.0, 0.0, 0.
                          (4.0, 6.0, 4.0), 0.0, 1)
 UNIFORM_1M = Workload("uniform_1m", 1 << 20, (40.0, 20.0, 20.0), (0.0, 0.0, 0.0),
                       (40.0, 20.0, 20.0), 0.1, 2)
+UNIFORM_32M = Workload("uniform_32m", 1 << 25, (160.0, 40.0, 80.0), (0.0, 0.0, 0.0),
+                       (160.0, 40.0, 80.0), 0.1, 5)
 
 
 @dataclass(frozen=True)

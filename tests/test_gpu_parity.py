@@ -124,6 +124,26 @@ def test_uniform_1m_one_substep_table_2m(gpu, orc):
     assert err["pos"].max() <= REL_TOL, err["pos"].max()
 
 
+def test_uniform_32m_one_substep(gpu, orc):
+    """Full size of the largest scene: 2^25 particles uniform over a 160x40x80 box, T = N.
+    Every stage of one substep is bit-exact against the oracle on both kernel paths."""
+    from sph_hip.workloads import UNIFORM_32M as W
+    params = W.params(table_cells=0)
+    pos, vel = W.state()
+    p1 = gpu.SphParams.from_buffer_copy(params)
+    p1.substeps = 1
+    print(f"oracle n={W.n}", flush=True)
+    rpos, rvel, ref = orc.substep(p1, pos, vel)
+    for kernel in ("direct", "brick"):
+        print(f"case n={W.n} T={W.n} kernel={kernel}", flush=True)
+        got = _run_one_substep(gpu, params, pos, vel, kernel)
+        for s in STAGES_INT + STAGES_F32:
+            _assert_bitexact(f"{kernel}:{s}", got[s], ref[s])
+        _assert_bitexact(f"{kernel}:pos", got["pos"], rpos)
+        _assert_bitexact(f"{kernel}:vel", got["vel"], rvel)
+        del got
+
+
 @pytest.mark.parametrize("kernel", ["direct", "brick"])
 def test_dam_break_multi_frame_bitexact(gpu, orc, kernel):
     """10 frames x 3 substeps: the whole trajectory matches the oracle bit for bit."""
