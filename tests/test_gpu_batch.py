@@ -79,3 +79,24 @@ def test_scene_alone_equals_scene_in_mix(gpu):
         mixed = b.download(k)
     _bitexact("pos", mixed[0], alone[0])
     _bitexact("vel", mixed[1], alone[1])
+
+
+def test_full_mixed_batch_one_frame(gpu, orc):
+    """Full size of the mixed workload: its 256 scenes (5.4M particles, 47 of them packed
+    above their target density, whose particles record up to ~200 neighbours) stepped one
+    frame together; every scene's state is bit-exact against the oracle run on it alone."""
+    from sph_hip.workloads import mixed_scenes
+    specs = mixed_scenes()
+    params = [s.params() for s in specs]
+    states = [s.state() for s in specs]
+    print(f"full mixed batch: {len(specs)} scenes, {sum(s.n for s in specs)} particles", flush=True)
+    with gpu.Batch(params, [s.n for s in specs]) as b:
+        for i, (p, v) in enumerate(states):
+            b.upload(i, p, v)
+        b.step(1)
+        got = [b.download(i) for i in range(len(specs))]
+    print("oracle", flush=True)
+    for i, (p, v) in enumerate(states):
+        rp, rv = orc.step(params[i], p, v, 1)
+        _bitexact(f"scene {i} pos", got[i][0], rp)
+        _bitexact(f"scene {i} vel", got[i][1], rv)
