@@ -25,9 +25,11 @@ struct ChainBuffers {
     uint32_t *keys;      // cell key per slot (hash pass output)
     uint32_t *sort_k[2]; // radix ping-pong
     uint32_t *sort_v[2];
-    float4 *pred_pos;    // predicted positions, sorted order
-    float4 *pos_sorted;  // positions, sorted order
-    float4 *vel_sorted;  // velocities, sorted order
+    float4 *pred_pos;    // predicted positions, sorted order (w: near density once the
+                         // list-recording density pass has run, else 1)
+    float4 *pos_sorted;  // positions, sorted order (after a per-lane force pass: the new ones)
+    float4 *vel_sorted;  // velocities, sorted order (w: density once the list-recording
+                         // density pass has run)
     float2 *dens;        // (density, near density), sorted order
     int2 *table;         // (start, end) per cell, -1 = empty
     uint32_t *ghist;     // [4][256] digit histograms
