@@ -202,6 +202,27 @@ def test_tiny_box_strays(gpu, orc):
         _assert_bitexact(f"{kernel}:vel", gv, rv)
 
 
+def test_box_thinner_than_particle_radius(gpu, orc):
+    """A box 0.2 high with particle radius 0.3: predict clamps y to min + r = 0.3, outside
+    the box, so every particle's force cell is (-1, -1, -1) and its row lookups start left
+    of the halo grid (the case whose bounds test once wrapped); two substeps bit-exact."""
+    import sph_hip
+    params = sph_hip.params_for(particle_radius=0.3, smoothing_radius=0.35, g=9.8,
+                                box_min=(0, 0, 0), box_max=(10, 0.2, 6), frame_dt=1 / 120,
+                                substeps=1, pressure_multiplier=500, target_density=55)
+    pos, vel = gpu.fill_block(2000, 6, (0, 0, 0), (10, 0.2, 6), 1.0)
+    rp, rv = orc.step(params, pos, vel, 2)
+    for kernel in ("direct", "brick"):
+        print(f"case thin box kernel={kernel}", flush=True)
+        with gpu.World(params, 2000) as w:
+            w.set_kernel(kernel)
+            w.upload(pos, vel)
+            w.step(2)
+            gp, gv = w.download()
+        _assert_bitexact(f"{kernel}:pos", gp, rp)
+        _assert_bitexact(f"{kernel}:vel", gv, rv)
+
+
 @pytest.mark.parametrize("n,T", [(1, 1), (2, 0), (3000, 0), (5000, 1 << 16), (4096, 1 << 24),
                                  (777, 5)])
 def test_edge_sizes(gpu, orc, n, T):
