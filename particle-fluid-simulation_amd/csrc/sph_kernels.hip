@@ -169,12 +169,15 @@ __device__ __forceinline__ uint32_t xcd_block() {
     return x * q + min(x, r) + b / kXcds;
 }
 
-// Occupancy floors: the batch instances (scene lookup per lane) would otherwise take 79
-// and 130 VGPRs, one wave per SIMD fewer than the single-scene ones; held at 7 and 4 waves
-// (3 spilled registers in the batch density pass) the batched step runs ~1.5 % faster.
+// Occupancy floors.  The single-scene density pass at 8 waves per SIMD (64 VGPRs, 2
+// spilled; its 128-lane blocks' list staging then fills the CU's 160 KB of LDS exactly)
+// runs 5 % faster than at the 7 its 66 VGPRs allow.  The batch instances (scene lookup per
+// lane) would take 79 and 130 VGPRs, one wave per SIMD below the single-scene ones; held
+// at 7 and 4 waves (3 spilled registers in the batch density pass) the batched step runs
+// ~1.5 % faster.
 template <class Scenes, int kU, int kB, bool kL>
-__global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(7))) void k_density(Scenes sc, const ChainBuffers b,
-                                                                                          uint32_t n) {
+__global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(std::is_same_v<Scenes, OneScene> ? 8 : 7)))
+void k_density(Scenes sc, const ChainBuffers b, uint32_t n) {
     __shared__ uint32_t s_list[kL ? kListLds * kB : 1];
     const uint32_t j = xcd_block() * kB + threadIdx.x;
     if (j >= n) return;
