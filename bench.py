@@ -111,7 +111,7 @@ KERNEL_NAMES = {"density": "k_density (calculateDensities + force neighbour list
 # the launches that follow the timed steps (profiles/pmc_uniform_1m_neighbour_passes.md),
 # each counter scaled by what it reads for device-to-device copies of known size in the
 # same run (FETCH x 2, WRITE x 1).
-TRAFFIC_BYTES_PER_LAUNCH = {"density": (2 * 59429 + 147236) * 1024, "force": (2 * 179891 + 58743) * 1024}
+TRAFFIC_BYTES_PER_LAUNCH = {"density": (2 * 66545 + 194955) * 1024, "force": (2 * 150344 + 58812) * 1024}
 TRAFFIC_STEPS = (5, 100)  # (warmup, steps) the traffic figures were measured after
 
 
