@@ -394,7 +394,7 @@ int sph_batch_density_map(sph_batch *b, uint32_t scene, float voxel_size, float 
         if (e != hipSuccess) return fail(SPH_ERR_OUT_OF_MEMORY, "density map buffer: %s", hipGetErrorString(e));
         b->map_cap = g.voxels;
     }
-    hipError_t e = launch_density_map(b->stream, b->scenes[scene], b->chain, g, b->d_map);
+    hipError_t e = launch_density_map(b->stream, b->scenes[scene], b->chain, true, g, b->d_map);
     if (e == hipSuccess)
         e = hipMemcpyAsync(out, b->d_map, g.voxels * sizeof(float), hipMemcpyDeviceToHost, b->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);

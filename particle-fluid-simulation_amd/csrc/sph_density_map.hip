@@ -131,12 +131,14 @@ int map_geometry(const sph_params &p, float voxel_size, MapGeometry &g) {
 }
 
 hipError_t launch_density_map(hipStream_t stream, const SceneConst &c, const Chain &chain,
-                              const MapGeometry &g, float *out) {
+                              bool grid_current, const MapGeometry &g, float *out) {
     const uint64_t blocks = (g.voxels + kBlock - 1u) / kBlock;
     const ChainBuffers &b = chain.b;
     if (chain.cell_order) {
+        // hz = 0 (no grid): every lookup goes through the key's cell-ordered run
         const GridRuns runs{b.grid, c.grid_base, (uint32_t)c.gx + 2u, (uint32_t)c.gy + 2u,
-                            c.grid_cells ? (uint32_t)c.gz + 2u : 0u, c.table_base, c.mask, b.crun};
+                            c.grid_cells && grid_current ? (uint32_t)c.gz + 2u : 0u, c.table_base, c.mask,
+                            b.crun};
         hipLaunchKernelGGL(k_density_map<GridRuns>, dim3((uint32_t)blocks), dim3(kBlock), 0, stream, c,
                            (const float4 *)b.pos_sorted, runs, g, out);
     } else {

@@ -23,8 +23,9 @@ struct MapGeometry {
 int map_geometry(const sph_params &p, float voxel_size, MapGeometry &g);
 
 // Writes g.voxels floats (x fastest) to device memory `out` on `stream`, from the state
-// and cell table `chain` holds after at least one substep of the scene `c`.
+// and cell table `chain` holds after at least one substep of the scene `c`.  grid_current:
+// the halo grid was filled for c's cell grid (else runs are found through their keys).
 hipError_t launch_density_map(hipStream_t stream, const SceneConst &c, const Chain &chain,
-                              const MapGeometry &g, float *out);
+                              bool grid_current, const MapGeometry &g, float *out);
 
 }  // namespace sph

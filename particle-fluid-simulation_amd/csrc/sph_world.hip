@@ -357,6 +357,7 @@ struct sph_world {
     GraphCache graph;
     bool debug = false;
     bool stepped = false;           // the table and intermediates match the state
+    bool grid_current = true;       // the halo grid was filled with the current cell grid
     int variant = SPH_KERNEL_AUTO;  // requested neighbour-pass kernels
     bool tuned = false;             // AUTO resolved (by the tuning cache or a measurement)
     float *d_map = nullptr;         // density map staging for host copies
@@ -538,6 +539,9 @@ int sph_world_set_params(sph_world *w, const sph_params *params) {
     if (scene_bricks(nc) != w->chain.brick_total || scene_buckets(nc) != w->chain.bucket_total ||
         nc.grid_cells != w->chain.grid_total)
         return fail(SPH_ERR_INVALID_ARG, "the box and smoothing radius fix the cell grid at creation");
+    // equal totals may still lay the grid out differently (e.g. two box extents swapped):
+    // until the next substep refills it, the halo grid does not match the new dimensions
+    if (nc.gx != w->scene.c.gx || nc.gy != w->scene.c.gy || nc.gz != w->scene.c.gz) w->grid_current = false;
     w->params = *params;
     w->params.table_cells = T;
     w->scene.c = nc;
@@ -576,6 +580,7 @@ int sph_world_step(sph_world *w, uint32_t frames) {
         if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
     }
     w->stepped = true;
+    w->grid_current = true;
     return SPH_OK;
 }
 
@@ -707,6 +712,7 @@ int sph_world_pass_times(sph_world *w, uint32_t substeps, sph_pass_times *out) {
     out->force_ms = acc[4] / substeps;
     out->total_ms = out->hash_ms + out->sort_ms + out->reorder_ms + out->density_ms + out->force_ms;
     w->stepped = true;
+    w->grid_current = true;
     return SPH_OK;
 }
 
@@ -742,7 +748,7 @@ int sph_world_density_map_device(sph_world *w, float voxel_size, float *dev_out,
     if (rc) return rc;
     if (count < g.voxels) return fail(SPH_ERR_INVALID_ARG, "density map needs %llu floats, got %zu",
                                       (unsigned long long)g.voxels, count);
-    hipError_t e = launch_density_map(w->stream, w->scene.c, w->chain, g, dev_out);
+    hipError_t e = launch_density_map(w->stream, w->scene.c, w->chain, w->grid_current, g, dev_out);
     return e == hipSuccess ? SPH_OK : hip_fail(e, "density map launch");
 }
 
@@ -763,7 +769,7 @@ int sph_world_density_map(sph_world *w, float voxel_size, float *out, size_t cou
         if (e != hipSuccess) return fail(SPH_ERR_OUT_OF_MEMORY, "density map buffer: %s", hipGetErrorString(e));
         w->map_cap = g.voxels;
     }
-    hipError_t e = launch_density_map(w->stream, w->scene.c, w->chain, g, w->d_map);
+    hipError_t e = launch_density_map(w->stream, w->scene.c, w->chain, w->grid_current, g, w->d_map);
     if (e == hipSuccess)
         e = hipMemcpyAsync(out, w->d_map, g.voxels * sizeof(float), hipMemcpyDeviceToHost, w->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(w->stream);

@@ -103,3 +103,28 @@ def test_batch_density_map_per_scene(gpu, orc):
         got = [b.density_map(i, 0.2) for i in range(len(counts))]
     for i, (q, v) in enumerate(states):
         _bitexact(f"scene {i}", got[i], _oracle_map(orc, params[i], q, v, 1, 0.2))
+
+
+@pytest.mark.parametrize("kernel", ["direct", "brick"])
+def test_world_density_map_after_box_change(gpu, orc, kernel):
+    """set_params may swap two box extents (same cell, brick and bucket totals, different
+    grid dimensions); a map taken before the next substep reads the last substep's runs,
+    laid out for the old dimensions, under the new box."""
+    from sph_hip.workloads import DAM_BREAK_16K as W
+    pa = W.params()
+    pb = gpu.SphParams.from_buffer_copy(pa)
+    pb.max_xyz[0], pb.max_xyz[1] = pa.max_xyz[1], pa.max_xyz[0]  # 10 x 6 x 6 -> 6 x 10 x 6
+    pos, vel = W.state()
+    print(f"density map after a box change kernel={kernel}", flush=True)
+    with gpu.World(pa, W.n) as w:
+        w.set_kernel(kernel)
+        w.upload(pos, vel)
+        w.step(1)
+        w.set_params(pb)
+        got = w.density_map(0.2)
+    for _ in range(pa.substeps - 1):
+        pos, vel, _ = orc.substep(pa, pos, vel)
+    pos, vel, inter = orc.substep(pa, pos, vel)
+    ref = orc.density_map(pb, pos, inter["start"], inter["end"], 0.2)
+    assert np.count_nonzero(ref) > 1000
+    _bitexact("density map", got, ref)
