@@ -70,6 +70,40 @@ def density_map_timing(world, voxel_size: float, reps: int = 5) -> dict:
             "voxels_per_s": voxels / best}
 
 
+def pcie_inclusive(world, frames: int = 20) -> dict:
+    """The drop-in loop with the state crossing PCIe, which the boundary's host buffers
+    imply for a host without device-memory interop: one frame stepped and the positions
+    and velocities downloaded every frame (state uploaded once), and the same with the
+    state also uploaded every frame.  Host buffers are ordinary pageable arrays, as a C
+    host's would be; each download waits for the step."""
+    import numpy as np
+    n = world.n
+    pos = np.empty((n, 4), np.float32)
+    vel = np.empty((n, 4), np.float32)
+    down = lambda: world._lib.sph_world_download_state(world._h, pos.ctypes.data, vel.ctypes.data, None)
+    if down() != 0:
+        raise RuntimeError("download failed")
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        world.step(1)
+        if down() != 0:
+            raise RuntimeError("download failed")
+    t_sd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        world.upload(pos, vel)
+        world.step(1)
+        if down() != 0:
+            raise RuntimeError("download failed")
+    t_usd = time.perf_counter() - t0
+    return {"frames": frames, "bytes_per_frame_each_way": n * 32,
+            "step_download_particle_steps_per_s": n * frames / t_sd,
+            "step_download_ms_per_step": t_sd * 1e3 / frames,
+            "upload_step_download_particle_steps_per_s": n * frames / t_usd,
+            "upload_step_download_ms_per_step": t_usd * 1e3 / frames,
+            "host_buffers": "pageable"}
+
+
 def copy_bandwidth_gbs(mib: int = 512, reps: int = 5) -> float:
     """Device-to-device copy rate (bytes read + written per second), best of `reps`: the
     measured counterpart of the HBM peak."""
@@ -375,6 +409,8 @@ def main() -> int:
     # updateDensityMap of the last substep at a voxel of R/3.5 (16M voxels for uniform_1m,
     # the size of the reference's own 560 x 160 x 160 image)
     dmap = density_map_timing(world, 0.1)
+    # the same step with the state crossing PCIe every frame (never the bench's value)
+    pcie = pcie_inclusive(world)
 
     # Sanity: the state stays finite and in the box.
     p_out, v_out = world.download()
@@ -429,6 +465,7 @@ def main() -> int:
         "pass_ms": pt,
         "pass_ms_by_kernel": variants,
         "density_map": dmap,
+        "pcie_inclusive": pcie,
         "mixed_batch": mixed,
         "mixed_batch_fitting_scenes": mixed_fit,
         "serving": serve,
