@@ -96,12 +96,32 @@ def pcie_inclusive(world, frames: int = 20) -> dict:
         if down() != 0:
             raise RuntimeError("download failed")
     t_usd = time.perf_counter() - t0
+    # the same step + download into pinned host memory (hipHostMalloc of the runtime the
+    # library runs on), which a host can allocate for its readback buffers
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so.7")
+    ptrs = [C.c_void_p(), C.c_void_p()]
+    for p in ptrs:
+        if hip.hipHostMalloc(C.byref(p), C.c_size_t(n * 16), 0) != 0:
+            raise RuntimeError("hipHostMalloc failed")
+    try:
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            world.step(1)
+            if world._lib.sph_world_download_state(world._h, ptrs[0].value, ptrs[1].value, None) != 0:
+                raise RuntimeError("download failed")
+        t_pin = time.perf_counter() - t0
+    finally:
+        for p in ptrs:
+            hip.hipHostFree(p)
     return {"frames": frames, "bytes_per_frame_each_way": n * 32,
             "step_download_particle_steps_per_s": n * frames / t_sd,
             "step_download_ms_per_step": t_sd * 1e3 / frames,
             "upload_step_download_particle_steps_per_s": n * frames / t_usd,
             "upload_step_download_ms_per_step": t_usd * 1e3 / frames,
-            "host_buffers": "pageable"}
+            "host_buffers": "pageable",
+            "step_download_pinned_particle_steps_per_s": n * frames / t_pin,
+            "step_download_pinned_ms_per_step": t_pin * 1e3 / frames}
 
 
 def copy_bandwidth_gbs(mib: int = 512, reps: int = 5) -> float:
