@@ -210,7 +210,7 @@ void k_density(Scenes sc, const ChainBuffers b, uint32_t n) {
 // Force + integrate: applyPressureAndViscosityForce.comp:275-337 followed by
 // updateVelocities.comp:71-119.  Neighbour velocities come from vel_sorted, which no lane
 // writes (the reference races on that buffer; here every read sees the pre-force value).
-// Works in cell order; the new state of cell-ordered slot j goes to hash slot out_slot[j].
+// Works in cell order; the new state of cell-ordered slot j goes to hash slot hslot[j].
 template <class Scenes, int kU, int kB, bool kL>
 __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4))) void k_force(Scenes sc, const ChainBuffers b,
                                                                                         uint32_t n) {
