@@ -72,7 +72,9 @@ typedef struct sph_world sph_world;
 int sph_world_create(const sph_params *params, uint32_t n, sph_world **out);
 
 /* Replaces updateUniforms (main.c:37-82) for the physics fields.  n and table_cells are
- * fixed at creation; a different table_cells returns SPH_ERR_INVALID_ARG. */
+ * fixed at creation; a different table_cells returns SPH_ERR_INVALID_ARG, and so does a box
+ * or smoothing radius that changes the number of cells, bricks or buckets of the cell grid
+ * (one that keeps them, e.g. two box extents swapped, is accepted). */
 int sph_world_set_params(sph_world *w, const sph_params *params);
 
 /* Replaces uploadDataToBufferViaStaging of the position and velocity buffers
