@@ -69,6 +69,28 @@ def test_create_rejects_bad_arguments_without_a_device():
     assert lib.sph_world_step(None, 1) == -1
 
 
+def test_batch_rejects_bad_arguments_without_a_device():
+    import sph_hip
+    lib = sph_hip.load_library()
+    h = C.c_void_p()
+    p = sph_hip.params_for(particle_radius=0.3, smoothing_radius=0.35, g=9.8, box_min=(0, 0, 0),
+                           box_max=(10, 6, 6), frame_dt=1 / 120, substeps=3,
+                           pressure_multiplier=500, target_density=55)
+    params = (sph_hip.SphParams * 2)(p, p)
+    counts = (C.c_uint32 * 2)(100, 0)
+    create = lib.sph_batch_create
+    assert create(C.c_uint32(2), params, counts, C.byref(h)) == -1
+    assert b"no particles" in lib.sph_last_error()
+    assert create(C.c_uint32(0), params, counts, C.byref(h)) == -1
+    assert create(C.c_uint32(1), None, counts, C.byref(h)) == -1
+    q = sph_hip.SphParams.from_buffer_copy(p)
+    q.substeps = 2
+    counts = (C.c_uint32 * 2)(100, 100)
+    assert create(C.c_uint32(2), (sph_hip.SphParams * 2)(p, q), counts, C.byref(h)) == -1
+    assert b"same substep count" in lib.sph_last_error()
+    assert lib.sph_batch_step(None, 1) == -1
+
+
 def test_fill_block_is_seeded_and_in_range():
     import sph_hip
     a = sph_hip.fill_block(5000, 3, (1, 2, 3), (4, 5, 6), 0.1)
