@@ -104,14 +104,26 @@ def pcie_inclusive(world, frames: int = 20) -> dict:
     for p in ptrs:
         if hip.hipHostMalloc(C.byref(p), C.c_size_t(n * 16), 0) != 0:
             raise RuntimeError("hipHostMalloc failed")
+    lib, h = world._lib, world._h
     try:
         t0 = time.perf_counter()
         for _ in range(frames):
             world.step(1)
-            if world._lib.sph_world_download_state(world._h, ptrs[0].value, ptrs[1].value, None) != 0:
+            if lib.sph_world_download_state(h, ptrs[0].value, ptrs[1].value, None) != 0:
                 raise RuntimeError("download failed")
         t_pin = time.perf_counter() - t0
+        # overlapped: frame f's snapshot is copied to the host while frame f + 1 steps
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            world.step(1)
+            if lib.sph_world_snapshot_wait(h) != 0 or \
+                    lib.sph_world_snapshot_async(h, ptrs[0].value, ptrs[1].value) != 0:
+                raise RuntimeError("snapshot failed")
+        if lib.sph_world_snapshot_wait(h) != 0:
+            raise RuntimeError("snapshot failed")
+        t_ovl = time.perf_counter() - t0
     finally:
+        lib.sph_world_snapshot_wait(h)  # no copy may still target the buffers freed here
         for p in ptrs:
             hip.hipHostFree(p)
     return {"frames": frames, "bytes_per_frame_each_way": n * 32,
@@ -121,7 +133,9 @@ def pcie_inclusive(world, frames: int = 20) -> dict:
             "upload_step_download_ms_per_step": t_usd * 1e3 / frames,
             "host_buffers": "pageable",
             "step_download_pinned_particle_steps_per_s": n * frames / t_pin,
-            "step_download_pinned_ms_per_step": t_pin * 1e3 / frames}
+            "step_download_pinned_ms_per_step": t_pin * 1e3 / frames,
+            "step_snapshot_overlapped_pinned_particle_steps_per_s": n * frames / t_ovl,
+            "step_snapshot_overlapped_pinned_ms_per_step": t_ovl * 1e3 / frames}
 
 
 def copy_bandwidth_gbs(mib: int = 512, reps: int = 5) -> float:

@@ -93,6 +93,17 @@ int sph_world_sync(sph_world *w);
  * receives updateVelocities.comp's speed ramp (:120-141) computed from the velocities. */
 int sph_world_download_state(sph_world *w, float *pos_vec4, float *vel_vec4, float *color_vec4);
 
+/* Overlapped readback, for a host that draws from host memory: the reference draws from the
+ * device buffers it steps (commands.c:521-525) and has no counterpart.  Copies the state as
+ * of this call into a device-side snapshot on the world's stream, then into pos_vec4 /
+ * vel_vec4 (either may be NULL) on a second stream, and returns without waiting, so that
+ * sph_world_step can run meanwhile.  The host arrays must stay valid and untouched until
+ * sph_world_snapshot_wait returns; pinned arrays (hipHostMalloc) let the copy overlap the
+ * step.  A snapshot call first waits for the previous one.  Holds 32 B per particle of
+ * device memory from the first call on. */
+int sph_world_snapshot_async(sph_world *w, float *pos_vec4, float *vel_vec4);
+int sph_world_snapshot_wait(sph_world *w);
+
 /* Intermediate buffers of the most recent substep, in the reference's formats, for parity
  * checks.  Stages other than CELL_KEYS, SORTED_*, START, END, PRED_POS and DENSITY need
  * debug capture (sph_world_set_debug) to have been on during that substep. */

@@ -361,6 +361,12 @@ struct sph_world {
     int variant = SPH_KERNEL_AUTO;  // requested neighbour-pass kernels
     bool tuned = false;             // AUTO resolved (by the tuning cache or a measurement)
     float *d_map = nullptr;         // density map staging for host copies
+    // overlapped readback (sph_world_snapshot_async): state snapshot [2n] (positions, then
+    // velocities), the stream that copies it to the host, and the two hand-over events
+    float4 *d_snap = nullptr;
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t snap_taken = nullptr, snap_copied = nullptr;
+    bool snap_pending = false;
     size_t map_cap = 0;             // floats allocated at d_map
 };
 
@@ -608,6 +614,41 @@ int sph_world_download_state(sph_world *w, float *pos4, float *vel4, float *colo
     return SPH_OK;
 }
 
+int sph_world_snapshot_wait(sph_world *w) {
+    if (!w) return fail(SPH_ERR_INVALID_ARG, "world is NULL");
+    if (!w->snap_pending) return SPH_OK;
+    const hipError_t e = hipEventSynchronize(w->snap_copied);
+    w->snap_pending = false;
+    return e == hipSuccess ? SPH_OK : hip_fail(e, "snapshot wait");
+}
+
+int sph_world_snapshot_async(sph_world *w, float *pos4, float *vel4) {
+    if (!w) return fail(SPH_ERR_INVALID_ARG, "world is NULL");
+    int rc = sph_world_snapshot_wait(w);  // the snapshot buffer is free again
+    if (rc) return rc;
+    hipError_t e = hipSuccess;
+    if (!w->copy_stream) {
+        e = hipMalloc(&w->d_snap, (size_t)w->n * 32);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&w->copy_stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&w->snap_taken, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&w->snap_copied, hipEventDisableTiming);
+        if (e != hipSuccess) return hip_fail(e, "snapshot set-up");
+    }
+    const size_t bytes = (size_t)w->n * 16;
+    float4 *snap_vel = w->d_snap + w->n;
+    if (pos4) e = hipMemcpyAsync(w->d_snap, w->chain.b.pos, bytes, hipMemcpyDeviceToDevice, w->stream);
+    if (e == hipSuccess && vel4)
+        e = hipMemcpyAsync(snap_vel, w->chain.b.vel, bytes, hipMemcpyDeviceToDevice, w->stream);
+    if (e == hipSuccess) e = hipEventRecord(w->snap_taken, w->stream);
+    if (e == hipSuccess) e = hipStreamWaitEvent(w->copy_stream, w->snap_taken, 0);
+    if (e == hipSuccess && pos4) e = hipMemcpyAsync(pos4, w->d_snap, bytes, hipMemcpyDeviceToHost, w->copy_stream);
+    if (e == hipSuccess && vel4) e = hipMemcpyAsync(vel4, snap_vel, bytes, hipMemcpyDeviceToHost, w->copy_stream);
+    if (e == hipSuccess) e = hipEventRecord(w->snap_copied, w->copy_stream);
+    if (e != hipSuccess) return hip_fail(e, "snapshot");
+    w->snap_pending = true;
+    return SPH_OK;
+}
+
 int sph_world_set_kernel(sph_world *w, int variant) {
     if (!w) return fail(SPH_ERR_INVALID_ARG, "world is NULL");
     if (variant < SPH_KERNEL_AUTO || variant > SPH_KERNEL_BRICK)
@@ -779,6 +820,13 @@ int sph_world_density_map(sph_world *w, float voxel_size, float *out, size_t cou
 void sph_world_destroy(sph_world *w) {
     if (!w) return;
     if (w->stream) hipStreamSynchronize(w->stream);
+    if (w->copy_stream) {
+        hipStreamSynchronize(w->copy_stream);
+        hipStreamDestroy(w->copy_stream);
+    }
+    if (w->snap_taken) hipEventDestroy(w->snap_taken);
+    if (w->snap_copied) hipEventDestroy(w->snap_copied);
+    if (w->d_snap) hipFree(w->d_snap);
     if (w->d_map) hipFree(w->d_map);
     w->graph.drop();
     w->chain.release();

@@ -88,6 +88,8 @@ def load_library() -> C.CDLL:
         "sph_world_step": (C.c_int, [vp, C.c_uint32]),
         "sph_world_sync": (C.c_int, [vp]),
         "sph_world_download_state": (C.c_int, [vp, vp, vp, vp]),
+        "sph_world_snapshot_async": (C.c_int, [vp, vp, vp]),
+        "sph_world_snapshot_wait": (C.c_int, [vp]),
         "sph_world_set_debug": (C.c_int, [vp, C.c_int]),
         "sph_world_set_kernel": (C.c_int, [vp, C.c_int]),
         "sph_world_autotune": (C.c_int, [vp, C.c_uint32, P(C.c_int), P(C.c_double)]),

@@ -77,6 +77,19 @@ class World:
             self._h, pos.ctypes.data, vel.ctypes.data, col.ctypes.data if color else None))
         return (pos, vel, col) if color else (pos, vel)
 
+    def snapshot_async(self, pos: np.ndarray, vel: np.ndarray) -> None:
+        """Starts copying the current state into pos / vel (n x 4 float32, C order) and
+        returns; the arrays are held until snapshot_wait()."""
+        for a in (pos, vel):
+            if a.dtype != np.float32 or a.shape != (self.n, 4) or not a.flags.c_contiguous:
+                raise ValueError("snapshot arrays must be C-ordered float32 of shape (n, 4)")
+        self._snap = (pos, vel)
+        check(self._lib.sph_world_snapshot_async(self._h, pos.ctypes.data, vel.ctypes.data))
+
+    def snapshot_wait(self) -> None:
+        check(self._lib.sph_world_snapshot_wait(self._h))
+        self._snap = None
+
     def set_kernel(self, variant: str) -> None:
         """Neighbour-pass kernels: "auto", "direct" (one lane per particle) or "brick"."""
         check(self._lib.sph_world_set_kernel(self._h, KERNELS[variant]))
