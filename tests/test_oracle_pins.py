@@ -18,6 +18,26 @@ HASH_KAT = [
 ]
 
 
+def test_hash_fixture_matches_oracle_and_table(orc):
+    """tests/golden/hash_kat.json (written by make_hash_kat.py from the shader's steps in
+    plain Python integers) agrees with the oracle on every cell and with the table above
+    on the cells they share."""
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "hash_kat.json")
+    kat = json.load(open(path))
+    table = {c: h for c, h, *_ in HASH_KAT}
+    assert len(kat) >= len(table)
+    for e in kat:
+        cell = tuple(e["cell"])
+        assert orc.cell_hash(*cell) == e["hash"], cell
+        for bits, key in e["key"].items():
+            assert e["hash"] & ((1 << int(bits)) - 1) == key
+        if cell in table:
+            assert table[cell] == e["hash"], cell
+    assert all(c in {tuple(e["cell"]) for e in kat} for c in table)
+
+
 def _params(orc_mod, **kw):
     import sph_hip
     base = dict(particle_radius=0.3, smoothing_radius=0.35, g=9.8, box_min=(0, 0, 0),
