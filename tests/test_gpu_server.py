@@ -59,3 +59,30 @@ def test_budget_admission_queues_requests(gpu):
         assert st["completed"] == 3
         for rid in ids:
             srv.result(rid)
+
+
+def test_serving_stream_full_size_requests(gpu, orc):
+    """The bench's serving stream at its request sizes (4,096-262,144 particles, dam-break
+    and uniform-fill starts, own box and table per request), submitted together with 2
+    frames each: every result is bit-exact against the oracle run on that request alone."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    reqs = bench.request_stream(8, seed=4, rate_hz=10.0)
+    total = sum(r[2].shape[0] for r in reqs)
+    print(f"serving stream: {len(reqs)} requests, {total} particles", flush=True)
+    assert max(r[2].shape[0] for r in reqs) > 100000
+    with gpu.Server(hbm_budget_bytes=8 << 30, max_scenes=16, max_particles=total,
+                    max_table_cells=sum(r[1].table_cells for r in reqs), substeps=3) as srv:
+        ids = [srv.submit(params, pos, vel, 2) for _, params, pos, vel, _ in reqs]
+        results = []
+        for rid in ids:
+            srv.wait(rid, 120.0)
+            results.append(srv.result(rid))
+        assert srv.stats()["failed"] == 0
+    print("oracle", flush=True)
+    for i, (_, params, pos, vel, _) in enumerate(reqs):
+        rp, rv = orc.step(params, pos, vel, 2)
+        _bitexact(f"request {i} pos", results[i][0], rp)
+        _bitexact(f"request {i} vel", results[i][1], rv)
