@@ -124,6 +124,22 @@ def test_uniform_1m_one_substep_table_2m(gpu, orc):
     assert err["pos"].max() <= REL_TOL, err["pos"].max()
 
 
+def test_uniform_1m_three_frames_bitexact(gpu, orc):
+    """Three frames (9 substeps) of the 1M scene with T = 2^21 through the autotuned path:
+    the trajectory stays bit-exact at full size while the fill starts to settle."""
+    from sph_hip.workloads import UNIFORM_1M as W
+    params = W.params(table_cells=1 << 21)
+    pos, vel = W.state()
+    print(f"case n={W.n} frames=3", flush=True)
+    with gpu.World(params, W.n) as w:
+        w.upload(pos, vel)
+        w.step(3)
+        gp, gv = w.download()
+    rp, rv = orc.step(params, pos, vel, 3)
+    _assert_bitexact("pos", gp, rp)
+    _assert_bitexact("vel", gv, rv)
+
+
 def test_uniform_32m_one_substep(gpu, orc):
     """Full size of the largest scene: 2^25 particles uniform over a 160x40x80 box, T = N.
     Every stage of one substep is bit-exact against the oracle on both kernel paths."""
