@@ -277,6 +277,26 @@ void sph_server_destroy(sph_server *s);
 int sph_scene_fill_block(uint32_t n, uint64_t seed, const float lo[3], const float hi[3],
                          float vel_sigma, float *pos_vec4, float *vel_vec4);
 
+/* The reference's own start-up scene.  sph_params_reference_default fills the compile-time
+ * constants of main.c:11-30 and the box of main.c:356-358: (-140,-40,-40)..(140,40,40),
+ * particle radius 0.3, R 2.5, g 100, frame dt 0.01 in 1 substep, mass 1, pressure
+ * multiplier 15000, target density 1, viscosity 0, near pressure 0, T = N.
+ *
+ * sph_scene_fill_reference replaces initializePositionsBuffer and initializeVelocitiesBuffer
+ * (renderer.c:568-645): n positions uniform in the block centred in x/y/z with 25 / 45 /
+ * 45 % of the box's extents (inset by the particle radius), evaluated with the reference's
+ * float and double rounding, four libc rand() draws per particle (randf(),
+ * renderer.c:506-508), and velocity (0, 0, -100).  `seed` is the srand() argument (0 in
+ * main.c:351).  `rng` names the C library rand() of the reference build:
+ * SPH_RNG_GLIBC (Linux; RAND_MAX 2^31 - 1) or SPH_RNG_MSVCRT (the Windows/MinGW build of
+ * build.bat; RAND_MAX 32767).  The reference's 'R' key (main.c:85-98) refills the buffers
+ * without re-seeding, so the k-th reset's scene is `resets` = k (the start-up scene is 0).
+ * The caller's own rand() state is not touched. */
+typedef enum sph_rng { SPH_RNG_GLIBC = 0, SPH_RNG_MSVCRT = 1 } sph_rng;
+void sph_params_reference_default(sph_params *p);
+int sph_scene_fill_reference(const sph_params *params, uint32_t n, uint32_t seed, int rng,
+                             uint32_t resets, float *pos_vec4, float *vel_vec4);
+
 #ifdef __cplusplus
 }
 #endif

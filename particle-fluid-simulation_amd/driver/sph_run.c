@@ -11,14 +11,13 @@
  * (-140,-40,-40)..(140,40,40), R 2.5, g 100, dt 0.01 with 1 substep, mass 1, pressure
  * multiplier 15000, target density 1, particle radius 0.3, near pressure 0, and the
  * initial block of initializePositionsBuffer (renderer.c:569-608) with velocity
- * (0,0,-100) (renderer.c:642), drawn from libc rand() after srand(0) (main.c:351).
+ * (0,0,-100) (renderer.c:642), drawn from libc rand() after srand(0) (main.c:351), both
+ * from the library (sph_params_reference_default, sph_scene_fill_reference).
  */
 #include <stdio.h>
 #include <stdlib.h>
 
 #include "sph_world.h"
-
-static float randf(void) { return (float)rand() / (RAND_MAX + 1.0f); }
 
 static int check(int rc, const char *what) {
     if (rc != SPH_OK) fprintf(stderr, "%s failed (%d): %s\n", what, rc, sph_last_error());
@@ -28,38 +27,14 @@ static int check(int rc, const char *what) {
 int main(int argc, char **argv) {
     const unsigned frames = argc > 1 ? (unsigned)atoi(argv[1]) : 100u;
     const unsigned n = argc > 2 ? (unsigned)atoi(argv[2]) : 300000u;
-    sph_params p = {0};
-    const float s = 40.0f;
-    p.min_xyz[0] = -3.5f * s; p.min_xyz[1] = -1.0f * s; p.min_xyz[2] = -1.0f * s;
-    p.max_xyz[0] = 3.5f * s;  p.max_xyz[1] = 1.0f * s;  p.max_xyz[2] = 1.0f * s;
-    p.particle_radius = 0.3f;
-    p.smoothing_radius = 2.5f;
-    p.g = 100.0f;
-    p.particle_mass = 1.0f;
-    p.pressure_multiplier = 15000.0f;
-    p.target_density = 1.0f;
-    p.viscosity_strength = 0.0f;
-    p.near_pressure_multiplier = 0.0f;
-    p.table_cells = 0; /* T = N, as the reference sizes its tables */
-    sph_params_derive(&p, 0.01, 1, 2.5);
+    sph_params p;
+    sph_params_reference_default(&p); /* main.c:11-30,356-358; T = N */
 
     float *pos = malloc((size_t)n * 16), *vel = malloc((size_t)n * 16);
     if (!pos || !vel) return 1;
-    srand(0);
-    const float r = p.particle_radius;
-    const float scale[3] = {0.25f, 0.45f, 0.45f};
-    for (unsigned i = 0; i < n; ++i) {
-        for (int a = 0; a < 3; ++a) {
-            /* randf() * (scale * extent - 2 * r) + r + min + (1 - scale) * 0.5 * extent:
-             * float up to "+ min", then double because 0.5 is a double literal. */
-            const float extent = p.max_xyz[a] - p.min_xyz[a];
-            const float head = randf() * (scale[a] * extent - 2 * r) + r + p.min_xyz[a];
-            pos[4 * i + a] = (float)((double)head + (1 - scale[a]) * 0.5 * (double)extent);
-        }
-        pos[4 * i + 3] = 0.0f;
-        (void)randf(); /* the reference draws one more number per particle (renderer.c:587) */
-        vel[4 * i + 0] = 0.0f; vel[4 * i + 1] = 0.0f; vel[4 * i + 2] = -100.0f; vel[4 * i + 3] = 0.0f;
-    }
+    /* initializePositionsBuffer / initializeVelocitiesBuffer after srand(0) */
+    if (check(sph_scene_fill_reference(&p, n, 0, SPH_RNG_GLIBC, 0, pos, vel), "sph_scene_fill_reference"))
+        return 1;
 
     sph_world *w = NULL;
     if (check(sph_world_create(&p, n, &w), "sph_world_create")) return 1;

@@ -5,7 +5,7 @@ loudly when the HIP library is missing.
 """
 from .abi import (  # noqa: F401
     SphError, SphParams, PassTimes, load_library, lib_path, DUMP_STAGES, KERNELS,
-    params_for, fill_block, density_map_dims,
+    params_for, fill_block, density_map_dims, fill_reference, reference_default_params, RNGS,
 )
 from .world import World  # noqa: F401
 from .batch import Batch  # noqa: F401

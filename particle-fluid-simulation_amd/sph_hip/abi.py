@@ -129,6 +129,9 @@ def load_library() -> C.CDLL:
         "sph_server_destroy": (None, [vp]),
         "sph_scene_fill_block": (C.c_int, [C.c_uint32, C.c_uint64, P(C.c_float * 3),
                                            P(C.c_float * 3), C.c_float, vp, vp]),
+        "sph_params_reference_default": (None, [P(SphParams)]),
+        "sph_scene_fill_reference": (C.c_int, [P(SphParams), C.c_uint32, C.c_uint32, C.c_int,
+                                               C.c_uint32, vp, vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -171,6 +174,27 @@ def params_for(*, particle_radius: float, smoothing_radius: float, g: float, box
     p.table_cells = table_cells
     load_library().sph_params_derive(C.byref(p), frame_dt, substeps, smoothing_radius)
     return p
+
+
+RNGS = {"glibc": 0, "msvcrt": 1}
+
+
+def reference_default_params(table_cells: int = 0) -> SphParams:
+    """The reference's start-up constants (main.c:11-30, box main.c:356-358)."""
+    p = SphParams()
+    load_library().sph_params_reference_default(C.byref(p))
+    p.table_cells = table_cells
+    return p
+
+
+def fill_reference(params: SphParams, n: int, seed: int = 0, rng: str = "glibc", resets: int = 0):
+    """initializePositionsBuffer / initializeVelocitiesBuffer (renderer.c:568-645) after
+    srand(seed) with the named C library's rand(); returns (pos, vel) float32 [n, 4]."""
+    pos = np.zeros((n, 4), np.float32)
+    vel = np.zeros((n, 4), np.float32)
+    check(load_library().sph_scene_fill_reference(C.byref(params), n, seed, RNGS[rng], resets,
+                                                  pos.ctypes.data, vel.ctypes.data))
+    return pos, vel
 
 
 def fill_block(n: int, seed: int, lo, hi, vel_sigma: float = 0.0):

@@ -132,3 +132,67 @@ def test_lane_configs_without_a_device():
     lib = sph_hip.load_library()
     assert lib.sph_lane_configs() == 10
     assert lib.sph_world_set_lane_config(None, 0) == -1
+
+
+def _reference_block(params, draws, n):
+    """renderer.c:573-582 from a list of rand() outputs (4 per particle), in the
+    reference's rounding: float arithmetic up to `+ min`, then a double sum (0.5 is a double
+    literal) rounded to float on assignment."""
+    f32 = np.float32
+    scale = (f32(0.25), f32(0.45), f32(0.45))
+    r = f32(params.particle_radius)
+    out = np.zeros((n, 4), np.float32)
+    for i in range(n):
+        for a in range(3):
+            rf = f32(f32(draws[4 * i + a]) / f32(2147483648.0))
+            ext = f32(f32(params.max_xyz[a]) - f32(params.min_xyz[a]))
+            head = f32(f32(f32(rf * f32(f32(scale[a] * ext) - f32(f32(2.0) * r))) + r) + f32(params.min_xyz[a]))
+            out[i, a] = f32(float(head) + float(f32(f32(1.0) - scale[a])) * 0.5 * float(ext))
+    return out
+
+
+def test_reference_fill_matches_libc_rand():
+    """sph_scene_fill_reference with the glibc generator draws exactly what this system's
+    libc rand() draws after srand(seed), four numbers per particle, and a reset continues the
+    same stream; the velocities are (0, 0, -100)."""
+    import sph_hip
+    libc = C.CDLL("libc.so.6")
+    p = sph_hip.reference_default_params()
+    assert tuple(p.min_xyz) == (-140.0, -40.0, -40.0) and tuple(p.max_xyz) == (140.0, 40.0, 40.0)
+    assert (p.smoothing_radius, p.g, p.pressure_multiplier, p.target_density) == (2.5, 100.0, 15000.0, 1.0)
+    assert p.dt == np.float32(0.01) and p.substeps == 1
+    n = 500
+    for seed in (0, 7, 3000000000):
+        libc.srand(C.c_uint(seed))
+        draws = [libc.rand() for _ in range(8 * n)]
+        pos, vel = sph_hip.fill_reference(p, n, seed=seed)
+        assert np.array_equal(pos.view(np.uint32), _reference_block(p, draws, n).view(np.uint32)), seed
+        assert (vel[:, :3] == (0.0, 0.0, -100.0)).all() and (vel[:, 3] == 0).all() and (pos[:, 3] == 0).all()
+        pos1, _ = sph_hip.fill_reference(p, n, seed=seed, resets=1)
+        assert np.array_equal(pos1.view(np.uint32), _reference_block(p, draws[4 * n:], n).view(np.uint32))
+    # the block: 25 / 45 / 45 % of the box, centred, inset by the particle radius
+    pos, _ = sph_hip.fill_reference(p, 300000)
+    lo = np.array([-35.0 + 0.3, -18.0 + 0.3, -18.0 + 0.3])
+    hi = np.array([35.0 - 0.3, 18.0 - 0.3, 18.0 - 0.3])
+    assert (pos[:, :3] >= lo - 1e-4).all() and (pos[:, :3] <= hi + 1e-4).all()
+    assert (pos[:, :3].min(0) < lo + 0.1).all() and (pos[:, :3].max(0) > hi - 0.1).all()
+
+
+def test_reference_fill_msvcrt_generator():
+    """The MSVCRT generator (the reference's Windows build): its well-known first outputs
+    after srand(1), 41 18467 6334 26500 19169, give the first particle's coordinates."""
+    import sph_hip
+    p = sph_hip.reference_default_params()
+    pos, _ = sph_hip.fill_reference(p, 2, seed=1, rng="msvcrt")
+    f32 = np.float32
+    draws = [41, 18467, 6334, 26500, 19169]
+    scale = (f32(0.25), f32(0.45), f32(0.45))
+    for a in range(3):
+        rf = f32(f32(draws[a]) / f32(32768.0))
+        ext = f32(f32(p.max_xyz[a]) - f32(p.min_xyz[a]))
+        head = f32(f32(f32(rf * f32(f32(scale[a] * ext) - f32(0.6))) + f32(0.3)) + f32(p.min_xyz[a]))
+        assert pos[0, a] == f32(float(head) + float(f32(1.0) - scale[a]) * 0.5 * float(ext))
+    # the second particle starts at the fifth draw (the fourth is the unused shift draw)
+    rf = f32(f32(draws[4]) / f32(32768.0))
+    assert pos[1, 0] == f32(float(f32(f32(f32(rf * f32(f32(f32(0.25) * f32(280.0)) - f32(0.6))) + f32(0.3)) +
+                                  f32(-140.0))) + float(f32(0.75)) * 0.5 * 280.0)

@@ -1,0 +1,94 @@
+"""The reference's own start-up scene on the GPU against the FP32 oracle: the constants of
+main.c:11-30 (N = 300,000, R = 2.5, g = 100, k = 15000, rho0 = 1, dt = 0.01 in one
+substep) in the box (-140,-40,-40)..(140,40,40) of main.c:356-358 (a negative, off-origin
+minimum), T = N (not a power of two, so the key mask N - 1 has holes), the block and
+velocity of initializePositionsBuffer / initializeVelocitiesBuffer after srand(0)
+(renderer.c:568-645, main.c:351).  ~54 particles per cell; runs of up to ~380 slots."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from test_gpu_parity import STAGES_F32, STAGES_INT, _assert_bitexact, _run_one_substep
+
+pytestmark = pytest.mark.gpu
+
+N = 300000
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "particle-fluid-simulation_amd", "bin", "sph_run")
+
+
+def _scene(gpu, n=N):
+    p = gpu.reference_default_params()
+    pos, vel = gpu.fill_reference(p, n)
+    return p, pos, vel
+
+
+@pytest.mark.parametrize("kernel", ["direct", "brick"])
+def test_reference_scene_one_substep_every_stage(gpu, orc, kernel):
+    p, pos, vel = _scene(gpu)
+    assert p.min_xyz[0] == -140.0 and p.table_cells == 0
+    print(f"reference scene n={N} kernel={kernel}", flush=True)
+    rpos, rvel, ref = orc.substep(p, pos, vel)
+    got = _run_one_substep(gpu, p, pos, vel, kernel)
+    for s in STAGES_INT + STAGES_F32:
+        _assert_bitexact(f"{kernel}:{s}", got[s], ref[s])
+    _assert_bitexact("pos", got["pos"], rpos)
+    _assert_bitexact("vel", got["vel"], rvel)
+    # keys with holes: the mask N - 1 = 299999 leaves bits unused
+    assert int(ref["sorted_keys"].max()) <= N - 1 and np.count_nonzero(ref["start"] >= 0) < N // 50
+
+
+def test_reference_scene_ten_frames_and_density_map(gpu, orc):
+    """Ten frames through the autotuned path, bit-exact, then the density map of the last
+    substep at the reference's DENSITY_CELL_SIZE 0.5 (main.c:14): the 560 x 160 x 160 image
+    of initializeDensityMap (renderer.c:774-804), bit-exact."""
+    p, pos, vel = _scene(gpu)
+    print(f"reference scene n={N} frames=10 + density map", flush=True)
+    with gpu.World(p, N) as w:
+        w.upload(pos, vel)
+        w.step(10)
+        gp, gv = w.download()
+        gmap = w.density_map(0.5)
+    rp9, rv9 = orc.step(p, pos, vel, 9)
+    rp, rv, inter = orc.substep(p, rp9, rv9)
+    _assert_bitexact("pos", gp, rp)
+    _assert_bitexact("vel", gv, rv)
+    assert gmap.shape == (160, 160, 560)
+    rmap = orc.density_map(p, rp, inter["start"], inter["end"], 0.5)
+    assert np.count_nonzero(rmap) > 100000
+    _assert_bitexact("density map", gmap, rmap)
+
+
+def test_reference_scene_msvcrt_start(gpu, orc):
+    """The same scene drawn with the Windows build's rand() (MSVCRT), one frame bit-exact."""
+    p = gpu.reference_default_params()
+    n = 65536
+    pos, vel = gpu.fill_reference(p, n, rng="msvcrt")
+    with gpu.World(p, n) as w:
+        w.upload(pos, vel)
+        w.step(2)
+        gp, gv = w.download()
+    rp, rv = orc.step(p, pos, vel, 2)
+    _assert_bitexact("pos", gp, rp)
+    _assert_bitexact("vel", gv, rv)
+
+
+def test_driver_matches_oracle(gpu, orc):
+    """The headless C driver (driver/sph_run.c) builds the scene through the library's
+    initialiser, steps `frames` frames and 10 timed substeps, and prints particle 0; the
+    oracle run of the same frames + 10 substeps prints the same line, digit for digit."""
+    n, frames = 20000, 3
+    print(f"sph_run {frames} {n}", flush=True)
+    out = subprocess.run([EXE, str(frames), str(n)], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stderr
+    assert f"particles {n}, frames {frames}" in out.stdout
+    assert re.search(r"total\s+[0-9.]+ ms per substep", out.stdout)
+    line = [s for s in out.stdout.splitlines() if s.startswith("particle 0:")][0]
+    p, pos, vel = _scene(gpu, n)
+    q, v = orc.step(p, pos, vel, frames + 10)
+    want = ("particle 0: pos (%.4f, %.4f, %.4f) vel (%.4f, %.4f, %.4f)"
+            % (q[0, 0], q[0, 1], q[0, 2], v[0, 0], v[0, 1], v[0, 2]))
+    assert line == want
