@@ -446,10 +446,14 @@ def main() -> int:
     # the same step with the state crossing PCIe every frame (never the bench's value)
     pcie = pcie_inclusive(world)
 
-    # Sanity: the state stays finite and in the box.
+    # Sanity: the state stays finite and in the box (the integrate clamp keeps every
+    # position at least particle_radius + 0.5 inside each wall).
     p_out, v_out = world.download()
     import numpy as np
     ok = bool(np.isfinite(p_out).all() and np.isfinite(v_out).all())
+    lo = np.array(params.min_xyz, np.float32)
+    hi = np.array(params.max_xyz, np.float32)
+    in_box = bool(ok and (p_out[:, :3] >= lo).all() and (p_out[:, :3] <= hi).all())
     world.close()
 
     mixed = mixed_fit = None
@@ -504,6 +508,7 @@ def main() -> int:
         "mixed_batch_fitting_scenes": mixed_fit,
         "serving": serve,
         "state_finite": ok,
+        "state_in_box": in_box,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -236,8 +236,9 @@ __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4))) void k_
     force_finish(c, P, V, myd.x, Fp, Fv, po, vo, vf);
     const uint32_t o = b.hslot[j];
     if (b.dbg_vel_force) {
-        // V.w holds the density when lists are recorded; the velocity's own w is the state's
-        if (kL) vf.w = b.vel[o].w + 1.0f;
+        // V.w holds the density when lists are recorded; the placement kept the input
+        // velocity's own w in the capture buffer
+        if (kL) vf.w = b.dbg_vel_force[j].w + 1.0f;
         b.dbg_vel_force[j] = vf;
     }
     b.pos[o] = po;

@@ -127,7 +127,8 @@ __global__ __launch_bounds__(kBlock) void k_place(
     const float4 *__restrict__ pos, const float4 *__restrict__ vel, const int2 *__restrict__ table,
     const uint32_t *__restrict__ bstart, const uint2 *__restrict__ run_info,
     float4 *__restrict__ pos_sorted, float4 *__restrict__ vel_sorted, float4 *__restrict__ pred_pos,
-    float4 *__restrict__ dbg_pred_vel, uint32_t *__restrict__ hslot, int2 *__restrict__ crun) {
+    float4 *__restrict__ dbg_pred_vel, float4 *__restrict__ dbg_vel_force, uint32_t *__restrict__ hslot,
+    int2 *__restrict__ crun) {
     const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
     if (j >= n) return;
     const uint32_t key = skeys[j];
@@ -146,6 +147,9 @@ __global__ __launch_bounds__(kBlock) void k_place(
     vel_sorted[j2] = V;
     pred_pos[j2] = make_float4(p.x, p.y, p.z, 1.0f);
     if (dbg_pred_vel) dbg_pred_vel[j2] = make_float4(v.x, v.y, v.z, 1.0f);
+    // the input velocity's w for the debug capture of the force pass's velocities[j] + (.., 1):
+    // the list-recording density pass reuses vel_sorted[].w for the density
+    if (dbg_vel_force) dbg_vel_force[j2] = make_float4(0.0f, 0.0f, 0.0f, V.w);
     hslot[j2] = j;
 }
 
@@ -181,7 +185,7 @@ hipError_t launch_place(hipStream_t stream, const Scenes &sc, const ChainBuffers
                         uint32_t grid_total) {
     hipLaunchKernelGGL(k_place<Scenes>, dim3(blocks_for(n)), dim3(kBlock), 0, stream, sc, sorted_keys,
                        sorted_idx, n, b.pos, b.vel, b.table, b.bcount, b.run_info, b.pos_sorted,
-                       b.vel_sorted, b.pred_pos, b.dbg_pred_vel, b.hslot, b.crun);
+                       b.vel_sorted, b.pred_pos, b.dbg_pred_vel, b.dbg_vel_force, b.hslot, b.crun);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || grid_total == 0) return e;
     hipLaunchKernelGGL(k_grid<Scenes>, dim3(blocks_for(grid_total)), dim3(kBlock), 0, stream, sc,

@@ -92,6 +92,14 @@ struct CellRuns {
     }
 };
 
+// Whether cells cx - 1 .. cx + 1 of a row all lie in a halo-grid row of hx >= 2 entries
+// (entry l holds cell l - 1), i.e. 0 <= cx and cx + 2 < hx, as one unsigned compare.  The
+// form lx + 2 < hx with lx = (uint32)cx wraps for cx = -1 and -2 (the cell of a point
+// outside the box) and would admit an index 2^32 - 1 past the row.
+__host__ __device__ constexpr bool row3_in_grid(int32_t cx, uint32_t hx) {
+    return (uint32_t)cx < hx - 2u;
+}
+
 // Finds cell runs in the cell-ordered layout: through the scene's halo grid when the cell
 // lies in it (hz = 0: the scene has none), else through the key's cell-ordered run.
 struct GridRuns {
@@ -109,12 +117,10 @@ struct GridRuns {
         if (lx < hx && ly < hy && lz < hz) return grid[grid_base + lx + hx * (ly + hy * lz)];
         return slow_raw(cx, cy, cz);
     }
-    // The three cells (cx - 1 .. cx + 1, cy, cz) of a row: consecutive grid entries.  The x
-    // test is lx < hx - 2 (hx >= 2), not lx + 2 < hx, which wraps for cx = -1 and -2 (the
-    // cell of a point outside the box) and would admit an index 2^32 - 1 past the row.
+    // The three cells (cx - 1 .. cx + 1, cy, cz) of a row: consecutive grid entries.
     __device__ __forceinline__ void row3(int cx, int cy, int cz, int2 out[3]) const {
         const uint32_t lx = (uint32_t)cx, ly = (uint32_t)(cy + 1), lz = (uint32_t)(cz + 1);
-        if (lx < hx - 2u && ly < hy && lz < hz) {
+        if (row3_in_grid(cx, hx) && ly < hy && lz < hz) {
             const int2 *g = grid + grid_base + lx + hx * (ly + hy * lz);
             out[0] = g[0];
             out[1] = g[1];

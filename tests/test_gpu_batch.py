@@ -100,3 +100,45 @@ def test_full_mixed_batch_one_frame(gpu, orc):
         rp, rv = orc.step(params[i], p, v, 1)
         _bitexact(f"scene {i} pos", got[i][0], rp)
         _bitexact(f"scene {i} vel", got[i][1], rv)
+
+
+def shifted(gpu, params, pos, offset):
+    """The same scene translated by `offset`: a box whose minimum is non-zero (negative on
+    some axes, not a multiple of R), positions moved with it (rounded to float32 and kept
+    inside the new box).  Cell coordinates floor((p - min) / R) round differently here than
+    at the origin, which exercises the sample-point bounds skip and the bucket of each run."""
+    p2 = gpu.SphParams.from_buffer_copy(params)
+    for a in range(3):
+        p2.min_xyz[a] = float(np.float32(offset[a]))
+        p2.max_xyz[a] = float(np.float32(np.float32(params.max_xyz[a]) + np.float32(offset[a])))
+    q = pos.copy()
+    lo = np.array(p2.min_xyz, np.float32)
+    hi = np.array(p2.max_xyz, np.float32)
+    q[:, :3] = np.clip((pos[:, :3] + np.asarray(offset, np.float32)).astype(np.float32), lo, hi)
+    return p2, q
+
+
+OFFSETS = [(-7.31, 2.77, -0.123), (13.5, -41.05, 0.6), (-0.35, -0.7, -3.3), (1.0e3, 5.25, -2.0e2)]
+
+
+@pytest.mark.parametrize("kernel", ["direct", "brick"])
+def test_batch_scenes_with_offset_boxes(gpu, orc, kernel):
+    """Scenes whose boxes start away from the origin (each its own asymmetric offset), two
+    frames in one batch: every scene bit-exact against the oracle run on it alone."""
+    specs = _specs()[:len(OFFSETS)]
+    params, states = [], []
+    for sp, off in zip(specs, OFFSETS):
+        pos, vel = sp.state()
+        p2, q = shifted(gpu, sp.params(), pos, off)
+        params.append(p2)
+        states.append((q, vel))
+    with gpu.Batch(params, [s.n for s in specs]) as b:
+        b.set_kernel(kernel)
+        for i, (p, v) in enumerate(states):
+            b.upload(i, p, v)
+        b.step(2)
+        got = [b.download(i) for i in range(len(specs))]
+    for i, (p, v) in enumerate(states):
+        rp, rv = orc.step(params[i], p, v, 2)
+        _bitexact(f"scene {i} pos", got[i][0], rp)
+        _bitexact(f"scene {i} vel", got[i][1], rv)

@@ -176,6 +176,35 @@ def test_dam_break_multi_frame_bitexact(gpu, orc, kernel):
     _assert_bitexact("vel", gv, rv)
 
 
+def test_download_colour_matches_oracle(gpu, orc):
+    """The colour download_state returns (updateVelocities.comp:120-141's speed ramp) equals
+    the oracle's ramp of the oracle's velocities, bit for bit, after 5 frames of the dam
+    break (speeds across the ramp's first breaks), for a world and for a batch scene."""
+    from sph_hip.workloads import DAM_BREAK_16K as W
+    params = W.params()
+    pos, vel = W.state()
+    with gpu.World(params, W.n) as w:
+        w.upload(pos, vel)
+        w.step(5)
+        gp, gv, gc = w.download(color=True)
+    rp, rv = orc.step(params, pos, vel, 5)
+    _assert_bitexact("vel", gv, rv)
+    ref_c = orc.color(rv)
+    assert np.unique(ref_c[:, 0]).size > 100
+    _assert_bitexact("colour", gc, ref_c)
+    import ctypes as C
+    with gpu.Batch([params, params], [W.n, 4096]) as b:
+        b.upload(0, pos, vel)
+        b.upload(1, pos[:4096], vel[:4096])
+        b.step(5)
+        bp = np.empty((W.n, 4), np.float32)
+        bv = np.empty((W.n, 4), np.float32)
+        bc = np.empty((W.n, 4), np.float32)
+        gpu.abi.check(b._lib.sph_batch_download_scene(b._h, 0, bp.ctypes.data, bv.ctypes.data,
+                                                      bc.ctypes.data))
+    _assert_bitexact("batch colour", bc, ref_c)
+
+
 @pytest.mark.parametrize("config", range(10))
 def test_dam_break_every_lane_config(gpu, orc, config):
     """Every per-lane launch configuration (scan unroll x workgroup size, with and without
@@ -271,6 +300,36 @@ def test_walls_and_coincident(gpu, orc):
     pos[192:256] = pos[256:320]
     vel[192:256] = vel[256:320]
     _check_substep(gpu, orc, params, pos, vel)
+
+
+@pytest.mark.parametrize("config", [3, 8, "brick"])
+def test_w_lanes_vary_per_particle(gpu, orc, config):
+    """The uploaded w of positions and velocities differs per particle: every stage of one
+    substep, the force pass's velocities[j] + (F/rho dt, 1) included, keeps each particle's
+    own w, on a list configuration, a cell-walk configuration and the brick kernels."""
+    from sph_hip.workloads import DAM_BREAK_16K as W
+    params = W.params()
+    params.substeps = 1
+    pos, vel = W.state(6000)
+    rng = np.random.default_rng(11)
+    pos[:, 3] = rng.uniform(-5, 5, pos.shape[0]).astype(np.float32)
+    vel[:, 3] = rng.uniform(-5, 5, vel.shape[0]).astype(np.float32)
+    with gpu.World(params, pos.shape[0]) as w:
+        if config == "brick":
+            w.set_kernel("brick")
+        else:
+            w.set_lane_config(config)
+        w.set_debug(True)
+        w.upload(pos, vel)
+        w.step(1)
+        got = {s: w.dump(s) for s in STAGES_INT + STAGES_F32}
+        got["pos"], got["vel"] = w.download()
+    rpos, rvel, ref = orc.substep(params, pos, vel)
+    assert np.array_equal(ref["vel_force"][:, 3], vel[ref["sorted_index"], 3] + np.float32(1))
+    for s in STAGES_INT + STAGES_F32:
+        _assert_bitexact(f"{config}:{s}", got[s], ref[s])
+    _assert_bitexact("pos", got["pos"], rpos)
+    _assert_bitexact("vel", got["vel"], rvel)
 
 
 def test_rejects_bad_state(gpu):

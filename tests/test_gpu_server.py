@@ -86,3 +86,27 @@ def test_serving_stream_full_size_requests(gpu, orc):
         rp, rv = orc.step(params, pos, vel, 2)
         _bitexact(f"request {i} pos", results[i][0], rp)
         _bitexact(f"request {i} vel", results[i][1], rv)
+
+
+def test_served_scenes_with_offset_boxes(gpu, orc):
+    """Requests whose boxes start away from the origin (asymmetric, non-integer offsets),
+    served together: each result bit-exact against the oracle run on that request alone."""
+    from sph_hip.workloads import mixed_scenes
+    from test_gpu_batch import OFFSETS, shifted
+    specs = mixed_scenes(count=len(OFFSETS), seed=60, n_min=500, n_max=5000, extent=(2.0, 9.0))
+    reqs = []
+    for sp, off in zip(specs, OFFSETS):
+        pos, vel = sp.state()
+        p2, q = shifted(gpu, sp.params(), pos, off)
+        reqs.append((p2, q, vel))
+    with gpu.Server(hbm_budget_bytes=1 << 30, max_scenes=8, max_particles=1 << 16,
+                    max_table_cells=1 << 17, substeps=3) as srv:
+        ids = [srv.submit(p, q, v, 2 + i) for i, (p, q, v) in enumerate(reqs)]
+        results = []
+        for rid in ids:
+            srv.wait(rid, 120.0)
+            results.append(srv.result(rid))
+    for i, (p, q, v) in enumerate(reqs):
+        rp, rv = orc.step(p, q, v, 2 + i)
+        _bitexact(f"request {i} pos", results[i][0], rp)
+        _bitexact(f"request {i} vel", results[i][1], rv)
