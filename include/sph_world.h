@@ -104,6 +104,20 @@ int sph_world_download_state(sph_world *w, float *pos_vec4, float *vel_vec4, flo
 int sph_world_snapshot_async(sph_world *w, float *pos_vec4, float *vel_vec4);
 int sph_world_snapshot_wait(sph_world *w);
 
+/* Device-resident state for a renderer that draws from device memory, as the reference
+ * draws its particles straight from the buffers it steps (commands.c:521-525, shader.vert),
+ * e.g. Vulkan buffers imported into HIP with hipImportExternalMemory.
+ *   sph_world_bind_state: the world steps the caller's device buffers (n vec4 each,
+ *     16-byte aligned, device memory) in place instead of its own; upload / download /
+ *     snapshot then read and write them too.  The caller keeps ownership and must keep
+ *     them alive while bound.  Both NULL returns to the world's own buffers (whose contents
+ *     are then stale: upload a state).  Binding waits for the world's queued work.
+ *   sph_world_device_state: the device pointers the world currently steps (its own or the
+ *     bound ones); their contents are the state after sph_world_sync. */
+int sph_world_bind_state(sph_world *w, float *pos_vec4_dev, float *vel_vec4_dev);
+int sph_world_device_state(const sph_world *w, float **pos_vec4_dev, float **vel_vec4_dev);
+
+
 /* Intermediate buffers of the most recent substep, in the reference's formats, for parity
  * checks.  Stages other than CELL_KEYS, SORTED_*, START, END, PRED_POS and DENSITY need
  * debug capture (sph_world_set_debug) to have been on during that substep. */

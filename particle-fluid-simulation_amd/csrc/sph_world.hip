@@ -368,6 +368,8 @@ struct sph_world {
     hipEvent_t snap_taken = nullptr, snap_copied = nullptr;
     bool snap_pending = false;
     size_t map_cap = 0;             // floats allocated at d_map
+    // the world's own state buffers while the caller's are bound (sph_world_bind_state)
+    float4 *own_pos = nullptr, *own_vel = nullptr;
 };
 
 namespace {
@@ -817,9 +819,56 @@ int sph_world_density_map(sph_world *w, float voxel_size, float *out, size_t cou
     return e == hipSuccess ? SPH_OK : hip_fail(e, "density map");
 }
 
+int sph_world_bind_state(sph_world *w, float *pos4_dev, float *vel4_dev) {
+    if (!w) return fail(SPH_ERR_INVALID_ARG, "world is NULL");
+    if ((pos4_dev == nullptr) != (vel4_dev == nullptr))
+        return fail(SPH_ERR_INVALID_ARG, "bind both state buffers or neither");
+    if (pos4_dev && pos4_dev == vel4_dev) return fail(SPH_ERR_INVALID_ARG, "positions and velocities alias");
+    const uintptr_t mis = (uintptr_t)pos4_dev | (uintptr_t)vel4_dev;
+    if (mis & 15u) return fail(SPH_ERR_INVALID_ARG, "state buffers must be 16-byte aligned");
+    int rc = sph_world_snapshot_wait(w);
+    if (rc) return rc;
+    hipError_t e = hipStreamSynchronize(w->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    if (pos4_dev) {
+        hipPointerAttribute_t a{}, b{};
+        e = hipPointerGetAttributes(&a, pos4_dev);
+        if (e == hipSuccess) e = hipPointerGetAttributes(&b, vel4_dev);
+        if (e != hipSuccess || a.type == hipMemoryTypeHost || b.type == hipMemoryTypeHost ||
+            a.type == hipMemoryTypeUnregistered || b.type == hipMemoryTypeUnregistered) {
+            (void)hipGetLastError();
+            return fail(SPH_ERR_INVALID_ARG, "state buffers must be device memory");
+        }
+        if (!w->own_pos) {
+            w->own_pos = w->chain.b.pos;
+            w->own_vel = w->chain.b.vel;
+        }
+        w->chain.b.pos = reinterpret_cast<float4 *>(pos4_dev);
+        w->chain.b.vel = reinterpret_cast<float4 *>(vel4_dev);
+    } else if (w->own_pos) {
+        w->chain.b.pos = w->own_pos;
+        w->chain.b.vel = w->own_vel;
+        w->own_pos = w->own_vel = nullptr;
+    }
+    w->graph.drop();  // the captured launches name the previous buffers
+    w->stepped = false;
+    return SPH_OK;
+}
+
+int sph_world_device_state(const sph_world *w, float **pos4_dev, float **vel4_dev) {
+    if (!w) return fail(SPH_ERR_INVALID_ARG, "world is NULL");
+    if (pos4_dev) *pos4_dev = reinterpret_cast<float *>(w->chain.b.pos);
+    if (vel4_dev) *vel4_dev = reinterpret_cast<float *>(w->chain.b.vel);
+    return SPH_OK;
+}
+
 void sph_world_destroy(sph_world *w) {
     if (!w) return;
     if (w->stream) hipStreamSynchronize(w->stream);
+    if (w->own_pos) {  // never free the caller's buffers
+        w->chain.b.pos = w->own_pos;
+        w->chain.b.vel = w->own_vel;
+    }
     if (w->copy_stream) {
         hipStreamSynchronize(w->copy_stream);
         hipStreamDestroy(w->copy_stream);

@@ -86,6 +86,17 @@ class World:
         self._snap = (pos, vel)
         check(self._lib.sph_world_snapshot_async(self._h, pos.ctypes.data, vel.ctypes.data))
 
+    def bind_state(self, pos_dev: int | None, vel_dev: int | None) -> None:
+        """Steps the caller's device buffers (n vec4 each) in place; None, None unbinds."""
+        check(self._lib.sph_world_bind_state(self._h, C.c_void_p(pos_dev) if pos_dev else None,
+                                             C.c_void_p(vel_dev) if vel_dev else None))
+
+    def device_state(self) -> tuple:
+        """Device addresses (positions, velocities) of the buffers the world steps."""
+        p, v = C.c_void_p(), C.c_void_p()
+        check(self._lib.sph_world_device_state(self._h, C.byref(p), C.byref(v)))
+        return p.value, v.value
+
     def snapshot_wait(self) -> None:
         check(self._lib.sph_world_snapshot_wait(self._h))
         self._snap = None
