@@ -117,6 +117,24 @@ int sph_world_snapshot_wait(sph_world *w);
 int sph_world_bind_state(sph_world *w, float *pos_vec4_dev, float *vel_vec4_dev);
 int sph_world_device_state(const sph_world *w, float **pos_vec4_dev, float **vel_vec4_dev);
 
+/* Allocation-free step.  No counterpart in the reference: for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Enqueues frames * params->substeps substeps of one scene on `stream` (a
+ * hipStream_t; NULL = the null stream) over the caller's device state, n vec4 positions and
+ * velocities (16-byte aligned device memory), updated in place into the reference's sorted
+ * order, exactly as a sph_world created with these parameters, given this state and stepped
+ * by sph_world_step would leave them (bit for bit).  Returns once enqueued.
+ * It calls no hipMalloc / hipFree: every intermediate buffer lives in one static device
+ * array of at most 72 MiB that each GPU the library runs on holds until the process ends,
+ * which bounds n <= SPH_NOALLOC_MAX_PARTICLES and the table (table_cells, or n when 0) <=
+ * SPH_NOALLOC_MAX_TABLE_CELLS; larger scenes return SPH_ERR_CAPACITY.  All calls on one GPU
+ * share that array: calls that overlap on one GPU (other streams, other threads) race and
+ * must be serialised by the caller.  Positions must be finite and inside the box, as for
+ * sph_world_upload_state (not checked here: the state is never copied to the host). */
+#define SPH_NOALLOC_MAX_PARTICLES 262144u
+#define SPH_NOALLOC_MAX_TABLE_CELLS 524288u
+int sph_step_noalloc(const sph_params *params, uint32_t n, float *pos_vec4_dev, float *vel_vec4_dev,
+                     uint32_t frames, void *stream);
 
 /* Intermediate buffers of the most recent substep, in the reference's formats, for parity
  * checks.  Stages other than CELL_KEYS, SORTED_*, START, END, PRED_POS and DENSITY need

@@ -107,6 +107,7 @@ def load_library() -> C.CDLL:
         "sph_world_destroy": (None, [vp]),
         "sph_world_bind_state": (C.c_int, [vp, vp, vp]),
         "sph_world_device_state": (C.c_int, [vp, P(vp), P(vp)]),
+        "sph_step_noalloc": (C.c_int, [P(SphParams), C.c_uint32, vp, vp, C.c_uint32, vp]),
         "sph_last_error": (C.c_char_p, []),
         "sph_batch_create": (C.c_int, [C.c_uint32, P(SphParams), P(C.c_uint32), P(vp)]),
         "sph_batch_set_kernel": (C.c_int, [vp, C.c_int]),
