@@ -331,8 +331,9 @@ def serving(count: int, rate_hz: float, seed: int = 4, resident: int = 0,
             g=base.g, box_min=(0, 0, 0), box_max=box, frame_dt=base.frame_dt, substeps=base.substeps,
             pressure_multiplier=base.pressure_multiplier, target_density=base.target_density,
             table_cells=1 << max(0, (resident - 1).bit_length()))
-        pos, vel = sph_hip.fill_block(resident, seed, (0, 0, 0), box, 0.1)
-        big = srv.submit(params, pos, vel, 1 << 30)
+        big_state = sph_hip.fill_block(resident, seed, (0, 0, 0), box, 0.1)
+        big_params = params
+        big = srv.submit(big_params, *big_state, 1 << 30)
     t0 = time.perf_counter()
     ids = []
     for t, params, pos, vel, steps in reqs:
@@ -346,6 +347,23 @@ def serving(count: int, rate_hz: float, seed: int = 4, resident: int = 0,
     wall = time.perf_counter() - t0
     st = srv.stats()
     srv.close()
+    if big is not None:
+        # the resident scene stepped alone as one world for as many frames as the served run
+        # made (the fill settles and its neighbour passes slow down as it runs), timing the
+        # last 10: the frame time the served frames are compared with
+        w = sph_hip.World(big_params, resident)
+        w.upload(*big_state)
+        w.step(max(0, int(st["frames"]) - 10))
+        w.sync()
+        w.autotune(3)  # on the settled state, like the served chain's fixed configuration
+        w.step(1)
+        w.sync()
+        t1 = time.perf_counter()
+        w.step(10)
+        w.sync()
+        st["resident_alone_ms_per_frame"] = (time.perf_counter() - t1) * 1e3 / 10
+        st["resident_alone_frames"] = int(st["frames"])
+        w.close()
     st.update({"requests": count, "rate_hz": rate_hz, "wall_s": wall,
                "request_particles": sum(r[2].shape[0] for r in reqs),
                "request_particle_steps": float(sum(r[2].shape[0] * r[4] for r in reqs)),

@@ -132,10 +132,13 @@ int repack(sph_server *s, const std::vector<uint32_t> &old_begin) {
     }
     hipError_t e = hipSuccess;
     if (moved) {
-        // Continuing scenes go through the (idle between frames) sorted-state buffers.
+        // Continuing scenes whose slot range changed go through the (idle between frames)
+        // sorted-state buffers, so that a scene moving into a range another one vacates
+        // never overlaps itself; scenes that keep their slots (a long-lived scene admitted
+        // first keeps slot 0) are not touched.
         for (size_t i = 0; i < count && e == hipSuccess; ++i) {
             Request *r = s->resident[i];
-            if (r->fresh) continue;
+            if (r->fresh || old_begin[i] == r->p_begin) continue;
             const size_t bytes = (size_t)r->n * 16;
             e = hipMemcpyAsync(s->chain.b.pos_sorted + r->p_begin, s->chain.b.pos + old_begin[i], bytes,
                                hipMemcpyDeviceToDevice, s->stream);
@@ -145,7 +148,7 @@ int repack(sph_server *s, const std::vector<uint32_t> &old_begin) {
         }
         for (size_t i = 0; i < count && e == hipSuccess; ++i) {
             Request *r = s->resident[i];
-            if (r->fresh) continue;
+            if (r->fresh || old_begin[i] == r->p_begin) continue;
             const size_t bytes = (size_t)r->n * 16;
             e = hipMemcpyAsync(s->chain.b.pos + r->p_begin, s->chain.b.pos_sorted + r->p_begin, bytes,
                                hipMemcpyDeviceToDevice, s->stream);

@@ -110,3 +110,44 @@ def test_served_scenes_with_offset_boxes(gpu, orc):
         rp, rv = orc.step(p, q, v, 2 + i)
         _bitexact(f"request {i} pos", results[i][0], rp)
         _bitexact(f"request {i} vel", results[i][1], rv)
+
+
+def test_served_resident_32m_beside_the_stream(gpu, orc):
+    """Mixed serving at scale on the served path: the 2^25-particle uniform fill of a
+    160 x 40 x 80 box (T = N) resident for two frames, submitted first, beside six requests
+    of the serving stream that retire after one, two and three frames, under a 200 GB
+    budget.  Membership changes twice while the big scene is resident (it keeps slot 0 and
+    is not moved) and once after it retires; every result is bit-exact against the oracle
+    run on that request alone."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import bench
+    from sph_hip.workloads import UNIFORM_32M as W
+    big_params = W.params(table_cells=0)
+    big_pos, big_vel = W.state()
+    reqs = bench.request_stream(6, seed=4, rate_hz=10.0)
+    steps = [1, 2, 3, 1, 2, 3]
+    total = W.n + sum(r[2].shape[0] for r in reqs)
+    cells = W.n + sum(r[1].table_cells for r in reqs)
+    print(f"served 2^25 + {len(reqs)} requests, {total} particles", flush=True)
+    with gpu.Server(hbm_budget_bytes=200 << 30, max_scenes=16, max_particles=total,
+                    max_table_cells=cells, substeps=3) as srv:
+        big = srv.submit(big_params, big_pos, big_vel, 2)
+        ids = [srv.submit(params, pos, vel, steps[i]) for i, (_, params, pos, vel, _) in enumerate(reqs)]
+        results = []
+        for rid in ids:
+            srv.wait(rid, 600.0)
+            results.append(srv.result(rid))
+        srv.wait(big, 600.0)
+        got_big = srv.result(big)
+        st = srv.stats()
+    assert st["failed"] == 0 and st["completed"] == 7 and st["frames"] >= 3
+    print("oracle", flush=True)
+    for i, (_, params, pos, vel, _) in enumerate(reqs):
+        rp, rv = orc.step(params, pos, vel, steps[i])
+        _bitexact(f"request {i} pos", results[i][0], rp)
+        _bitexact(f"request {i} vel", results[i][1], rv)
+    rp, rv = orc.step(big_params, big_pos, big_vel, 2)
+    _bitexact("resident pos", got_big[0], rp)
+    _bitexact("resident vel", got_big[1], rv)
