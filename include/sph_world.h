@@ -185,6 +185,12 @@ int sph_world_kernel_config(const sph_world *w, char *buf, size_t bytes);
  * testing every configuration.  All configurations give bit-identical results. */
 int sph_world_set_lane_config(sph_world *w, int config);
 int sph_lane_configs(void);
+/* Pins the radix sort's digit width (8, 11 or 12 bits; the autotuner otherwise chooses it
+ * per size bucket among the widths that need fewer passes than 8-bit digits).  A world
+ * holds look-back memory for the widths whose per-tile words stay within 16 B per particle
+ * (or 8 MiB); others return SPH_ERR_CAPACITY.  All widths give the same stable order, the
+ * reference's 4 x 8-bit sort (commands.c:106-161) included. */
+int sph_world_set_radix_bits(sph_world *w, int bits);
 
 int sph_world_set_debug(sph_world *w, int enable);
 int sph_world_debug_dump(sph_world *w, int stage, void *dst, size_t bytes);

@@ -5,7 +5,7 @@
 // multiRadixSort.comp:57-89, in-WG ordered scatter :96-139), so its output is the pairs
 // ordered by (key, index); this sort produces exactly that order.
 //
-// Design (one kernel per 8-bit digit, "onesweep" style):
+// Design (one kernel per digit of 8, 11 or 12 bits, chosen per size bucket; "onesweep" style):
 //   * the digit histograms of every pass are accumulated once, by the hash kernel;
 //   * each workgroup takes the next tile in launch order from an atomic counter, ranks its
 //     keys with wave64 ballots (8 ballots give each lane the mask of lanes holding the same
@@ -13,7 +13,7 @@
 //     publishes its per-digit counts and resolves its global offsets by decoupled
 //     look-back over earlier tiles, then scatters through LDS so that each digit's run is
 //     written contiguously.
-//   * Only ceil(log2(T)/8) passes run: keys are below the table size T.
+//   * Only ceil(log2(T)/w) passes of w-bit digits run: keys are below the table size T.
 #include "sph_device.hpp"
 #include "sph_kernels.hpp"
 
@@ -52,24 +52,28 @@ __device__ __forceinline__ void store_relaxed(uint32_t *p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-template <int kItems>
+// One LSD pass over kW-bit digits (kW = 8, 11 or 12: 256, 2048 or 4096 bins).  Thread
+// `tid` owns the kDPT consecutive digits tid * kDPT .. tid * kDPT + kDPT - 1 in the per-digit
+// phase (tile counts, scans, look-back).
+template <int kItems, int kW>
 __global__ __launch_bounds__(kBlock) void k_radix_pass(
     const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
     uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, uint32_t n, uint32_t shift,
     uint32_t bits, const uint32_t *__restrict__ ghist, uint32_t *lookback,
     uint32_t *tile_counter) {
     constexpr int kTile = kBlock * kItems;
-    __shared__ uint32_t s_whist[kWaves][256];
+    constexpr int kBins = 1 << kW;
+    constexpr int kDPT = kBins / kBlock;
+    __shared__ uint32_t s_whist[kWaves][kBins];  // per wave: count, then LDS start of the wave's run
     __shared__ uint32_t s_keys[kTile];
     __shared__ uint32_t s_vals[kTile];
-    __shared__ uint32_t s_digit_off[256];
-    __shared__ uint32_t s_gbase[256];
+    __shared__ int32_t s_delta[kBins];           // global start minus LDS start of each digit
     __shared__ uint32_t s_tot[kWaves];
     __shared__ uint32_t s_tile;
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
-    for (int i = tid; i < kWaves * 256; i += kBlock) (&s_whist[0][0])[i] = 0u;
+    for (int i = tid; i < kWaves * kBins; i += kBlock) (&s_whist[0][0])[i] = 0u;
     __syncthreads();
     const uint32_t tile = s_tile;
     const uint32_t wave_base = tile * kTile + (uint32_t)w * (64u * kItems);
@@ -108,47 +112,83 @@ __global__ __launch_bounds__(kBlock) void k_radix_pass(
     }
     __syncthreads();
 
-    // Thread `tid` owns digit `tid`: per-wave exclusive offsets and the tile's count.
-    uint32_t tcount = 0;
-    {
+    // Per owned digit: the tile's count, with each wave's count turned into the exclusive
+    // count of the waves before it.
+    uint32_t tcount[kDPT];
+    uint32_t tsum = 0, gsum = 0;
+#pragma unroll
+    for (int q = 0; q < kDPT; ++q) {
+        const int d = tid * kDPT + q;
         uint32_t sum = 0;
 #pragma unroll
         for (int ww = 0; ww < kWaves; ++ww) {
-            const uint32_t c = s_whist[ww][tid];
-            s_whist[ww][tid] = sum;
+            const uint32_t c = s_whist[ww][d];
+            s_whist[ww][d] = sum;
             sum += c;
         }
-        tcount = sum;
+        tcount[q] = sum;
+        tsum += sum;
+        gsum += ghist[d];
     }
-    const uint32_t gexcl = block_exclusive_scan(ghist[tid], s_tot);
-    const uint32_t local_off = block_exclusive_scan(tcount, s_tot);
-    s_digit_off[tid] = local_off;
+    const uint32_t gbase = block_exclusive_scan(gsum, s_tot);
+    const uint32_t lbase = block_exclusive_scan(tsum, s_tot);
 
-    // Decoupled look-back: the global start of this tile's digit-`tid` run.
-    uint32_t *mine = lookback + (size_t)tile * 256u + (uint32_t)tid;
-    uint32_t excl;
-    if (tile == 0) {
-        excl = gexcl;
-        store_relaxed(mine, kFlagPrefix | (excl + tcount));
-    } else {
-        store_relaxed(mine, kFlagAggregate | tcount);
-        uint32_t acc = 0;
-        int32_t t = (int32_t)tile - 1;
-        while (true) {
-            const uint32_t v = load_relaxed(lookback + (size_t)t * 256u + (uint32_t)tid);
+    // Decoupled look-back for the owned digits, walked together: the global start of this
+    // tile's run of each digit.
+    uint32_t excl[kDPT], acc[kDPT], loff[kDPT];
+    int32_t at[kDPT];
+    uint32_t pending = 0;
+    {
+        uint32_t g = gbase, l = lbase;
+#pragma unroll
+        for (int q = 0; q < kDPT; ++q) {
+            const int d = tid * kDPT + q;
+            loff[q] = l;
+            l += tcount[q];
+            const uint32_t gd = g;
+            g += ghist[d];
+            uint32_t *mine = lookback + (size_t)tile * kBins + (uint32_t)d;
+            if (tile == 0) {
+                excl[q] = gd;
+                store_relaxed(mine, kFlagPrefix | (gd + tcount[q]));
+            } else {
+                store_relaxed(mine, kFlagAggregate | tcount[q]);
+                acc[q] = 0u;
+                at[q] = (int32_t)tile - 1;
+                pending |= 1u << q;
+            }
+        }
+    }
+    while (pending) {
+        bool waited = false;
+#pragma unroll
+        for (int q = 0; q < kDPT; ++q) {
+            if (!((pending >> q) & 1u)) continue;
+            const int d = tid * kDPT + q;
+            const uint32_t v = load_relaxed(lookback + (size_t)at[q] * kBins + (uint32_t)d);
             const uint32_t f = v & kFlagMask;
             if (f == 0u) {
-                __builtin_amdgcn_s_sleep(1);
+                waited = true;
                 continue;
             }
-            acc += v & kValueMask;
-            if (f == kFlagPrefix) break;
-            --t;
+            acc[q] += v & kValueMask;
+            if (f == kFlagPrefix) {
+                excl[q] = acc[q];
+                store_relaxed(lookback + (size_t)tile * kBins + (uint32_t)d, kFlagPrefix | (acc[q] + tcount[q]));
+                pending &= ~(1u << q);
+            } else {
+                --at[q];
+            }
         }
-        excl = acc;
-        store_relaxed(mine, kFlagPrefix | (excl + tcount));
+        if (waited) __builtin_amdgcn_s_sleep(1);
     }
-    s_gbase[tid] = excl;
+#pragma unroll
+    for (int q = 0; q < kDPT; ++q) {
+        const int d = tid * kDPT + q;
+        s_delta[d] = (int32_t)excl[q] - (int32_t)loff[q];
+#pragma unroll
+        for (int ww = 0; ww < kWaves; ++ww) s_whist[ww][d] += loff[q];
+    }
     __syncthreads();
 
     // Order the tile by digit in LDS, then write each digit's run contiguously.
@@ -157,7 +197,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_pass(
         const uint32_t pos = wave_base + (uint32_t)k * 64u + (uint32_t)lane;
         if (pos < n) {
             const uint32_t d = (key[k] >> shift) & dmask;
-            const uint32_t lp = s_digit_off[d] + s_whist[w][d] + rank[k];
+            const uint32_t lp = s_whist[w][d] + rank[k];
             s_keys[lp] = key[k];
             s_vals[lp] = val[k];
         }
@@ -168,7 +208,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_pass(
     for (uint32_t i = tid; i < tile_n; i += kBlock) {
         const uint32_t kk = s_keys[i];
         const uint32_t d = (kk >> shift) & dmask;
-        const uint32_t gp = s_gbase[d] + (i - s_digit_off[d]);
+        const uint32_t gp = (uint32_t)(s_delta[d] + (int32_t)i);
         kout[gp] = kk;
         vout[gp] = s_vals[i];
     }
@@ -183,10 +223,21 @@ uint32_t radix_tile_size() { return (uint32_t)(kBlock * kItemsDefault); }
 hipError_t radix_pass(hipStream_t stream, const uint32_t *kin, const uint32_t *vin,
                       uint32_t *kout, uint32_t *vout, uint32_t n, uint32_t shift,
                       uint32_t bits, const uint32_t *ghist, uint32_t *lookback,
-                      uint32_t *tile_counter) {
+                      uint32_t *tile_counter, uint32_t digit_bits) {
     const uint32_t tiles = (n + radix_tile_size() - 1u) / radix_tile_size();
-    hipLaunchKernelGGL(k_radix_pass<kItemsDefault>, dim3(tiles), dim3(kBlock), 0, stream, kin,
-                       vin, kout, vout, n, shift, bits, ghist, lookback, tile_counter);
+    switch (digit_bits) {
+        case 11:
+            hipLaunchKernelGGL((k_radix_pass<kItemsDefault, 11>), dim3(tiles), dim3(kBlock), 0, stream, kin,
+                               vin, kout, vout, n, shift, bits, ghist, lookback, tile_counter);
+            break;
+        case 12:
+            hipLaunchKernelGGL((k_radix_pass<kItemsDefault, 12>), dim3(tiles), dim3(kBlock), 0, stream, kin,
+                               vin, kout, vout, n, shift, bits, ghist, lookback, tile_counter);
+            break;
+        default:
+            hipLaunchKernelGGL((k_radix_pass<kItemsDefault, 8>), dim3(tiles), dim3(kBlock), 0, stream, kin,
+                               vin, kout, vout, n, shift, bits, ghist, lookback, tile_counter);
+    }
     return hipGetLastError();
 }
 

@@ -212,11 +212,25 @@ static int batch_autotune(sph_batch *b, uint32_t substeps) {
     hipEvent_t t0 = nullptr, t1 = nullptr;
     if (e == hipSuccess) e = hipEventCreate(&t0);
     if (e == hipSuccess) e = hipEventCreate(&t1);
+    // Then, with the fastest kernels, the radix widths that need fewer passes than 8-bit
+    // digits (code = kernels | width << 8).
     std::vector<int> cands;
     for (int c = 0; c < kNumLaneConfigs; ++c) cands.push_back(c);
     if (b->b_total) cands.push_back(-1);
-    for (int v : cands) {
-        if (e != hipSuccess) break;
+    b->chain.set_radix_bits(8u);
+    const size_t n_kernel_cands = cands.size();
+    for (size_t ci = 0;; ++ci) {
+        if (ci == n_kernel_cands && e == hipSuccess && best >= 0) {
+            const uint32_t kb = b->chain.key_bits;
+            for (uint32_t width : {11u, 12u})
+                if (b->chain.lookback_words_for(width) <= b->chain.lookback_cap &&
+                    (kb + width - 1u) / width < (kb + 7u) / 8u)
+                    cands.push_back(best | (int)(width << 8));
+        }
+        if (ci >= cands.size() || e != hipSuccess) break;
+        const int code = cands[ci];
+        const int v = code < 0 ? -1 : (code & 0xFF);
+        b->chain.set_radix_bits(code < 0 || (code >> 8) == 0 ? 8u : (uint32_t)(code >> 8));
         if (v >= 0) b->chain.lane_cfg = v;
         int rc = batch_apply(b, v < 0 ? SPH_KERNEL_BRICK : SPH_KERNEL_DIRECT, false);
         if (rc) { e = hipErrorUnknown; break; }
@@ -229,7 +243,7 @@ static int batch_autotune(sph_batch *b, uint32_t substeps) {
         if (e == hipSuccess) e = hipEventSynchronize(t1);
         float ms = 0.0f;
         if (e == hipSuccess) e = hipEventElapsedTime(&ms, t0, t1);
-        if (e == hipSuccess && ms < best_ms) { best_ms = ms; best = v; }
+        if (e == hipSuccess && ms < best_ms) { best_ms = ms; best = code; }
     }
     if (e == hipSuccess) e = hipMemcpyAsync(b->chain.b.pos, sp, bytes, hipMemcpyDeviceToDevice, b->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(b->chain.b.vel, sv, bytes, hipMemcpyDeviceToDevice, b->stream);
@@ -241,7 +255,8 @@ static int batch_autotune(sph_batch *b, uint32_t substeps) {
     if (e != hipSuccess) return hip_fail(e, "batch autotune");
     b->stepped = false;  // the table is the last trial's, not the restored state's
     b->tuned_variant = best < 0 ? SPH_KERNEL_BRICK : SPH_KERNEL_DIRECT;
-    if (best >= 0) b->chain.lane_cfg = best;
+    if (best >= 0) b->chain.lane_cfg = best & 0xFF;
+    b->chain.set_radix_bits(best < 0 || (best >> 8) == 0 ? 8u : (uint32_t)(best >> 8));
     return batch_apply(b, b->tuned_variant, false);
 }
 

@@ -42,6 +42,8 @@ struct Chain {
     bool bricks = false;       // density/force through the brick kernels (else per lane)
     bool cell_order = true;    // the last enqueued substep left its arrays in cell order
     int lane_cfg = kDefaultLaneConfig;  // launch configuration of the per-lane kernels
+    uint32_t radix_bits = 8;   // radix digit width (8, 11 or 12)
+    uint64_t lookback_cap = 0; // look-back words allocated (fixes which widths are possible)
     uint32_t n_cap = 0, t_cap = 0, b_cap = 0, u_cap = 0, g_cap = 0;  // allocated capacity
 
     // Allocates for the given capacity and configures the chain to use all of it.
@@ -51,6 +53,12 @@ struct Chain {
     // and grid_total halo-grid cells (within capacity).
     void configure(uint32_t n, uint32_t table_total, uint32_t brick_total, uint32_t bucket_total,
                    uint32_t grid_total);
+    // Look-back words a radix width needs at the configured size.
+    uint64_t lookback_words_for(uint32_t digit_bits) const;
+    // Selects the radix digit width; false when it is not 8, 11 or 12 or its look-back
+    // words exceed what was allocated.
+    bool set_radix_bits(uint32_t digit_bits);
+    uint32_t ghist_words() const { return passes << radix_bits; }
     int set_debug(bool on);
     void release();
     const uint32_t *sorted_keys() const;

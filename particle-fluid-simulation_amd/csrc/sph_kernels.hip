@@ -19,7 +19,7 @@ constexpr int kBlock = 256;
 // Hash: updateCellEntries.comp:69-80 (key of the CURRENT position, value = slot index,
 // implicit here), plus the digit histograms of the radix passes and the clears that
 // clearStartIndices.comp:47-52 performs, moved to a point where the table is dead.
-template <class Scenes>
+template <class Scenes, int kW>
 __global__ __launch_bounds__(kBlock) void k_hash(
     Scenes sc, const float4 *__restrict__ pos, uint32_t *__restrict__ keys, uint32_t n,
     uint32_t passes, uint32_t *__restrict__ ghist, uint32_t hash_blocks,
@@ -28,10 +28,12 @@ __global__ __launch_bounds__(kBlock) void k_hash(
     uint32_t *__restrict__ counters,
     uint32_t *__restrict__ bcount, uint32_t bucket_total, uint32_t *__restrict__ blookback,
     uint32_t blookback_words) {
-    __shared__ uint32_t s_hist[4][256];
+    constexpr uint32_t kBins = 1u << kW;         // radix digits of kW bits
+    constexpr uint32_t kMaxP = (32u + kW - 1u) / kW;
+    __shared__ uint32_t s_hist[kMaxP][kBins];
     const uint32_t tid = threadIdx.x;
     if (blockIdx.x < hash_blocks) {
-        for (uint32_t i = tid; i < 4u * 256u; i += kBlock) (&s_hist[0][0])[i] = 0u;
+        for (uint32_t i = tid; i < passes * kBins; i += kBlock) (&s_hist[0][0])[i] = 0u;
         __syncthreads();
         const uint32_t per_block = ((n + hash_blocks - 1u) / hash_blocks + kBlock - 1u) / kBlock * kBlock;
         const uint32_t begin = blockIdx.x * per_block;
@@ -44,12 +46,12 @@ __global__ __launch_bounds__(kBlock) void k_hash(
             const uint32_t cz = (uint32_t)cell_of(p.z, c.minz, c.R);
             const uint32_t key = c.table_base + (cell_hash(cx, cy, cz) & c.mask);
             keys[i] = key;
-            for (uint32_t q = 0; q < passes; ++q) atomicAdd(&s_hist[q][(key >> (8u * q)) & 255u], 1u);
+            for (uint32_t q = 0; q < passes; ++q) atomicAdd(&s_hist[q][(key >> (kW * q)) & (kBins - 1u)], 1u);
         }
         __syncthreads();
-        for (uint32_t q = 0; q < passes; ++q) {
-            const uint32_t v = s_hist[q][tid];
-            if (v) atomicAdd(&ghist[q * 256u + tid], v);
+        for (uint32_t i = tid; i < passes * kBins; i += kBlock) {
+            const uint32_t v = (&s_hist[0][0])[i];
+            if (v) atomicAdd(&ghist[i], v);
         }
     } else {
         const uint32_t b = blockIdx.x - hash_blocks;
@@ -78,10 +80,10 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
     float4 *__restrict__ pred_pos, float4 *__restrict__ dbg_pred_vel, int2 *__restrict__ table,
     uint32_t *__restrict__ ghist, uint32_t *__restrict__ hfull, uint32_t *__restrict__ brick_flag,
     uint32_t *__restrict__ stray, uint32_t *__restrict__ counters, uint32_t *__restrict__ bcount,
-    uint2 *__restrict__ run_info) {
+    uint2 *__restrict__ run_info, uint32_t ghist_words) {
     const uint32_t tid = threadIdx.x;
     if (blockIdx.x == 0)
-        for (uint32_t i = tid; i < 4u * 256u; i += kBlock) ghist[i] = 0u;
+        for (uint32_t i = tid; i < ghist_words; i += kBlock) ghist[i] = 0u;
     const uint32_t j = blockIdx.x * kBlock + tid;
     bool to_stray = false;
     if (j < n) {
@@ -255,7 +257,7 @@ inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1u) / kBlock; }
 template <class Scenes>
 hipError_t launch_hash(hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n,
                        uint32_t table_total, uint32_t passes, uint32_t lookback_words,
-                       uint32_t bucket_total) {
+                       uint32_t bucket_total, uint32_t digit_bits) {
     // 2K keys per hashing workgroup: enough workgroups to fill the chip, few enough global
     // histogram atomics; the clearing workgroups run beside them.
     const uint32_t hash_blocks = (n + 2047u) / 2048u;
@@ -265,7 +267,8 @@ hipError_t launch_hash(hipStream_t stream, const Scenes &sc, const ChainBuffers 
     uint32_t clear_blocks = (clear_work + 4u * kBlock - 1u) / (4u * kBlock);
     if (clear_blocks < 1u) clear_blocks = 1u;
     if (clear_blocks > 2048u) clear_blocks = 2048u;
-    hipLaunchKernelGGL(k_hash<Scenes>, dim3(hash_blocks + clear_blocks), dim3(kBlock), 0, stream,
+    auto kernel = digit_bits == 12 ? k_hash<Scenes, 12> : digit_bits == 11 ? k_hash<Scenes, 11> : k_hash<Scenes, 8>;
+    hipLaunchKernelGGL(kernel, dim3(hash_blocks + clear_blocks), dim3(kBlock), 0, stream,
                        sc, b.pos, b.keys, n, passes, b.ghist, hash_blocks, b.table,
                        bucket_total ? b.crun : nullptr, table_total,
                        b.lookback, lookback_words, b.tile_ctr, b.counters, b.bcount, bucket_total,
@@ -276,12 +279,12 @@ hipError_t launch_hash(hipStream_t stream, const Scenes &sc, const ChainBuffers 
 template <class Scenes>
 hipError_t launch_reorder(hipStream_t stream, const Scenes &sc, const ChainBuffers &b,
                           const uint32_t *sorted_keys, const uint32_t *sorted_idx, uint32_t n,
-                          bool bricks) {
+                          bool bricks, uint32_t ghist_words) {
     hipLaunchKernelGGL(k_reorder<Scenes>, dim3(blocks_for(n)), dim3(kBlock), 0, stream, sc,
                        sorted_keys, sorted_idx, n, b.pos, b.vel, b.pos_sorted, b.vel_sorted,
                        b.pred_pos, b.dbg_pred_vel, b.table, b.ghist, b.hfull,
                        bricks ? b.brick_flag : nullptr, b.stray, b.counters, bricks ? nullptr : b.bcount,
-                       b.run_info);
+                       b.run_info, ghist_words);
     return hipGetLastError();
 }
 
@@ -346,9 +349,10 @@ hipError_t launch_force(hipStream_t stream, const Scenes &sc, const ChainBuffers
 
 #define SPH_INSTANTIATE(S)                                                                       \
     template hipError_t launch_hash<S>(hipStream_t, const S &, const ChainBuffers &, uint32_t,   \
-                                       uint32_t, uint32_t, uint32_t, uint32_t);                  \
+                                       uint32_t, uint32_t, uint32_t, uint32_t, uint32_t);        \
     template hipError_t launch_reorder<S>(hipStream_t, const S &, const ChainBuffers &,          \
-                                          const uint32_t *, const uint32_t *, uint32_t, bool);   \
+                                          const uint32_t *, const uint32_t *, uint32_t, bool,    \
+                                          uint32_t);                                             \
     template hipError_t launch_density<S>(hipStream_t, const S &, const ChainBuffers &, uint32_t, int); \
     template hipError_t launch_force<S>(hipStream_t, const S &, const ChainBuffers &, uint32_t, int);
 

@@ -32,8 +32,8 @@ struct ChainBuffers {
                          // density pass has run)
     float2 *dens;        // (density, near density), sorted order
     int2 *table;         // (start, end) per cell, -1 = empty
-    uint32_t *ghist;     // [4][256] digit histograms
-    uint32_t *lookback;  // [4][tiles][256] look-back words
+    uint32_t *ghist;     // [passes][bins] digit histograms (kGhistWords)
+    uint32_t *lookback;  // [passes][tiles][bins] look-back words
     uint32_t *tile_ctr;  // [4] tile counters
     float4 *dbg_pred_vel;   // optional (debug capture): predicted velocities, sorted
     float4 *dbg_vel_force;  // optional (debug capture): velocity after force, sorted
@@ -58,11 +58,15 @@ struct ChainBuffers {
 uint32_t radix_tile_size();
 // tile_ctr[0..3]: radix passes; [4]: bucket scan.
 constexpr uint32_t kTileCounters = 8;
+// Radix digit widths: 8, 11 or 12 bits (256, 2048 or 4096 bins), at most 4 passes; the
+// digit histograms of all passes, [pass][bins], fit kGhistWords.
+constexpr uint32_t kGhistWords = 3u * 4096u;
 
+// One stable LSD pass over `bits` (<= digit_bits) bits at `shift`, digit_bits = 8, 11, 12.
 hipError_t radix_pass(hipStream_t stream, const uint32_t *kin, const uint32_t *vin,
                       uint32_t *kout, uint32_t *vout, uint32_t n, uint32_t shift,
                       uint32_t bits, const uint32_t *ghist, uint32_t *lookback,
-                      uint32_t *tile_counter);
+                      uint32_t *tile_counter, uint32_t digit_bits);
 
 // Hash pass: keys of the current positions, digit histograms for `passes` passes, and
 // clearing of the cell table, the look-back words and the tile counters; with
@@ -70,7 +74,7 @@ hipError_t radix_pass(hipStream_t stream, const uint32_t *kin, const uint32_t *v
 template <class Scenes>
 hipError_t launch_hash(hipStream_t stream, const Scenes &sc, const ChainBuffers &b,
                        uint32_t n, uint32_t table_total, uint32_t passes,
-                       uint32_t lookback_words, uint32_t bucket_total);
+                       uint32_t lookback_words, uint32_t bucket_total, uint32_t digit_bits);
 
 // Cell start/end table from the sorted keys; clears histograms.  With `bricks`, also
 // gathers the state into hash order with predicted positions, marks non-empty bricks and
@@ -79,7 +83,7 @@ hipError_t launch_hash(hipStream_t stream, const Scenes &sc, const ChainBuffers 
 template <class Scenes>
 hipError_t launch_reorder(hipStream_t stream, const Scenes &sc, const ChainBuffers &b,
                           const uint32_t *sorted_keys, const uint32_t *sorted_idx, uint32_t n,
-                          bool bricks);
+                          bool bricks, uint32_t ghist_words);
 
 // Cell-ordered layout (sph_layout.hip): the bucket scan, then the gather of the state into
 // cell order with predicted positions and each key's cell-ordered run, then the halo grids.

@@ -3,6 +3,7 @@
 // once into a hipGraph and replayed for every substep.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -167,9 +168,22 @@ void Chain::configure(uint32_t n_, uint32_t table_total_, uint32_t brick_total_,
     bucket_total = bucket_total_;
     grid_total = grid_total_;
     key_bits = key_bits_for(table_total);
-    passes = (key_bits + 7u) / 8u;
+    passes = (key_bits + radix_bits - 1u) / radix_bits;
     tiles = (n + radix_tile_size() - 1u) / radix_tile_size();
-    lookback_words = passes * tiles * 256u;
+    lookback_words = passes * tiles << radix_bits;
+}
+
+uint64_t Chain::lookback_words_for(uint32_t w) const {
+    return (uint64_t)((key_bits + w - 1u) / w) * tiles << w;
+}
+
+bool Chain::set_radix_bits(uint32_t w) {
+    if (w != 8u && w != 11u && w != 12u) return false;
+    if (lookback_words_for(w) > lookback_cap) return false;
+    radix_bits = w;
+    passes = (key_bits + radix_bits - 1u) / radix_bits;
+    lookback_words = passes * tiles << radix_bits;
+    return true;
 }
 
 int Chain::allocate(uint32_t n_, uint32_t table_total_, uint32_t brick_total_,
@@ -179,8 +193,16 @@ int Chain::allocate(uint32_t n_, uint32_t table_total_, uint32_t brick_total_,
     b_cap = brick_total_;
     u_cap = bucket_total_;
     g_cap = grid_total_;
+    radix_bits = 8;
     configure(n_, table_total_, brick_total_, bucket_total_, grid_total_);
     bricks = brick_total > 0;
+    // look-back words for every radix width whose words stay within 16 B per particle (or
+    // 8 MiB): wider digits mean fewer passes but 2^w words per tile
+    lookback_cap = lookback_words_for(8);
+    for (uint32_t w : {11u, 12u}) {
+        const uint64_t words = lookback_words_for(w);
+        if (words * 4 <= std::max<uint64_t>(16ull * n, 8ull << 20) && words > lookback_cap) lookback_cap = words;
+    }
     struct Req { void **p; size_t bytes; };
     const size_t N = n;
     Req reqs[] = {
@@ -189,8 +211,8 @@ int Chain::allocate(uint32_t n_, uint32_t table_total_, uint32_t brick_total_,
         {(void **)&b.sort_k[1], N * 4}, {(void **)&b.sort_v[1], N * 4},
         {(void **)&b.pred_pos, N * 16}, {(void **)&b.pos_sorted, N * 16},
         {(void **)&b.vel_sorted, N * 16}, {(void **)&b.dens, N * 8},
-        {(void **)&b.table, (size_t)table_total * 8}, {(void **)&b.ghist, 4 * 256 * 4},
-        {(void **)&b.lookback, (size_t)(lookback_words ? lookback_words : 1) * 4},
+        {(void **)&b.table, (size_t)table_total * 8}, {(void **)&b.ghist, kGhistWords * 4},
+        {(void **)&b.lookback, (size_t)(lookback_cap ? lookback_cap : 1) * 4},
         {(void **)&b.tile_ctr, kTileCounters * 4}, {(void **)&b.hfull, N * 4}, {(void **)&b.stray, N * 4},
         {(void **)&b.bcount, (size_t)bucket_total * 4},
         {(void **)&b.blookback, (size_t)((bucket_total + kBucketTile - 1u) / kBucketTile) * 4},
@@ -209,7 +231,7 @@ int Chain::allocate(uint32_t n_, uint32_t table_total_, uint32_t brick_total_,
             return fail(SPH_ERR_OUT_OF_MEMORY, "hipMalloc(%zu): %s", r.bytes, hipGetErrorString(e));
         }
     }
-    hipError_t e = hipMemset(b.ghist, 0, 4 * 256 * 4);
+    hipError_t e = hipMemset(b.ghist, 0, kGhistWords * 4);
     if (e == hipSuccess) e = hipMemset(b.counters, 0, 8 * 4);
     if (e == hipSuccess) e = hipMemset(b.brick_flag, 0, (size_t)(brick_total ? brick_total : 1) * 4);
     if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -254,19 +276,21 @@ hipError_t Chain::enqueue(hipStream_t s, const Scenes &sc, PassEvents *ev) {
     hipError_t e;
     if (ev) hipEventRecord(ev->ev[0], s);
     cell_order = !bricks;
-    e = launch_hash(s, sc, b, n, table_total, passes, lookback_words, cell_order ? bucket_total : 0u);
+    e = launch_hash(s, sc, b, n, table_total, passes, lookback_words, cell_order ? bucket_total : 0u, radix_bits);
     if (e != hipSuccess) return e;
     if (ev) hipEventRecord(ev->ev[1], s);
+    const uint32_t bins = 1u << radix_bits;
     for (uint32_t q = 0; q < passes; ++q) {
         const uint32_t *kin = q == 0 ? b.keys : b.sort_k[(q - 1u) & 1u];
         const uint32_t *vin = q == 0 ? nullptr : b.sort_v[(q - 1u) & 1u];
-        const uint32_t bits = key_bits - 8u * q < 8u ? key_bits - 8u * q : 8u;
-        e = radix_pass(s, kin, vin, b.sort_k[q & 1u], b.sort_v[q & 1u], n, 8u * q, bits,
-                       b.ghist + 256u * q, b.lookback + (size_t)q * tiles * 256u, b.tile_ctr + q);
+        const uint32_t shift = radix_bits * q;
+        const uint32_t bits = key_bits - shift < radix_bits ? key_bits - shift : radix_bits;
+        e = radix_pass(s, kin, vin, b.sort_k[q & 1u], b.sort_v[q & 1u], n, shift, bits,
+                       b.ghist + bins * q, b.lookback + (size_t)q * tiles * bins, b.tile_ctr + q, radix_bits);
         if (e != hipSuccess) return e;
     }
     if (ev) hipEventRecord(ev->ev[2], s);
-    if ((e = launch_reorder(s, sc, b, sorted_keys(), sorted_idx(), n, bricks)) != hipSuccess) return e;
+    if ((e = launch_reorder(s, sc, b, sorted_keys(), sorted_idx(), n, bricks, ghist_words())) != hipSuccess) return e;
     if (bricks) {
         if ((e = launch_queue(s, b, brick_total)) != hipSuccess) return e;
     } else {
@@ -387,12 +411,17 @@ uint64_t tune_bucket(uint32_t n, uint32_t T, uint64_t bricks) {
 }
 
 // A tuning code names one configuration: 0..kNumLaneConfigs-1 = the per-lane kernels with
-// that launch configuration, kCodeBrick = the brick kernels.
+// that launch configuration, kCodeBrick = the brick kernels; bits 8 and up carry the radix
+// digit width (0: 8 bits).
 constexpr int kCodeBrick = 100;
+constexpr int kCodeMask = 0xFF;
 
 int world_apply_code(sph_world *w, int code) {
-    w->chain.bricks = code == kCodeBrick && w->chain.brick_total > 0;
-    if (code >= 0 && code < kNumLaneConfigs) w->chain.lane_cfg = code;
+    const int kernels = code & kCodeMask;
+    w->chain.bricks = kernels == kCodeBrick && w->chain.brick_total > 0;
+    if (kernels >= 0 && kernels < kNumLaneConfigs) w->chain.lane_cfg = kernels;
+    const uint32_t width = (uint32_t)code >> 8;
+    if (!w->chain.set_radix_bits(width ? width : 8u)) w->chain.set_radix_bits(8u);
     w->graph.drop();
     return SPH_OK;
 }
@@ -418,7 +447,19 @@ int world_autotune(sph_world *w, uint32_t substeps, int *chosen, double *ms_out)
     hipEvent_t t0 = nullptr, t1 = nullptr;
     (void)hipEventCreate(&t0);
     (void)hipEventCreate(&t1);
-    for (int code : cands) {
+    // kernels first (at 8-bit digits), then the radix widths with the chosen kernels
+    const size_t n_kernel_cands = cands.size();
+    for (size_t ci = 0;; ++ci) {
+        if (ci == n_kernel_cands && e == hipSuccess) {
+            // widths that need fewer passes than 8-bit digits, with the fastest kernels
+            const int kbest = best & kCodeMask;
+            for (uint32_t width : {11u, 12u})
+                if (w->chain.lookback_words_for(width) <= w->chain.lookback_cap &&
+                    (w->chain.key_bits + width - 1u) / width < (w->chain.key_bits + 7u) / 8u)
+                    cands.push_back(kbest | (int)(width << 8));
+        }
+        if (ci >= cands.size()) break;
+        const int code = cands[ci];
         world_apply_code(w, code);
         // one untimed substep, then the timed ones, each from the saved state
         e = hipMemcpyAsync(w->chain.b.pos, save_p, bytes, hipMemcpyDeviceToDevice, w->stream);
@@ -500,7 +541,9 @@ uint64_t sph_world_footprint(uint32_t n, uint32_t table_cells) {
     // run; buckets (at most 2n + 1024), the halo grid (at most 16n + 4096 cells) and
     // look-back words
     return (uint64_t)n * (16 * 2 + 4 + 16 + 16 * 3 + 8 + 8 + 4 + 4 * kList + 1) + T * 24 +
-           (2 * (uint64_t)n + 1024) * 4 + (16 * (uint64_t)n + 4096) * 8 + 4 * tiles * 256 * 4 + 4096;
+           (2 * (uint64_t)n + 1024) * 4 + (16 * (uint64_t)n + 4096) * 8 +
+           std::max<uint64_t>(4 * tiles * 256 * 4, std::max<uint64_t>(16 * (uint64_t)n, 8ull << 20)) +
+           kGhistWords * 4 + 4096;
 }
 
 int sph_world_create(const sph_params *params, uint32_t n, sph_world **out) {
@@ -661,7 +704,8 @@ int sph_world_set_kernel(sph_world *w, int variant) {
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
     w->variant = variant;
     w->tuned = variant != SPH_KERNEL_AUTO;
-    world_apply_code(w, variant == SPH_KERNEL_BRICK ? kCodeBrick : w->chain.lane_cfg);
+    world_apply_code(w, (variant == SPH_KERNEL_BRICK ? kCodeBrick : w->chain.lane_cfg) |
+                            (int)(w->chain.radix_bits == 8u ? 0u : w->chain.radix_bits << 8));
     return SPH_OK;
 }
 
@@ -689,7 +733,25 @@ int sph_world_set_lane_config(sph_world *w, int config) {
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
     w->variant = SPH_KERNEL_DIRECT;
     w->tuned = true;
-    world_apply_code(w, config);
+    world_apply_code(w, config | (int)(w->chain.radix_bits == 8u ? 0u : w->chain.radix_bits << 8));
+    return SPH_OK;
+}
+
+int sph_world_set_radix_bits(sph_world *w, int bits) {
+    if (!w) return fail(SPH_ERR_INVALID_ARG, "world is NULL");
+    if (bits != 8 && bits != 11 && bits != 12) return fail(SPH_ERR_INVALID_ARG, "radix digits of %d bits: use 8, 11 or 12", bits);
+    if (w->chain.lookback_words_for((uint32_t)bits) > w->chain.lookback_cap)
+        return fail(SPH_ERR_CAPACITY, "%d-bit radix digits need more look-back memory than this world holds", bits);
+    hipError_t e = hipStreamSynchronize(w->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    w->chain.set_radix_bits((uint32_t)bits);
+    w->graph.drop();
+    if (w->variant == SPH_KERNEL_AUTO) {  // keep the tuned kernels, pin this width
+        int rc = world_resolve_variant(w);
+        if (rc) return rc;
+        w->chain.set_radix_bits((uint32_t)bits);
+        w->variant = w->chain.bricks ? SPH_KERNEL_BRICK : SPH_KERNEL_DIRECT;
+    }
     return SPH_OK;
 }
 
@@ -702,12 +764,14 @@ int sph_world_kernel(const sph_world *w) {
 
 int sph_world_kernel_config(const sph_world *w, char *buf, size_t bytes) {
     if (!w || !buf || bytes == 0) return fail(SPH_ERR_INVALID_ARG, "NULL argument");
+    char radix[64];
+    snprintf(radix, sizeof radix, "%u-bit radix digits (%u passes)", w->chain.radix_bits, w->chain.passes);
     if (w->chain.bricks)
-        snprintf(buf, bytes, "brick %dx%dx%d cells, persistent queue", kBrick, kBrick, kBrick);
+        snprintf(buf, bytes, "brick %dx%dx%d cells, persistent queue, %s", kBrick, kBrick, kBrick, radix);
     else
-        snprintf(buf, bytes, "per-lane, %d candidates per scan step, %d-lane workgroups, %s",
+        snprintf(buf, bytes, "per-lane, %d candidates per scan step, %d-lane workgroups, %s, %s",
                  kLaneConfigs[w->chain.lane_cfg].unroll, kLaneConfigs[w->chain.lane_cfg].block,
-                 kLaneConfigs[w->chain.lane_cfg].lists ? "force neighbour lists" : "force cell walk");
+                 kLaneConfigs[w->chain.lane_cfg].lists ? "force neighbour lists" : "force cell walk", radix);
     return SPH_OK;
 }
 

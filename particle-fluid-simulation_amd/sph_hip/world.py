@@ -134,6 +134,10 @@ class World:
     def lane_configs() -> int:
         return int(load_library().sph_lane_configs())
 
+    def set_radix_bits(self, bits: int) -> None:
+        """Pins the radix digit width (8, 11 or 12 bits)."""
+        check(self._lib.sph_world_set_radix_bits(self._h, bits))
+
     def set_debug(self, on: bool) -> None:
         check(self._lib.sph_world_set_debug(self._h, 1 if on else 0))
 
