@@ -156,12 +156,16 @@ typedef enum sph_dump_stage {
  *   DIRECT: one lane per particle, runs read from global memory (the reference's scheme).
  *   BRICK:  workgroups own 4x4x4-cell bricks drained from a device work queue and read the
  *           6x6x6 halo's runs from LDS.
- *   AUTO:   the autotuner's choice (the default): before the first step of a size bucket
- *           (particle count, table size, brick count rounded to powers of two) each
- *           configuration (the brick kernels, and the per-lane kernels with 2 or 4
- *           candidates per scan step, 64-, 128- or 256-lane workgroups, with or without
- *           force neighbour lists recorded by the density pass) is timed on a
- *           saved copy of the world's state, and the fastest is cached for the process.
+ *   AUTO:   the autotuner's choice (the default): the first substeps a world of a new
+ *           size bucket (particle count, table size, brick count rounded to powers of two)
+ *           steps run each configuration in turn (the brick kernels, and the per-lane
+ *           kernels with 2 or 4 candidates per scan step, 64-, 128- or 256-lane workgroups,
+ *           with or without force neighbour lists recorded by the density pass), two
+ *           substeps each with the second timed, then the radix digit widths that need
+ *           fewer passes; every configuration gives bit-identical results, so these are the
+ *           world's real substeps (no state is saved, nothing is allocated).  The fastest
+ *           is then kept and cached for the process; choosing it waits once for the
+ *           substeps already queued.  sph_world_autotune measures on a saved copy instead.
  *           SPH_AUTOTUNE=0 in the environment disables the measurement (DIRECT is used);
  *           SPH_LANE_CONFIG=0..9 instead pins one per-lane configuration (for profiling:
  *           candidates per scan step x workgroup lanes 0 = 2x256, 1 = 4x256, 2 = 2x128,

@@ -394,6 +394,19 @@ struct sph_world {
     size_t map_cap = 0;             // floats allocated at d_map
     // the world's own state buffers while the caller's are bound (sph_world_bind_state)
     float4 *own_pos = nullptr, *own_vel = nullptr;
+    // Online autotuning of an AUTO world: its first substeps run the candidate
+    // configurations (two each, the second between events); all give bit-identical
+    // results, so these are the world's real substeps and nothing is saved or restored.
+    struct OnlineTune {
+        bool active = false;
+        int phase = 0;               // 0: kernel configurations, 1: radix widths
+        std::vector<int> codes;
+        std::vector<hipEvent_t> ev;  // [2 * code]: around the timed substep
+        size_t next = 0;             // next code to run
+        int sub = 0;                 // substeps run of the current code
+        int best = -1;
+        float best_ms = 0.0f;
+    } online;
 };
 
 namespace {
@@ -492,9 +505,103 @@ int world_autotune(sph_world *w, uint32_t substeps, int *chosen, double *ms_out)
     return SPH_OK;
 }
 
-// Resolves AUTO before the first step of a configuration.
-int world_resolve_variant(sph_world *w) {
+void online_release(sph_world *w) {
+    for (hipEvent_t e : w->online.ev)
+        if (e) (void)hipEventDestroy(e);
+    w->online.ev.clear();
+}
+
+// Ends online tuning without choosing (an explicit choice or measurement replaces it).
+void online_cancel(sph_world *w) {
+    if (!w->online.active) return;
+    (void)hipStreamSynchronize(w->stream);
+    online_release(w);
+    w->online.active = false;
+}
+
+int online_begin_phase(sph_world *w, std::vector<int> codes) {
+    online_release(w);
+    w->online.codes = std::move(codes);
+    w->online.ev.assign(2 * w->online.codes.size(), nullptr);
+    for (hipEvent_t &e : w->online.ev)
+        if (hipEventCreate(&e) != hipSuccess) return fail(SPH_ERR_DEVICE, "hipEventCreate");
+    w->online.next = 0;
+    w->online.sub = 0;
+    return SPH_OK;
+}
+
+int online_start(sph_world *w) {
+    std::vector<int> codes;
+    for (int c = 0; c < kNumLaneConfigs; ++c) codes.push_back(c);
+    if (w->chain.brick_total) codes.push_back(kCodeBrick);
+    w->online = sph_world::OnlineTune{};
+    w->online.active = true;
+    return online_begin_phase(w, std::move(codes));
+}
+
+// All codes of the phase ran: wait for them (work already queued), keep the fastest; after
+// the kernels, try the radix widths that need fewer passes with the fastest kernels.
+int online_finish_phase(sph_world *w) {
+    auto &o = w->online;
+    hipError_t e = hipEventSynchronize(o.ev.back());
+    if (e != hipSuccess) return hip_fail(e, "online autotune");
+    for (size_t i = 0; i < o.codes.size(); ++i) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, o.ev[2 * i], o.ev[2 * i + 1]) != hipSuccess) continue;
+        if (o.best < 0 || ms < o.best_ms) {
+            o.best = o.codes[i];
+            o.best_ms = ms;
+        }
+    }
+    if (o.best < 0) o.best = kDefaultLaneConfig;
+    if (o.phase == 0) {
+        std::vector<int> widths;
+        const int kbest = o.best & kCodeMask;
+        for (uint32_t width : {11u, 12u})
+            if (w->chain.lookback_words_for(width) <= w->chain.lookback_cap &&
+                (w->chain.key_bits + width - 1u) / width < (w->chain.key_bits + 7u) / 8u)
+                widths.push_back(kbest | (int)(width << 8));
+        if (!widths.empty()) {
+            o.phase = 1;
+            return online_begin_phase(w, std::move(widths));
+        }
+    }
+    online_release(w);
+    o.active = false;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tune_cache[tune_bucket(w->n, w->T, w->chain.brick_total)] = o.best;
+    }
+    world_apply_code(w, o.best);
+    w->tuned = true;
+    return SPH_OK;
+}
+
+// One substep of online tuning.
+hipError_t online_substep(sph_world *w) {
+    auto &o = w->online;
+    if (o.sub == 0) {
+        world_apply_code(w, o.codes[o.next]);
+        o.sub = 1;
+        return w->chain.enqueue(w->stream, w->scene, nullptr);
+    }
+    hipError_t e = hipEventRecord(o.ev[2 * o.next], w->stream);
+    if (e == hipSuccess) e = w->chain.enqueue(w->stream, w->scene, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(o.ev[2 * o.next + 1], w->stream);
+    o.sub = 0;
+    ++o.next;
+    return e;
+}
+
+// Resolves AUTO before the first step of a configuration: the tuning cache, a pinned
+// configuration, or a measurement (online over the coming substeps when `online` is set,
+// else now, on a saved copy of the state).
+int world_resolve_variant(sph_world *w, bool online = false) {
     if (w->tuned || w->variant != SPH_KERNEL_AUTO) return SPH_OK;
+    if (w->online.active) {
+        if (online) return SPH_OK;
+        online_cancel(w);
+    }
     const uint64_t key = tune_bucket(w->n, w->T, w->chain.brick_total);
     int code = -1;
     {
@@ -510,6 +617,8 @@ int world_resolve_variant(sph_world *w) {
             code = pinned;  // a fixed per-lane launch configuration (profiling)
         } else if (off && std::strcmp(off, "0") == 0) {
             code = kDefaultLaneConfig;
+        } else if (online) {
+            return online_start(w);
         } else {
             int rc = world_autotune(w, 3, &code, nullptr);
             if (rc) return rc;
@@ -617,16 +726,25 @@ int sph_world_step(sph_world *w, uint32_t frames) {
     if (!w) return fail(SPH_ERR_INVALID_ARG, "world is NULL");
     const uint64_t substeps = (uint64_t)frames * w->params.substeps;
     if (substeps == 0) return SPH_OK;
-    int rc0 = world_resolve_variant(w);
+    int rc0 = world_resolve_variant(w, true);
     if (rc0) return rc0;
-    if (!w->graph.exec) {
-        int rc = w->graph.ensure_begin(w->stream);
-        if (rc) return rc;
-        hipError_t le = w->chain.enqueue(w->stream, w->scene, nullptr);
-        rc = w->graph.end(w->stream, le);
-        if (rc) return rc;
-    }
     for (uint64_t s = 0; s < substeps; ++s) {
+        if (w->online.active && w->online.next == w->online.codes.size()) {
+            int rc = online_finish_phase(w);
+            if (rc) return rc;
+        }
+        if (w->online.active) {
+            hipError_t e = online_substep(w);
+            if (e != hipSuccess) return hip_fail(e, "autotuning substep");
+            continue;
+        }
+        if (!w->graph.exec) {
+            int rc = w->graph.ensure_begin(w->stream);
+            if (rc) return rc;
+            hipError_t le = w->chain.enqueue(w->stream, w->scene, nullptr);
+            rc = w->graph.end(w->stream, le);
+            if (rc) return rc;
+        }
         hipError_t e = hipGraphLaunch(w->graph.exec, w->stream);
         if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
     }
@@ -702,6 +820,7 @@ int sph_world_set_kernel(sph_world *w, int variant) {
         return fail(SPH_ERR_INVALID_ARG, "brick kernels need a grid of at most 2^28 bricks");
     hipError_t e = hipStreamSynchronize(w->stream);
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    online_cancel(w);
     w->variant = variant;
     w->tuned = variant != SPH_KERNEL_AUTO;
     world_apply_code(w, (variant == SPH_KERNEL_BRICK ? kCodeBrick : w->chain.lane_cfg) |
@@ -714,6 +833,7 @@ int sph_world_autotune(sph_world *w, uint32_t substeps, int *chosen, double *ms_
     if (substeps == 0) return fail(SPH_ERR_INVALID_ARG, "substeps must be >= 1");
     hipError_t e = hipStreamSynchronize(w->stream);
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    online_cancel(w);
     int code = 0;
     int rc = world_autotune(w, substeps, &code, ms_per_substep);
     if (rc) return rc;
@@ -731,6 +851,7 @@ int sph_world_set_lane_config(sph_world *w, int config) {
         return fail(SPH_ERR_INVALID_ARG, "lane configuration %d outside [0, %d)", config, kNumLaneConfigs);
     hipError_t e = hipStreamSynchronize(w->stream);
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    online_cancel(w);
     w->variant = SPH_KERNEL_DIRECT;
     w->tuned = true;
     world_apply_code(w, config | (int)(w->chain.radix_bits == 8u ? 0u : w->chain.radix_bits << 8));
@@ -929,6 +1050,7 @@ int sph_world_device_state(const sph_world *w, float **pos4_dev, float **vel4_de
 void sph_world_destroy(sph_world *w) {
     if (!w) return;
     if (w->stream) hipStreamSynchronize(w->stream);
+    online_release(w);
     if (w->own_pos) {  // never free the caller's buffers
         w->chain.b.pos = w->own_pos;
         w->chain.b.vel = w->own_vel;

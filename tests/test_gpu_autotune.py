@@ -37,3 +37,23 @@ def test_auto_world_steps_like_oracle(gpu, orc):
     rp, rv = orc.step(params, pos, vel, 3)
     assert np.array_equal(gp.view(np.uint32), rp.view(np.uint32))
     assert np.array_equal(gv.view(np.uint32), rv.view(np.uint32))
+
+
+def test_online_tuning_runs_real_substeps(gpu, orc):
+    """An AUTO world of a size bucket no other test uses tunes over its own first substeps
+    (each configuration twice, then the radix widths), then steps through the chosen
+    configuration's graph: 30 frames (90 substeps) bit-exact against the oracle."""
+    from sph_hip.workloads import DAM_BREAK_16K as W
+    params = W.params(table_cells=1 << 17)
+    pos, vel = W.state(12345)
+    with gpu.World(params, 12345) as w:
+        w.upload(pos, vel)
+        w.step(2)    # 6 substeps: still tuning
+        w.step(28)   # the rest of the candidates, then the graph of the fastest
+        gp, gv = w.download()
+        cfg = w.kernel_config
+    print("chosen:", cfg)
+    assert "radix digits" in cfg
+    rp, rv = orc.step(params, pos, vel, 30)
+    assert np.array_equal(gp.view(np.uint32), rp.view(np.uint32))
+    assert np.array_equal(gv.view(np.uint32), rv.view(np.uint32))
