@@ -189,6 +189,17 @@ int sph_world_kernel_config(const sph_world *w, char *buf, size_t bytes);
  * testing every configuration.  All configurations give bit-identical results. */
 int sph_world_set_lane_config(sph_world *w, int config);
 int sph_lane_configs(void);
+
+/* Precision of the neighbour passes.  EXACT (the default): every float operation is the
+ * reference's, correctly rounded in its order, and results are bit-identical to the FP32
+ * oracle.  TOLERANCE: the world may also use (and its autotuner may pick) launch
+ * configurations 10 and 11, which evaluate the pair terms with the hardware's approximate
+ * square root and reciprocal; cell keys, the sorted pairs and the table stay exact, and one
+ * substep's densities, forces, positions and velocities agree with the reference's
+ * equations within 1e-5 relative (north_star's stated tolerance).  Selecting a precision
+ * re-tunes an AUTO world; configurations 10 and 11 need TOLERANCE. */
+typedef enum sph_precision { SPH_PRECISION_EXACT = 0, SPH_PRECISION_TOLERANCE = 1 } sph_precision;
+int sph_world_set_precision(sph_world *w, int mode);
 /* Pins the radix sort's digit width (8, 11 or 12 bits; the autotuner otherwise chooses it
  * per size bucket among the widths that need fewer passes than 8-bit digits).  A world
  * holds look-back memory for the widths whose per-tile words stay within 16 B per particle

@@ -215,7 +215,7 @@ static int batch_autotune(sph_batch *b, uint32_t substeps) {
     // Then, with the fastest kernels, the radix widths that need fewer passes than 8-bit
     // digits (code = kernels | width << 8).
     std::vector<int> cands;
-    for (int c = 0; c < kNumLaneConfigs; ++c) cands.push_back(c);
+    for (int c = 0; c < kNumExactLaneConfigs; ++c) cands.push_back(c);
     if (b->b_total) cands.push_back(-1);
     b->chain.set_radix_bits(8u);
     const size_t n_kernel_cands = cands.size();

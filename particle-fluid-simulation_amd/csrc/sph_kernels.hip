@@ -177,7 +177,8 @@ __device__ __forceinline__ uint32_t xcd_block() {
 // lane) would take 79 and 130 VGPRs, one wave per SIMD below the single-scene ones; held
 // at 7 and 4 waves (3 spilled registers in the batch density pass) the batched step runs
 // ~1.5 % faster.
-template <class Scenes, int kU, int kB, bool kL>
+// kF: the tolerance path's approximate square root and reciprocal (pair_sqrt / pair_div).
+template <class Scenes, int kU, int kB, bool kL, bool kF>
 __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(std::is_same_v<Scenes, OneScene> ? 8 : 7)))
 void k_density(Scenes sc, const ChainBuffers b, uint32_t n) {
     __shared__ uint32_t s_list[kL ? kListLds * kB : 1];
@@ -189,11 +190,11 @@ void k_density(Scenes sc, const ChainBuffers b, uint32_t n) {
         const DensityCellsOf<GridRuns> cells{grid_runs(c, b), sample_axis(s.x, c.minx, c.maxx, c.R),
                                              sample_axis(s.y, c.miny, c.maxy, c.R),
                                              sample_axis(s.z, c.minz, c.maxz, c.R)};
-        b.dens[j] = density_walk_rows<kU>(c, s, b.pred_pos, cells);
+        b.dens[j] = density_walk_rows<kU, kF>(c, s, b.pred_pos, cells);
         return;
     }
     uint32_t count;
-    const float2 d = density_with_list<kU>(c, j, s, b.pred_pos, grid_runs(c, b), s_list + threadIdx.x, kB,
+    const float2 d = density_with_list<kU, kF>(c, j, s, b.pred_pos, grid_runs(c, b), s_list + threadIdx.x, kB,
                                            b.nlist, n, count);
     b.dens[j] = d;
     // The densities also ride in the w lanes of the particle's predicted velocity (density)
@@ -213,7 +214,7 @@ void k_density(Scenes sc, const ChainBuffers b, uint32_t n) {
 // updateVelocities.comp:71-119.  Neighbour velocities come from vel_sorted, which no lane
 // writes (the reference races on that buffer; here every read sees the pre-force value).
 // Works in cell order; the new state of cell-ordered slot j goes to hash slot hslot[j].
-template <class Scenes, int kU, int kB, bool kL>
+template <class Scenes, int kU, int kB, bool kL, bool kF>
 __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4))) void k_force(Scenes sc, const ChainBuffers b,
                                                                                         uint32_t n) {
     __shared__ uint32_t s_queue[kQueue * kB];
@@ -229,10 +230,10 @@ __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const uint32_t cnt = kL ? b.ncount[j] : kListInvalid;
     if (cnt != kListInvalid) {
         // the recorded neighbours, in the walk's order
-        force_list<8>(c, me, cnt, b.nlist, n, b.pred_pos, b.vel_sorted, Fp, Fv);
+        force_list<8, kF>(c, me, cnt, b.nlist, n, b.pred_pos, b.vel_sorted, Fp, Fv);
     } else {
         const ForceCellsOf<GridRuns> cells{grid_runs(c, b), force_cell(c, s)};
-        force_walk_rows<kU>(c, me, b.pred_pos, b.vel_sorted, b.dens, cells, s_queue + threadIdx.x, kB, Fp, Fv);
+        force_walk_rows<kU, kF>(c, me, b.pred_pos, b.vel_sorted, b.dens, cells, s_queue + threadIdx.x, kB, Fp, Fv);
     }
     float4 po, vo, vf;
     force_finish(c, P, V, myd.x, Fp, Fv, po, vo, vf);
@@ -289,50 +290,59 @@ hipError_t launch_reorder(hipStream_t stream, const Scenes &sc, const ChainBuffe
 }
 
 const LaneConfig kLaneConfigs[kNumLaneConfigs] = {
-    {2, 256, true}, {4, 256, true}, {2, 128, true}, {4, 128, true}, {2, 64, true},
-    {2, 256, false}, {4, 256, false}, {2, 128, false}, {4, 128, false}, {2, 64, false},
+    {2, 256, true, false}, {4, 256, true, false}, {2, 128, true, false}, {4, 128, true, false},
+    {2, 64, true, false}, {2, 256, false, false}, {4, 256, false, false}, {2, 128, false, false},
+    {4, 128, false, false}, {2, 64, false, false},
+    {4, 128, true, true}, {4, 128, false, true},  // the tolerance path's variants of 3 and 8
 };
 
-template <class Scenes, int kU, int kB, bool kL>
+template <class Scenes, int kU, int kB, bool kL, bool kF>
 hipError_t launch_density_t(hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n) {
-    hipLaunchKernelGGL((k_density<Scenes, kU, kB, kL>), dim3((n + kB - 1u) / kB), dim3(kB), 0, stream, sc, b, n);
+    hipLaunchKernelGGL((k_density<Scenes, kU, kB, kL, kF>), dim3((n + kB - 1u) / kB), dim3(kB), 0, stream, sc, b, n);
     return hipGetLastError();
 }
 
-template <class Scenes, int kU, int kB, bool kL>
+template <class Scenes, int kU, int kB, bool kL, bool kF>
 hipError_t launch_force_t(hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n) {
-    hipLaunchKernelGGL((k_force<Scenes, kU, kB, kL>), dim3((n + kB - 1u) / kB), dim3(kB), 0, stream, sc, b, n);
+    hipLaunchKernelGGL((k_force<Scenes, kU, kB, kL, kF>), dim3((n + kB - 1u) / kB), dim3(kB), 0, stream, sc, b, n);
     return hipGetLastError();
 }
 
-// Dispatches configuration `cfg` (an index into kLaneConfigs) to its instantiation.
-template <template <class, int, int, bool> class Launch, class Scenes>
+// Dispatches configuration `cfg` (an index into kLaneConfigs) to its instantiation.  The
+// tolerance variants exist for single scenes only (batches and served scenes stay exact).
+template <template <class, int, int, bool, bool> class Launch, class Scenes>
 hipError_t launch_cfg(int cfg, hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n) {
     switch (cfg) {
-        case 0: return Launch<Scenes, 2, 256, true>::run(stream, sc, b, n);
-        case 1: return Launch<Scenes, 4, 256, true>::run(stream, sc, b, n);
-        case 2: return Launch<Scenes, 2, 128, true>::run(stream, sc, b, n);
-        case 4: return Launch<Scenes, 2, 64, true>::run(stream, sc, b, n);
-        case 5: return Launch<Scenes, 2, 256, false>::run(stream, sc, b, n);
-        case 6: return Launch<Scenes, 4, 256, false>::run(stream, sc, b, n);
-        case 7: return Launch<Scenes, 2, 128, false>::run(stream, sc, b, n);
-        case 8: return Launch<Scenes, 4, 128, false>::run(stream, sc, b, n);
-        case 9: return Launch<Scenes, 2, 64, false>::run(stream, sc, b, n);
-        default: return Launch<Scenes, 4, 128, true>::run(stream, sc, b, n);
+        case 0: return Launch<Scenes, 2, 256, true, false>::run(stream, sc, b, n);
+        case 1: return Launch<Scenes, 4, 256, true, false>::run(stream, sc, b, n);
+        case 2: return Launch<Scenes, 2, 128, true, false>::run(stream, sc, b, n);
+        case 4: return Launch<Scenes, 2, 64, true, false>::run(stream, sc, b, n);
+        case 5: return Launch<Scenes, 2, 256, false, false>::run(stream, sc, b, n);
+        case 6: return Launch<Scenes, 4, 256, false, false>::run(stream, sc, b, n);
+        case 7: return Launch<Scenes, 2, 128, false, false>::run(stream, sc, b, n);
+        case 8: return Launch<Scenes, 4, 128, false, false>::run(stream, sc, b, n);
+        case 9: return Launch<Scenes, 2, 64, false, false>::run(stream, sc, b, n);
+        case 10:
+            if constexpr (std::is_same_v<Scenes, OneScene>) return Launch<Scenes, 4, 128, true, true>::run(stream, sc, b, n);
+            else return Launch<Scenes, 4, 128, true, false>::run(stream, sc, b, n);
+        case 11:
+            if constexpr (std::is_same_v<Scenes, OneScene>) return Launch<Scenes, 4, 128, false, true>::run(stream, sc, b, n);
+            else return Launch<Scenes, 4, 128, false, false>::run(stream, sc, b, n);
+        default: return Launch<Scenes, 4, 128, true, false>::run(stream, sc, b, n);
     }
 }
 
-template <class Scenes, int kU, int kB, bool kL>
+template <class Scenes, int kU, int kB, bool kL, bool kF>
 struct DensityLaunch {
     static hipError_t run(hipStream_t st, const Scenes &sc, const ChainBuffers &b, uint32_t n) {
-        return launch_density_t<Scenes, kU, kB, kL>(st, sc, b, n);
+        return launch_density_t<Scenes, kU, kB, kL, kF>(st, sc, b, n);
     }
 };
 
-template <class Scenes, int kU, int kB, bool kL>
+template <class Scenes, int kU, int kB, bool kL, bool kF>
 struct ForceLaunch {
     static hipError_t run(hipStream_t st, const Scenes &sc, const ChainBuffers &b, uint32_t n) {
-        return launch_force_t<Scenes, kU, kB, kL>(st, sc, b, n);
+        return launch_force_t<Scenes, kU, kB, kL, kF>(st, sc, b, n);
     }
 };
 

@@ -102,8 +102,10 @@ struct LaneConfig {
     int unroll;
     int block;
     bool lists;
+    bool tolerance;  // approximate square root / reciprocal: not bit-exact (opt-in per world)
 };
-constexpr int kNumLaneConfigs = 10;
+constexpr int kNumLaneConfigs = 12;
+constexpr int kNumExactLaneConfigs = 10;  // configurations 0..9 are bit-exact
 constexpr int kDefaultLaneConfig = 3;
 extern const LaneConfig kLaneConfigs[kNumLaneConfigs];
 

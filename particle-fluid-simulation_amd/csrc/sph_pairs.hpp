@@ -20,6 +20,22 @@ __device__ __forceinline__ float2 density_finish(const SceneConst &c, float M, f
     return make_float2(fmaxf(0.01f, density), fmaxf(0.01f, near_density));
 }
 
+// Square root and division of the neighbour terms.  The exact path (kFast = false) uses the
+// correctly rounded operations the reference's float expressions specify; the tolerance
+// path uses the hardware's approximate square root and reciprocal (about 1 ulp each), a
+// multiply in place of each division: results then agree with the reference to ~1e-6
+// relative instead of bit for bit.
+template <bool kFast>
+__device__ __forceinline__ float pair_sqrt(float x) {
+    if constexpr (kFast) return __builtin_amdgcn_sqrtf(x);
+    else return sqrtf(x);
+}
+template <bool kFast>
+__device__ __forceinline__ float pair_div(float a, float b) {
+    if constexpr (kFast) return a * __builtin_amdgcn_rcpf(b);
+    else return a / b;
+}
+
 struct ForceSelf {
     float4 s;            // predictedPositions[j]
     float3 u;            // predictedVelocities[j]
@@ -195,17 +211,19 @@ using ForceCells = ForceCellsOf<CellRuns<kHaloLds>>;
 // implies sqrt(d2) >= R: both kernels are 0 and M + m*0 == M for M >= +0, so such a
 // candidate changes nothing and its sqrt is skipped.
 // The terms of a candidate that passed the d2 <= R2_pre pre-filter.
+template <bool kFast = false>
 __device__ __forceinline__ void density_terms(const SceneConst &c, float d2, float &M, float &Mn) {
-    const float dist = sqrtf(d2);
+    const float dist = pair_sqrt<kFast>(d2);
     if (dist >= c.R) return;
     const float t = c.R - dist;
     M = M + c.m * (t * t);
     Mn = Mn + c.m * (t * t * t);
 }
 
+template <bool kFast = false>
 __device__ __forceinline__ void density_accumulate(const SceneConst &c, float d2, float &M, float &Mn) {
     if (d2 > c.R2_pre) return;
-    const float dist = sqrtf(d2);
+    const float dist = pair_sqrt<kFast>(d2);
     if (dist >= c.R) return;
     const float t = c.R - dist;
     M = M + c.m * (t * t);
@@ -256,28 +274,29 @@ __device__ __forceinline__ float2 density_walk(const SceneConst &c, const float4
 
 // Full arithmetic of one force candidate (applyPressureAndViscosityForce.comp:303-321) at
 // predicted position pi with velocity vi and (density, near density) di.
+template <bool kFast = false>
 __device__ __forceinline__ void force_pair(const SceneConst &c, const ForceSelf &me, const float3 &pi,
                                            const float3 &vi, const float2 &di, float3 &Fp, float3 &Fv) {
     const float ex = pi.x - me.s.x, ey = pi.y - me.s.y, ez = pi.z - me.s.z;
     const float d2 = ex * ex + ey * ey + ez * ez;
-    const float dist = sqrtf(d2);
+    const float dist = pair_sqrt<kFast>(d2);
     if (fabsf(dist) < 1e-7f) return;
-    const float dirx = ex / dist, diry = ey / dist, dirz = ez / dist;
+    const float dirx = pair_div<kFast>(ex, dist), diry = pair_div<kFast>(ey, dist), dirz = pair_div<kFast>(ez, dist);
     const float slope = dist >= c.R ? 0.0f : (dist - c.R) * c.slope_scale;
     const float shared = ((di.x - c.rho0) * c.k + me.my_pressure) / 2.0f;
-    const float a = shared * slope * c.m / di.x;
+    const float a = pair_div<kFast>(shared * slope * c.m, di.x);
     Fp.x = Fp.x + a * dirx;
     Fp.y = Fp.y + a * diry;
     Fp.z = Fp.z + a * dirz;
     const float infl = dist >= c.R ? 0.0f : (c.R - dist) * (c.R - dist);
-    const float b = c.m * infl / di.x;
+    const float b = pair_div<kFast>(c.m * infl, di.x);
     Fv.x = Fv.x + b * (vi.x - me.u.x);
     Fv.y = Fv.y + b * (vi.y - me.u.y);
     Fv.z = Fv.z + b * (vi.z - me.u.z);
     const float shared_near = (me.near_pressure + di.y * c.kn) / 2.0f;
     const float x = c.R - dist;
     const float nslope = dist >= c.R ? 0.0f : x * x * c.near_slope_scale;
-    const float cn = shared_near * nslope * c.m / di.y;
+    const float cn = pair_div<kFast>(shared_near * nslope * c.m, di.y);
     Fp.x = Fp.x + cn * dirx;
     Fp.y = Fp.y + cn * diry;
     Fp.z = Fp.z + cn * dirz;
@@ -286,7 +305,7 @@ __device__ __forceinline__ void force_pair(const SceneConst &c, const ForceSelf 
 // One force candidate by queue entry e: a global slot (kGlobalBit) or, on the brick path,
 // an LDS entry packed as s_a = (predicted position xyz, near density) and s_b = (velocity
 // xyz, density).
-template <bool kLds>
+template <bool kLds, bool kFast = false>
 __device__ __forceinline__ void force_term(const SceneConst &c, const ForceSelf &me, uint32_t e,
                                            const float4 *__restrict__ pred_pos,
                                            const float4 *__restrict__ vel_sorted,
@@ -295,10 +314,10 @@ __device__ __forceinline__ void force_term(const SceneConst &c, const ForceSelf 
     if (!kLds || (e & kGlobalBit)) {
         const uint32_t g = e & ~kGlobalBit;
         const float4 a = pred_pos[g], b = vel_sorted[g];
-        force_pair(c, me, make_float3(a.x, a.y, a.z), make_float3(b.x, b.y, b.z), dens[g], Fp, Fv);
+        force_pair<kFast>(c, me, make_float3(a.x, a.y, a.z), make_float3(b.x, b.y, b.z), dens[g], Fp, Fv);
     } else {
         const float4 a = s_a[e], b = s_b[e];
-        force_pair(c, me, make_float3(a.x, a.y, a.z), make_float3(b.x, b.y, b.z), make_float2(b.w, a.w), Fp, Fv);
+        force_pair<kFast>(c, me, make_float3(a.x, a.y, a.z), make_float3(b.x, b.y, b.z), make_float2(b.w, a.w), Fp, Fv);
     }
 }
 
@@ -307,7 +326,7 @@ __device__ __forceinline__ void force_term(const SceneConst &c, const ForceSelf 
 // pass waits on these dependent loads, not on the arithmetic).  The density pass that
 // recorded the list left each particle's density in vel_sorted[].w and its near density
 // in pred_pos[].w, so a neighbour is two 16-byte loads.
-template <int kU>
+template <int kU, bool kFast = false>
 __device__ __forceinline__ void force_list(const SceneConst &c, const ForceSelf &me, uint32_t cnt,
                                            const uint32_t *__restrict__ list, uint32_t n,
                                            const float4 *__restrict__ pred_pos,
@@ -325,7 +344,7 @@ __device__ __forceinline__ void force_list(const SceneConst &c, const ForceSelf 
 #pragma unroll
         for (int u = 0; u < kU; ++u)
             if (i + (uint32_t)u < cnt)
-                force_pair(c, me, make_float3(a[u].x, a[u].y, a[u].z), make_float3(v[u].x, v[u].y, v[u].z),
+                force_pair<kFast>(c, me, make_float3(a[u].x, a[u].y, a[u].z), make_float3(v[u].x, v[u].y, v[u].z),
                            make_float2(v[u].w, a[u].w), Fp, Fv);
     }
 }
@@ -400,7 +419,7 @@ __device__ __forceinline__ void load_candidates(const float4 *src, int k, int cn
         p[u] = (u == 0 || k + u < cnt) ? load_xyz(src + k + u) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
-template <int kU, class Cells>
+template <int kU, bool kFast = false, class Cells>
 __device__ __forceinline__ float2 density_walk_rows(const SceneConst &c, const float4 &s,
                                                     const float4 *__restrict__ pred_pos,
                                                     const Cells &cells) {
@@ -418,14 +437,14 @@ __device__ __forceinline__ float2 density_walk_rows(const SceneConst &c, const f
                     load_candidates<kU>(src, k, cnt, p);
 #pragma unroll
                     for (int u = 0; u < kU; ++u)
-                        if (k + u < cnt) density_accumulate(c, dist2(p[u], s), M, Mn);
+                        if (k + u < cnt) density_accumulate<kFast>(c, dist2(p[u], s), M, Mn);
                 }
             }
         }
     return density_finish(c, M, Mn);
 }
 
-template <int kU, class Cells>
+template <int kU, bool kFast = false, class Cells>
 __device__ __forceinline__ void force_walk_rows(const SceneConst &c, const ForceSelf &me,
                                                 const float4 *__restrict__ pred_pos,
                                                 const float4 *__restrict__ vel_sorted,
@@ -434,7 +453,7 @@ __device__ __forceinline__ void force_walk_rows(const SceneConst &c, const Force
     int qc = 0;
     auto drain = [&]() {
         for (int i = 0; i < qc; ++i)
-            force_term<false>(c, me, q[i * stride], pred_pos, vel_sorted, dens, nullptr, nullptr, Fp, Fv);
+            force_term<false, kFast>(c, me, q[i * stride], pred_pos, vel_sorted, dens, nullptr, nullptr, Fp, Fv);
         qc = 0;
     };
     for (int dz = 0; dz < 3; ++dz)
@@ -522,7 +541,7 @@ __device__ __forceinline__ void force_finish(const SceneConst &c, const float4 &
 // own walk and an invalid count, and the force pass walks its cells.
 constexpr int kListLds = 20;   // entries staged in LDS (the rest are stored directly)
 
-template <int kU>
+template <int kU, bool kFast = false>
 __device__ __forceinline__ float2 density_with_list(const SceneConst &c, uint32_t j, const float4 &s,
                                                     const float4 *__restrict__ pred_pos,
                                                     const GridRuns &runs, uint32_t *lane_list,
@@ -538,7 +557,7 @@ __device__ __forceinline__ float2 density_with_list(const SceneConst &c, uint32_
     if (!(same(ax) && same(ay) && same(az) && (int32_t)ax.mid == c0.x && (int32_t)ay.mid == c0.y &&
           (int32_t)az.mid == c0.z)) {
         count = kListInvalid;
-        return density_walk_rows<kU>(c, s, pred_pos, DensityCellsOf<GridRuns>{runs, ax, ay, az});
+        return density_walk_rows<kU, kFast>(c, s, pred_pos, DensityCellsOf<GridRuns>{runs, ax, ay, az});
     }
     const float R2 = c.R2;
     float M = 0.0f, Mn = 0.0f;
@@ -564,7 +583,7 @@ __device__ __forceinline__ float2 density_with_list(const SceneConst &c, uint32_
                         const float d2 = dist2(p[u], s);
                         // one branch for both uses: R2 <= R2_pre, and most candidates fail
                         if (!(d2 > c.R2_pre)) {
-                            if (member) density_terms(c, d2, M, Mn);
+                            if (member) density_terms<kFast>(c, d2, M, Mn);
                             const uint32_t slot = (uint32_t)(r[dx].base + k + u);
                             if (slot != j && !(d2 > R2)) {
                                 if (nl < (uint32_t)kListLds) lane_list[nl * stride] = slot;

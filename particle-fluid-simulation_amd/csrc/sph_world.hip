@@ -394,6 +394,8 @@ struct sph_world {
     size_t map_cap = 0;             // floats allocated at d_map
     // the world's own state buffers while the caller's are bound (sph_world_bind_state)
     float4 *own_pos = nullptr, *own_vel = nullptr;
+    bool tolerance = false;         // SPH_PRECISION_TOLERANCE: the tuner may pick the
+                                    // approximate-math configurations
     // Online autotuning of an AUTO world: its first substeps run the candidate
     // configurations (two each, the second between events); all give bit-identical
     // results, so these are the world's real substeps and nothing is saved or restored.
@@ -418,10 +420,13 @@ namespace {
 std::mutex g_tune_mu;
 std::map<uint64_t, int> g_tune_cache;
 
-uint64_t tune_bucket(uint32_t n, uint32_t T, uint64_t bricks) {
+uint64_t tune_bucket(uint32_t n, uint32_t T, uint64_t bricks, bool tolerance) {
     return ((uint64_t)key_bits_for(n) << 48) | ((uint64_t)key_bits_for(T) << 32) |
-           (uint64_t)key_bits_for(bricks ? bricks : 1);
+           ((uint64_t)tolerance << 31) | (uint64_t)key_bits_for(bricks ? bricks : 1);
 }
+
+// Launch configurations the world may use at its precision.
+int lane_configs_for(const sph_world *w) { return w->tolerance ? kNumLaneConfigs : kNumExactLaneConfigs; }
 
 // A tuning code names one configuration: 0..kNumLaneConfigs-1 = the per-lane kernels with
 // that launch configuration, kCodeBrick = the brick kernels; bits 8 and up carry the radix
@@ -453,7 +458,7 @@ int world_autotune(sph_world *w, uint32_t substeps, int *chosen, double *ms_out)
         return hip_fail(e, "autotune state copy");
     }
     std::vector<int> cands;
-    for (int c = 0; c < kNumLaneConfigs; ++c) cands.push_back(c);
+    for (int c = 0; c < lane_configs_for(w); ++c) cands.push_back(c);
     if (w->chain.brick_total) cands.push_back(kCodeBrick);
     int best = kDefaultLaneConfig;
     double best_ms = 1e30;
@@ -532,7 +537,7 @@ int online_begin_phase(sph_world *w, std::vector<int> codes) {
 
 int online_start(sph_world *w) {
     std::vector<int> codes;
-    for (int c = 0; c < kNumLaneConfigs; ++c) codes.push_back(c);
+    for (int c = 0; c < lane_configs_for(w); ++c) codes.push_back(c);
     if (w->chain.brick_total) codes.push_back(kCodeBrick);
     w->online = sph_world::OnlineTune{};
     w->online.active = true;
@@ -570,7 +575,7 @@ int online_finish_phase(sph_world *w) {
     o.active = false;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        g_tune_cache[tune_bucket(w->n, w->T, w->chain.brick_total)] = o.best;
+        g_tune_cache[tune_bucket(w->n, w->T, w->chain.brick_total, w->tolerance)] = o.best;
     }
     world_apply_code(w, o.best);
     w->tuned = true;
@@ -602,7 +607,7 @@ int world_resolve_variant(sph_world *w, bool online = false) {
         if (online) return SPH_OK;
         online_cancel(w);
     }
-    const uint64_t key = tune_bucket(w->n, w->T, w->chain.brick_total);
+    const uint64_t key = tune_bucket(w->n, w->T, w->chain.brick_total, w->tolerance);
     int code = -1;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -613,7 +618,7 @@ int world_resolve_variant(sph_world *w, bool online = false) {
         const char *off = std::getenv("SPH_AUTOTUNE");
         const char *pin = std::getenv("SPH_LANE_CONFIG");
         const int pinned = pin ? std::atoi(pin) : -1;
-        if (pin && *pin >= '0' && *pin <= '9' && pinned >= 0 && pinned < kNumLaneConfigs) {
+        if (pin && *pin >= '0' && *pin <= '9' && pinned >= 0 && pinned < lane_configs_for(w)) {
             code = pinned;  // a fixed per-lane launch configuration (profiling)
         } else if (off && std::strcmp(off, "0") == 0) {
             code = kDefaultLaneConfig;
@@ -838,7 +843,7 @@ int sph_world_autotune(sph_world *w, uint32_t substeps, int *chosen, double *ms_
     int rc = world_autotune(w, substeps, &code, ms_per_substep);
     if (rc) return rc;
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    g_tune_cache[tune_bucket(w->n, w->T, w->chain.brick_total)] = code;
+    g_tune_cache[tune_bucket(w->n, w->T, w->chain.brick_total, w->tolerance)] = code;
     w->variant = SPH_KERNEL_AUTO;
     w->tuned = true;
     *chosen = code == kCodeBrick ? SPH_KERNEL_BRICK : SPH_KERNEL_DIRECT;
@@ -849,6 +854,8 @@ int sph_world_set_lane_config(sph_world *w, int config) {
     if (!w) return fail(SPH_ERR_INVALID_ARG, "world is NULL");
     if (config < 0 || config >= kNumLaneConfigs)
         return fail(SPH_ERR_INVALID_ARG, "lane configuration %d outside [0, %d)", config, kNumLaneConfigs);
+    if (kLaneConfigs[config].tolerance && !w->tolerance)
+        return fail(SPH_ERR_STATE, "lane configuration %d is not bit-exact: select SPH_PRECISION_TOLERANCE first", config);
     hipError_t e = hipStreamSynchronize(w->stream);
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
     online_cancel(w);
@@ -878,6 +885,22 @@ int sph_world_set_radix_bits(sph_world *w, int bits) {
 
 int sph_lane_configs(void) { return kNumLaneConfigs; }
 
+int sph_world_set_precision(sph_world *w, int mode) {
+    if (!w) return fail(SPH_ERR_INVALID_ARG, "world is NULL");
+    if (mode != SPH_PRECISION_EXACT && mode != SPH_PRECISION_TOLERANCE)
+        return fail(SPH_ERR_INVALID_ARG, "unknown precision mode %d", mode);
+    hipError_t e = hipStreamSynchronize(w->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    online_cancel(w);
+    w->tolerance = mode == SPH_PRECISION_TOLERANCE;
+    if (!w->tolerance && kLaneConfigs[w->chain.lane_cfg].tolerance) {
+        world_apply_code(w, kDefaultLaneConfig);
+        w->variant = SPH_KERNEL_AUTO;
+    }
+    if (w->variant == SPH_KERNEL_AUTO) w->tuned = false;  // tune again for this precision
+    return SPH_OK;
+}
+
 int sph_world_kernel(const sph_world *w) {
     if (!w) return fail(SPH_ERR_INVALID_ARG, "world is NULL");
     return w->chain.bricks ? SPH_KERNEL_BRICK : SPH_KERNEL_DIRECT;
@@ -890,9 +913,10 @@ int sph_world_kernel_config(const sph_world *w, char *buf, size_t bytes) {
     if (w->chain.bricks)
         snprintf(buf, bytes, "brick %dx%dx%d cells, persistent queue, %s", kBrick, kBrick, kBrick, radix);
     else
-        snprintf(buf, bytes, "per-lane, %d candidates per scan step, %d-lane workgroups, %s, %s",
+        snprintf(buf, bytes, "per-lane, %d candidates per scan step, %d-lane workgroups, %s, %s%s",
                  kLaneConfigs[w->chain.lane_cfg].unroll, kLaneConfigs[w->chain.lane_cfg].block,
-                 kLaneConfigs[w->chain.lane_cfg].lists ? "force neighbour lists" : "force cell walk", radix);
+                 kLaneConfigs[w->chain.lane_cfg].lists ? "force neighbour lists" : "force cell walk", radix,
+                 kLaneConfigs[w->chain.lane_cfg].tolerance ? ", approximate square root and reciprocal" : "");
     return SPH_OK;
 }
 

@@ -97,6 +97,7 @@ def load_library() -> C.CDLL:
         "sph_world_kernel_config": (C.c_int, [vp, C.c_char_p, C.c_size_t]),
         "sph_world_set_lane_config": (C.c_int, [vp, C.c_int]),
         "sph_lane_configs": (C.c_int, []),
+        "sph_world_set_precision": (C.c_int, [vp, C.c_int]),
         "sph_world_set_radix_bits": (C.c_int, [vp, C.c_int]),
         "sph_world_debug_dump": (C.c_int, [vp, C.c_int, vp, C.c_size_t]),
         "sph_world_pass_times": (C.c_int, [vp, C.c_uint32, P(PassTimes)]),

@@ -134,6 +134,11 @@ class World:
     def lane_configs() -> int:
         return int(load_library().sph_lane_configs())
 
+    def set_precision(self, mode: str) -> None:
+        """"exact" (bit-identical to the reference's float order, the default) or
+        "tolerance" (approximate square root / reciprocal allowed, 1e-5 relative)."""
+        check(self._lib.sph_world_set_precision(self._h, {"exact": 0, "tolerance": 1}[mode]))
+
     def set_radix_bits(self, bits: int) -> None:
         """Pins the radix digit width (8, 11 or 12 bits)."""
         check(self._lib.sph_world_set_radix_bits(self._h, bits))

@@ -130,7 +130,7 @@ def test_server_rejects_bad_config_without_a_device():
 def test_lane_configs_without_a_device():
     import sph_hip
     lib = sph_hip.load_library()
-    assert lib.sph_lane_configs() == 10
+    assert lib.sph_lane_configs() == 12
     assert lib.sph_world_set_lane_config(None, 0) == -1
 
 

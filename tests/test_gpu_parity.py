@@ -210,7 +210,7 @@ def test_dam_break_every_lane_config(gpu, orc, config):
     """Every per-lane launch configuration (scan unroll x workgroup size, with and without
     force neighbour lists) gives the oracle's trajectory bit for bit over 3 frames."""
     from sph_hip.workloads import DAM_BREAK_16K as W
-    assert gpu.World.lane_configs() == 10
+    assert gpu.World.lane_configs() == 12
     params = W.params()
     pos, vel = W.state()
     with gpu.World(params, W.n) as w:
