@@ -173,14 +173,20 @@ KERNEL_BYTES_PER_PARTICLE = {"density": 24 + 8, "force": 24 + 8 + 32}
 KERNEL_NAMES = {"density": "k_density (calculateDensities + force neighbour lists)",
                 "force": "k_force (applyPressureAndViscosityForce + updateVelocities)"}
 
-# HBM bytes per launch of the neighbour kernels on uniform_1m (T = 2^21) after the default
-# 5 warmup + 100 timed steps (the fill settles and grows denser as it runs, so the figure
-# holds for that point only), measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE over
-# the launches that follow the timed steps (profiles/pmc_uniform_1m_neighbour_passes.md),
-# each counter scaled by what it reads for device-to-device copies of known size in the
-# same run (FETCH x 2, WRITE x 1).
-TRAFFIC_BYTES_PER_LAUNCH = {"density": (2 * 66545 + 194955) * 1024, "force": (2 * 150344 + 58812) * 1024}
-TRAFFIC_STEPS = (5, 100)  # (warmup, steps) the traffic figures were measured after
+# HBM bytes per launch of the neighbour kernels on uniform_1m (T = 2^21), keyed by the
+# (warmup, timed steps) of the run they were measured after: the fill settles and grows
+# denser as it runs, so each figure holds for that point of the trajectory only.  Measured
+# with rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, on the 30 launches the bench times its
+# dominant kernel over (those right after the timed steps), each counter scaled by what it
+# reads for device-to-device copies of known size in the same run (FETCH x 2, WRITE x 1).
+# A run at any other (warmup, steps) reports no traffic.
+TRAFFIC_PROFILE = "profiles/pmc_uniform_1m_traffic.md"
+TRAFFIC_KERNELS = ("per-lane, 4 candidates per scan step, 128-lane workgroups, force neighbour lists, "
+                   "8-bit radix digits (3 passes)")
+TRAFFIC_BYTES_PER_LAUNCH = {
+    (5, 20): {"density": (2 * 63752 + 118741) * 1024, "force": (2 * 128635 + 58841) * 1024},
+    (5, 100): {"density": (2 * 67085 + 175438) * 1024, "force": (2 * 148229 + 58844) * 1024},
+}
 
 
 def cpu_baseline(workload, table_cells: int, budget_s: float = 12.0) -> dict:
@@ -392,6 +398,8 @@ def main() -> int:
                     help="particles of a scene kept resident beside the request stream "
                          "(33554432 = the mixed-serving-at-scale case)")
     ap.add_argument("--serve-budget-gb", type=float, default=8.0)
+    ap.add_argument("--no-tolerance-leg", dest="tolerance_leg", action="store_false",
+                    help="skip timing the tolerance path (approximate square root / reciprocal)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (scenes shard "
@@ -447,6 +455,11 @@ def main() -> int:
     dom_s = pt[f"{dom}_ms"] * 1e-3
     dom_bytes = W.n * KERNEL_BYTES_PER_PARTICLE[dom]
     dom_gbs = dom_bytes / dom_s / 1e9
+    # measured HBM bytes hold for the kernel they were counted on (lane configuration 3,
+    # which the tuner picks on this workload) at the same point of the same trajectory
+    traffic = None
+    if W is wl.UNIFORM_1M and T == 1 << 21 and chosen == TRAFFIC_KERNELS:
+        traffic = TRAFFIC_BYTES_PER_LAUNCH.get((args.warmup, args.steps), {}).get(dom)
     # a 64 MiB device-to-device copy: the calibration the HBM byte counters are scaled by
     calib_src = torch.ones(16 << 20, dtype=torch.float32, device="cuda")
     torch.empty_like(calib_src).copy_(calib_src)
@@ -457,6 +470,32 @@ def main() -> int:
     copy_gbs = copy_bandwidth_gbs()
     work = flops_per_particle_substep(W)
     tflops = W.n * work["flops"] / substep_s / 1e12
+
+    # The same timed steps through the tolerance path (sph_world_set_precision: approximate
+    # square root and reciprocal in the neighbour terms, 1e-5 relative), from the same
+    # start, tuned after the same warmup; never the bench's value.
+    tol = None
+    if args.tolerance_leg:
+        wt = sph_hip.World(params, W.n)
+        wt.set_precision("tolerance")
+        wt.upload(pos, vel)
+        wt.step(max(0, args.warmup - 1))
+        wt.sync()
+        wt.autotune(5)
+        if args.warmup > 0:
+            wt.step(1)
+        wt.sync()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        wt.step(args.steps)
+        wt.sync()
+        t_tol = time.perf_counter() - t1
+        tol = {"value": W.n * args.steps / t_tol, "ms_per_step": t_tol * 1e3 / args.steps,
+               "kernels": wt.kernel_config, "pass_ms": wt.pass_times(30),
+               "over_exact": (W.n * args.steps / t_tol) / value}
+        wt.close()
+        print(f"[bench] tolerance path: {tol['ms_per_step']:.3f} ms/step ({tol['kernels']})", file=sys.stderr,
+              flush=True)
 
     # updateDensityMap of the last substep at a voxel of R/3.5 (16M voxels for uniform_1m,
     # the size of the reference's own 560 x 160 x 160 image)
@@ -506,8 +545,8 @@ def main() -> int:
                    "smoothing_radius": W.smoothing_radius, "path": "plain (one scene per launch)"},
         "roofline": {"bound": "hbm", "achieved": dom_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": dom_gbs / HBM_PEAK_GBS,
-                     "traffic": (TRAFFIC_BYTES_PER_LAUNCH[dom] if T == 1 << 21 and W is wl.UNIFORM_1M and
-                                 (args.warmup, args.steps) == TRAFFIC_STEPS else None),
+                     "traffic": traffic,
+                     "traffic_profile": TRAFFIC_PROFILE if traffic is not None else None,
                      "kernel": KERNEL_NAMES[dom], "avg_launch_ms": pt[f"{dom}_ms"],
                      "alg_bytes_per_launch": dom_bytes},
         "chain_roofline": {"alg_bytes_per_particle_substep": b_alg, "achieved_gbs": chain_gbs,
@@ -520,6 +559,7 @@ def main() -> int:
         "kernels": chosen,
         "pass_ms": pt,
         "pass_ms_by_kernel": variants,
+        "tolerance_path": tol,
         "density_map": dmap,
         "pcie_inclusive": pcie,
         "mixed_batch": mixed,
