@@ -21,14 +21,14 @@ namespace sph {
 
 namespace {
 
-constexpr int kBlock = 256;
-constexpr int kWaves = kBlock / 64;
 constexpr uint32_t kFlagAggregate = 1u << 30;
 constexpr uint32_t kFlagPrefix = 2u << 30;
 constexpr uint32_t kFlagMask = 3u << 30;
 constexpr uint32_t kValueMask = (1u << 30) - 1u;
+constexpr uint32_t kTileKeys = 4096;  // keys per tile, every digit width
 
-// Exclusive prefix sum over the block's 256 threads (one value each).
+// Exclusive prefix sum over the block's kBlk threads (one value each).
+template <int kBlk>
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *s_wave_tot) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     uint32_t x = v;
@@ -52,18 +52,24 @@ __device__ __forceinline__ void store_relaxed(uint32_t *p, uint32_t v) {
     __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// One LSD pass over kW-bit digits (kW = 8, 11 or 12: 256, 2048 or 4096 bins).  Thread
-// `tid` owns the kDPT consecutive digits tid * kDPT .. tid * kDPT + kDPT - 1 in the per-digit
-// phase (tile counts, scans, look-back).
-template <int kItems, int kW>
-__global__ __launch_bounds__(kBlock) void k_radix_pass(
+// One LSD pass over kW-bit digits (kW = 8, 11 or 12: 256, 2048 or 4096 bins) with kBlk
+// threads per 4096-key tile.  In the per-digit phase (tile counts, scans, look-back) the
+// first kBins / kDPT threads each own kDPT consecutive digits.  A tile's latency, not
+// bandwidth, bounds the pass: more waves per tile hide the loads, the ballots and the LDS
+// round trips of the ranking.
+template <int kBlk, int kW>
+__global__ __launch_bounds__(kBlk) void k_radix_pass(
     const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
     uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, uint32_t n, uint32_t shift,
     uint32_t bits, const uint32_t *__restrict__ ghist, uint32_t *lookback,
     uint32_t *tile_counter) {
-    constexpr int kTile = kBlock * kItems;
+    constexpr int kWaves = kBlk / 64;
+    constexpr int kItems = (int)kTileKeys / kBlk;
+    constexpr int kTile = (int)kTileKeys;
     constexpr int kBins = 1 << kW;
-    constexpr int kDPT = kBins / kBlock;
+    constexpr int kDPT = kBins >= kBlk ? kBins / kBlk : 1;
+    constexpr int kOwners = kBins / kDPT;
+    static_assert(kItems * kBlk == kTile, "tile must split evenly over the block");
     __shared__ uint32_t s_whist[kWaves][kBins];  // per wave: count, then LDS start of the wave's run
     __shared__ uint32_t s_keys[kTile];
     __shared__ uint32_t s_vals[kTile];
@@ -72,8 +78,9 @@ __global__ __launch_bounds__(kBlock) void k_radix_pass(
     __shared__ uint32_t s_tile;
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const bool owner = tid < kOwners;
     if (tid == 0) s_tile = atomicAdd(tile_counter, 1u);
-    for (int i = tid; i < kWaves * kBins; i += kBlock) (&s_whist[0][0])[i] = 0u;
+    for (int i = tid; i < kWaves * kBins; i += kBlk) (&s_whist[0][0])[i] = 0u;
     __syncthreads();
     const uint32_t tile = s_tile;
     const uint32_t wave_base = tile * kTile + (uint32_t)w * (64u * kItems);
@@ -89,14 +96,17 @@ __global__ __launch_bounds__(kBlock) void k_radix_pass(
         val[k] = valid ? (vin ? vin[pos] : pos) : 0u;
     }
 
-    // Rank within the wave, item by item, so equal digits keep their input order.
+    // Rank within the wave, item by item, so equal digits keep their input order.  All kW
+    // digit bits are compared: bits at and above `bits` are zero in every lane and leave
+    // the mask unchanged.
 #pragma unroll
     for (int k = 0; k < kItems; ++k) {
         const uint32_t pos = wave_base + (uint32_t)k * 64u + (uint32_t)lane;
         const bool valid = pos < n;
         const uint32_t d = (key[k] >> shift) & dmask;
         uint64_t peers = __ballot(valid);
-        for (uint32_t b = 0; b < bits; ++b) {
+#pragma unroll
+        for (int b = 0; b < kW; ++b) {
             const bool bit = (d >> b) & 1u;
             const uint64_t bb = __ballot(bit);
             peers &= bit ? bb : ~bb;
@@ -116,29 +126,31 @@ __global__ __launch_bounds__(kBlock) void k_radix_pass(
     // count of the waves before it.
     uint32_t tcount[kDPT];
     uint32_t tsum = 0, gsum = 0;
+    if (owner) {
 #pragma unroll
-    for (int q = 0; q < kDPT; ++q) {
-        const int d = tid * kDPT + q;
-        uint32_t sum = 0;
+        for (int q = 0; q < kDPT; ++q) {
+            const int d = tid * kDPT + q;
+            uint32_t sum = 0;
 #pragma unroll
-        for (int ww = 0; ww < kWaves; ++ww) {
-            const uint32_t c = s_whist[ww][d];
-            s_whist[ww][d] = sum;
-            sum += c;
+            for (int ww = 0; ww < kWaves; ++ww) {
+                const uint32_t c = s_whist[ww][d];
+                s_whist[ww][d] = sum;
+                sum += c;
+            }
+            tcount[q] = sum;
+            tsum += sum;
+            gsum += ghist[d];
         }
-        tcount[q] = sum;
-        tsum += sum;
-        gsum += ghist[d];
     }
-    const uint32_t gbase = block_exclusive_scan(gsum, s_tot);
-    const uint32_t lbase = block_exclusive_scan(tsum, s_tot);
+    const uint32_t gbase = block_exclusive_scan<kBlk>(gsum, s_tot);
+    const uint32_t lbase = block_exclusive_scan<kBlk>(tsum, s_tot);
 
     // Decoupled look-back for the owned digits, walked together: the global start of this
     // tile's run of each digit.
     uint32_t excl[kDPT], acc[kDPT], loff[kDPT];
     int32_t at[kDPT];
     uint32_t pending = 0;
-    {
+    if (owner) {
         uint32_t g = gbase, l = lbase;
 #pragma unroll
         for (int q = 0; q < kDPT; ++q) {
@@ -182,12 +194,14 @@ __global__ __launch_bounds__(kBlock) void k_radix_pass(
         }
         if (waited) __builtin_amdgcn_s_sleep(1);
     }
+    if (owner) {
 #pragma unroll
-    for (int q = 0; q < kDPT; ++q) {
-        const int d = tid * kDPT + q;
-        s_delta[d] = (int32_t)excl[q] - (int32_t)loff[q];
+        for (int q = 0; q < kDPT; ++q) {
+            const int d = tid * kDPT + q;
+            s_delta[d] = (int32_t)excl[q] - (int32_t)loff[q];
 #pragma unroll
-        for (int ww = 0; ww < kWaves; ++ww) s_whist[ww][d] += loff[q];
+            for (int ww = 0; ww < kWaves; ++ww) s_whist[ww][d] += loff[q];
+        }
     }
     __syncthreads();
 
@@ -205,7 +219,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_pass(
     __syncthreads();
     const uint32_t tile_begin = tile * kTile;
     const uint32_t tile_n = min((uint32_t)kTile, n - tile_begin);
-    for (uint32_t i = tid; i < tile_n; i += kBlock) {
+    for (uint32_t i = tid; i < tile_n; i += kBlk) {
         const uint32_t kk = s_keys[i];
         const uint32_t d = (kk >> shift) & dmask;
         const uint32_t gp = (uint32_t)(s_delta[d] + (int32_t)i);
@@ -214,28 +228,32 @@ __global__ __launch_bounds__(kBlock) void k_radix_pass(
     }
 }
 
-constexpr int kItemsDefault = 16;  // 4096-key tiles: half the look-back chain of 2048-key tiles
+// Threads per tile for each digit width.  8-bit digits: 1024 threads (16 waves, four per
+// SIMD) rank the 4096 keys with 4 items each.  The 11- and 12-bit passes keep 256 threads:
+// their per-wave histograms (2048 and 4096 bins) would not fit LDS for 16 waves.
+constexpr int kBlock8 = 1024;
+constexpr int kBlockWide = 256;
 
 }  // namespace
 
-uint32_t radix_tile_size() { return (uint32_t)(kBlock * kItemsDefault); }
+uint32_t radix_tile_size() { return kTileKeys; }
 
 hipError_t radix_pass(hipStream_t stream, const uint32_t *kin, const uint32_t *vin,
                       uint32_t *kout, uint32_t *vout, uint32_t n, uint32_t shift,
                       uint32_t bits, const uint32_t *ghist, uint32_t *lookback,
                       uint32_t *tile_counter, uint32_t digit_bits) {
-    const uint32_t tiles = (n + radix_tile_size() - 1u) / radix_tile_size();
+    const uint32_t tiles = (n + kTileKeys - 1u) / kTileKeys;
     switch (digit_bits) {
         case 11:
-            hipLaunchKernelGGL((k_radix_pass<kItemsDefault, 11>), dim3(tiles), dim3(kBlock), 0, stream, kin,
+            hipLaunchKernelGGL((k_radix_pass<kBlockWide, 11>), dim3(tiles), dim3(kBlockWide), 0, stream, kin,
                                vin, kout, vout, n, shift, bits, ghist, lookback, tile_counter);
             break;
         case 12:
-            hipLaunchKernelGGL((k_radix_pass<kItemsDefault, 12>), dim3(tiles), dim3(kBlock), 0, stream, kin,
+            hipLaunchKernelGGL((k_radix_pass<kBlockWide, 12>), dim3(tiles), dim3(kBlockWide), 0, stream, kin,
                                vin, kout, vout, n, shift, bits, ghist, lookback, tile_counter);
             break;
         default:
-            hipLaunchKernelGGL((k_radix_pass<kItemsDefault, 8>), dim3(tiles), dim3(kBlock), 0, stream, kin,
+            hipLaunchKernelGGL((k_radix_pass<kBlock8, 8>), dim3(tiles), dim3(kBlock8), 0, stream, kin,
                                vin, kout, vout, n, shift, bits, ghist, lookback, tile_counter);
     }
     return hipGetLastError();
