@@ -359,8 +359,13 @@ def serving(count: int, rate_hz: float, seed: int = 4, resident: int = 0,
         # last 10: the frame time the served frames are compared with
         w = sph_hip.World(big_params, resident)
         w.upload(*big_state)
+        t1 = time.perf_counter()
         w.step(max(0, int(st["frames"]) - 10))
         w.sync()
+        # mean over the same trajectory as the served frames (its online tuning included)
+        st["resident_alone_mean_ms_per_frame"] = ((time.perf_counter() - t1) * 1e3 /
+                                                  max(1, int(st["frames"]) - 10))
+        st["served_mean_ms_per_frame"] = wall * 1e3 / max(1, int(st["frames"]))
         w.autotune(3)  # on the settled state, like the served chain's fixed configuration
         w.step(1)
         w.sync()
