@@ -88,23 +88,28 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
     bool to_stray = false;
     if (j < n) {
         const uint32_t key = skeys[j];
+        const uint32_t prev = j ? skeys[j - 1u] : 0u;
+        const bool head = j == 0 || key != prev;
         int *tbl = reinterpret_cast<int *>(table);
-        if (j == 0) {
-            tbl[2u * key] = 0;
-        } else {
-            const uint32_t prev = skeys[j - 1u];
-            if (key != prev) {
-                tbl[2u * key] = (int)j;
-                tbl[2u * prev + 1u] = (int)j;
-            }
+        if (head) {
+            tbl[2u * key] = (int)j;
+            if (j) tbl[2u * prev + 1u] = (int)j;
         }
         if (j == n - 1u) tbl[2u * key + 1u] = (int)n;
-        if (bcount && (j == 0 || skeys[j - 1u] != key)) {
-            // Cell-ordered layout: the run's bucket (its first particle's cell) and its
-            // offset in the bucket.  The run's length comes from scanning its keys (runs
-            // are a few slots long; the table's end is written by another lane).
-            uint32_t e = j + 1u;
-            while (e < n && skeys[e] == key) ++e;
+        // Cell-ordered layout: the run's bucket (its first particle's cell) and its offset in
+        // the bucket.  A run ends at the next run head of the wave (one ballot); only the
+        // wave's last run, which may continue past the wave, scans the keys beyond it.
+        const uint64_t heads = bcount ? __ballot(head) : 0ull;
+        if (bcount && head) {
+            const uint32_t lane = tid & 63u;
+            const uint64_t above = heads & ~((2ull << lane) - 1ull);
+            uint32_t e;
+            if (above) {
+                e = j - lane + (uint32_t)(__ffsll((long long)above) - 1);
+            } else {
+                e = min(n, j - lane + 64u);
+                while (e < n && skeys[e] == key) ++e;
+            }
             const SceneConst c = sc.scene(j);
             const float4 P = pos[sidx ? sidx[j] : j];
             const uint32_t bucket = bucket_of(c, cell_of(P.x, c.minx, c.R), cell_of(P.y, c.miny, c.R),
