@@ -206,6 +206,23 @@ int sph_world_set_precision(sph_world *w, int mode);
  * (or 8 MiB); others return SPH_ERR_CAPACITY.  All widths give the same stable order, the
  * reference's 4 x 8-bit sort (commands.c:106-161) included. */
 int sph_world_set_radix_bits(sph_world *w, int bits);
+/* Force neighbour lists (recorded by the density pass for the force pass; no reference
+ * counterpart: the reference walks 27 cells in both passes).  A world starts with 64
+ * entries per particle, an uploaded state raises that to four times the neighbours of a
+ * uniform fill of its bounding box (a dense start such as the reference's own scene gets
+ * up to 254 at once), and at sph_world_sync / download_state (and when tuning) the lists
+ * grow to hold the longest that did not fit since the last check (twice the capacity, or
+ * that list plus a quarter, at most 254).  A particle whose list does not fit walks its
+ * cells in the force pass instead: results are the same bits at every capacity, only
+ * speed and memory (4 B per entry per particle) change.
+ *   sph_world_set_list_capacity: entries 1..254 fixes the capacity (reallocated now);
+ *     0 returns to the adaptive sizing above.
+ *   sph_world_list_capacity: the entries per particle allocated now.
+ *   sph_world_device_bytes: device memory the world holds now.  sph_world_footprint
+ *     bounds it at the largest list capacity. */
+int sph_world_set_list_capacity(sph_world *w, uint32_t entries);
+int sph_world_list_capacity(const sph_world *w);
+uint64_t sph_world_device_bytes(const sph_world *w);
 
 int sph_world_set_debug(sph_world *w, int enable);
 int sph_world_debug_dump(sph_world *w, int stage, void *dst, size_t bytes);
@@ -243,7 +260,9 @@ int sph_world_density_map(sph_world *w, float voxel_size, float *out, size_t cou
  * complete after sph_world_sync. */
 int sph_world_density_map_device(sph_world *w, float voxel_size, float *dev_out, size_t count);
 
-/* Device footprint in bytes of a world of n particles and table_cells cells (0 = n). */
+/* Device footprint in bytes of a world of n particles and table_cells cells (0 = n), with
+ * its force lists at their largest capacity (254 entries); sph_world_device_bytes gives
+ * what a world holds at its current capacity. */
 uint64_t sph_world_footprint(uint32_t n, uint32_t table_cells);
 
 void sph_world_destroy(sph_world *w);
@@ -273,6 +292,11 @@ int sph_batch_scene_uses_bricks(sph_batch *b, uint32_t scene);  /* 1, 0, or < 0 
 int sph_batch_pass_times(sph_batch *b, uint32_t substeps, sph_pass_times *out);
 /* sph_world_density_map for one scene of the batch (its own box and table range). */
 int sph_batch_density_map(sph_batch *b, uint32_t scene, float voxel_size, float *out, size_t count);
+/* Force-list capacity of the batch (one for all its scenes), as sph_world_set_list_capacity /
+ * sph_world_list_capacity: sized from each uploaded scene and grown at sph_batch_sync /
+ * download_scene unless fixed. */
+int sph_batch_set_list_capacity(sph_batch *b, uint32_t entries);
+int sph_batch_list_capacity(const sph_batch *b);
 uint32_t sph_batch_particles(const sph_batch *b);
 void sph_batch_destroy(sph_batch *b);
 
@@ -280,8 +304,9 @@ void sph_batch_destroy(sph_batch *b);
  * Serving: a request loop over the batched step.  No reference counterpart (the reference
  * runs one interactive scene).  Requests carry their parameters, initial state and step
  * count; a worker thread admits them in arrival order while the sum of resident footprints
- * (1152 B per particle, 24 B per table cell, 8 B per halo-grid cell) stays within
- * hbm_budget_bytes and the capacities hold, keeps their state resident in device memory,
+ * (136 B + 4 B per force-list entry per particle: 648 B at the default 128 entries; 24 B per
+ * table cell, 8 B per halo-grid cell) stays within hbm_budget_bytes and the capacities
+ * hold, keeps their state resident in device memory,
  * steps every resident scene together one frame at a time, and retires finished scenes
  * with their state copied back.
  * Requests arriving while others run are admitted at the next frame boundary (continuous
@@ -297,6 +322,7 @@ typedef struct sph_server_config {
     uint32_t max_table_cells;   /* resident table cells at once                           */
     uint32_t max_bricks;        /* resident bricks at once; 0 = 2^22                      */
     uint32_t substeps;          /* substeps per frame, shared by all requests             */
+    uint32_t list_entries;      /* force-list entries per particle (1..254); 0 = 128       */
 } sph_server_config;
 typedef enum sph_request_state {
     SPH_REQ_QUEUED = 0, SPH_REQ_RUNNING = 1, SPH_REQ_DONE = 2, SPH_REQ_FAILED = 3

@@ -43,6 +43,7 @@ struct sph_batch {
     bool stepped = false;
     float *d_map = nullptr;  // density map staging for host copies
     size_t map_cap = 0;
+    uint32_t list_pinned = 0;  // force-list entries fixed by sph_batch_set_list_capacity (0: adaptive)
     ManyScenes policy() const { return ManyScenes{d_scenes, d_scene_of, count}; }
 };
 
@@ -155,7 +156,7 @@ int sph_batch_create(uint32_t count, const sph_params *params, const uint32_t *c
         delete b;
         return hip_fail(e, "stream creation");
     }
-    int rc = b->chain.allocate(b->n_total, b->t_total, b->b_total ? b->b_total : 1u, ub, gb);
+    int rc = b->chain.allocate(b->n_total, b->t_total, b->b_total ? b->b_total : 1u, ub, gb, kListStart);
     if (rc) {
         hipStreamDestroy(b->stream);
         delete b;
@@ -196,9 +197,17 @@ static int batch_apply(sph_batch *b, int v, bool all) {
     return upload_scene_table(b);
 }
 
+// Synchronises the batch's stream; adaptive force lists grow to hold the longest one that
+// did not fit (one capacity for every scene).  Returns 1 when they grew, 0, or an error.
+static int batch_sync_lists(sph_batch *b) {
+    const int rc = b->chain.sync_lists(b->stream, !b->list_pinned);
+    if (rc == 1) b->graph.drop();
+    return rc;
+}
+
 // Times a few substeps of each configuration on a saved copy of the state and keeps the
 // fastest (results are identical either way).
-static int batch_autotune(sph_batch *b, uint32_t substeps) {
+static int batch_autotune_once(sph_batch *b, uint32_t substeps) {
     const size_t bytes = (size_t)b->n_total * 16;
     float4 *sp = nullptr, *sv = nullptr;
     hipError_t e = hipMalloc((void **)&sp, bytes);
@@ -260,6 +269,17 @@ static int batch_autotune(sph_batch *b, uint32_t substeps) {
     return batch_apply(b, b->tuned_variant, false);
 }
 
+// The measurement again while the trials grew the force lists (their list configurations
+// were timed with lists too short for the state).
+static int batch_autotune(sph_batch *b, uint32_t substeps) {
+    for (;;) {
+        int rc = batch_autotune_once(b, substeps);
+        if (rc) return rc;
+        const int grew = batch_sync_lists(b);
+        if (grew <= 0) return grew;
+    }
+}
+
 int sph_batch_set_kernel(sph_batch *b, int variant) {
     if (!b) return fail(SPH_ERR_INVALID_ARG, "batch is NULL");
     if (variant < SPH_KERNEL_AUTO || variant > SPH_KERNEL_BRICK)
@@ -282,6 +302,10 @@ int sph_batch_upload_scene(sph_batch *b, uint32_t scene, const float *pos4, cons
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     if (e != hipSuccess) return hip_fail(e, "upload");
     b->brick_choice[scene] = (uint8_t)choose_bricks(b->scenes[scene], b->n[scene], pos4, SPH_KERNEL_AUTO);
+    if (!b->list_pinned) {
+        const uint32_t want = list_entries_for_state(b->params[scene], b->n[scene], pos4);
+        if (want > b->chain.b.list_cap && b->chain.set_list_cap(want) == SPH_OK) b->graph.drop();
+    }
     if (b->variant == SPH_KERNEL_AUTO) b->tuned_variant = -1;  // re-tune for the new state
     b->stepped = false;
     return SPH_OK;
@@ -312,8 +336,26 @@ int sph_batch_step(sph_batch *b, uint32_t frames) {
 
 int sph_batch_sync(sph_batch *b) {
     if (!b) return fail(SPH_ERR_INVALID_ARG, "batch is NULL");
+    const int rc = batch_sync_lists(b);
+    return rc < 0 ? rc : SPH_OK;
+}
+
+int sph_batch_set_list_capacity(sph_batch *b, uint32_t entries) {
+    if (!b) return fail(SPH_ERR_INVALID_ARG, "batch is NULL");
+    if (entries > (uint32_t)kList) return fail(SPH_ERR_INVALID_ARG, "force lists hold at most %d entries", kList);
+    b->list_pinned = entries;
+    if (entries == 0) return SPH_OK;
     hipError_t e = hipStreamSynchronize(b->stream);
-    return e == hipSuccess ? SPH_OK : hip_fail(e, "hipStreamSynchronize");
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    const int rc = b->chain.set_list_cap(entries);
+    if (rc) return rc;
+    b->graph.drop();
+    return SPH_OK;
+}
+
+int sph_batch_list_capacity(const sph_batch *b) {
+    if (!b) return fail(SPH_ERR_INVALID_ARG, "batch is NULL");
+    return (int)b->chain.b.list_cap;
 }
 
 int sph_batch_download_scene(sph_batch *b, uint32_t scene, float *pos4, float *vel4, float *color4) {
@@ -330,8 +372,9 @@ int sph_batch_download_scene(sph_batch *b, uint32_t scene, float *pos4, float *v
     if (pos4) e = hipMemcpyAsync(pos4, b->chain.b.pos + off, n * 16, hipMemcpyDeviceToHost, b->stream);
     if (e == hipSuccess && vdst)
         e = hipMemcpyAsync(vdst, b->chain.b.vel + off, n * 16, hipMemcpyDeviceToHost, b->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     if (e != hipSuccess) return hip_fail(e, "download");
+    const int rc = batch_sync_lists(b);
+    if (rc < 0) return rc;
     if (color4) speed_ramp((uint32_t)n, vdst, color4);
     return SPH_OK;
 }

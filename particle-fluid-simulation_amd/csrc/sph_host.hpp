@@ -25,6 +25,11 @@ int validate_state(const sph_params &p, uint32_t n, const float *pos4, const flo
 uint32_t key_bits_for(uint64_t table_total);
 void speed_ramp(uint32_t n, const float *vel4, float *color4);
 
+// Force-list entries a chain starts with, and the fewest an uploaded state is given.
+constexpr uint32_t kListStart = 64;
+// Force-list entries for an uploaded state of n particles (vec4 positions).
+uint32_t list_entries_for_state(const sph_params &p, uint32_t n, const float *pos4);
+
 // hipEvents bracketing the five passes of one substep.
 struct PassEvents {
     hipEvent_t ev[6] = {};
@@ -45,10 +50,23 @@ struct Chain {
     uint32_t radix_bits = 8;   // radix digit width (8, 11 or 12)
     uint64_t lookback_cap = 0; // look-back words allocated (fixes which widths are possible)
     uint32_t n_cap = 0, t_cap = 0, b_cap = 0, u_cap = 0, g_cap = 0;  // allocated capacity
+    uint64_t bytes = 0;        // device memory allocated by allocate() and set_list_cap()
+    uint32_t *h_list = nullptr;  // pinned host word: the longest list that did not fit
 
-    // Allocates for the given capacity and configures the chain to use all of it.
+    // Allocates for the given capacity (force lists of list_entries per slot) and
+    // configures the chain to use all of it.
     int allocate(uint32_t n, uint32_t table_total, uint32_t brick_total, uint32_t bucket_total,
-                 uint32_t grid_total);
+                 uint32_t grid_total, uint32_t list_entries);
+    // Reallocates the force lists at `entries` per slot (1..kList) for n_cap slots: the new
+    // buffer is allocated before the old one is freed, so on failure the chain keeps the
+    // old capacity.  The caller has synchronised the chain's stream.
+    int set_list_cap(uint32_t entries);
+    // Synchronises `s`.  With `adapt` (and lists shorter than kList), first queues the read
+    // of the longest force list that did not fit since the last call, and afterwards grows
+    // the lists to twice their capacity or that list plus a quarter, in steps of 32 entries
+    // (at most kList).  Returns 1 when the lists grew (a captured graph must be dropped), 0,
+    // or an error.
+    int sync_lists(hipStream_t s, bool adapt);
     // Uses the first n slots, table_total cells, brick_total bricks, bucket_total buckets
     // and grid_total halo-grid cells (within capacity).
     void configure(uint32_t n, uint32_t table_total, uint32_t brick_total, uint32_t bucket_total,

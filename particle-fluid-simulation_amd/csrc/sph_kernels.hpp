@@ -43,6 +43,8 @@ struct ChainBuffers {
     uint32_t *queue;        // [brick_total] non-empty bricks of this substep
     uint32_t *stray;        // [n] slots outside every brick grid
     uint32_t *counters;     // [0] queue length [1] strays [2] density drain [3] force drain
+                            // [4] entries of the longest force list that did not fit
+                            // list_cap since the host last cleared it (0: none)
     // Cell-ordered layout (sph_layout.hip).
     uint32_t *bcount;       // [bucket_total] particles per bucket, then the bucket's first slot
     uint32_t *blookback;    // [bucket tiles] look-back words of the bucket scan
@@ -51,8 +53,9 @@ struct ChainBuffers {
     uint32_t *hslot;        // [n] hash slot of each cell-ordered slot
     int2 *grid;             // [grid_total] (start, end) in cell order of each halo-grid cell's run
     // Force neighbour lists recorded by the per-lane density pass (cell-ordered slots).
-    uint32_t *nlist;        // [kList][n]: entry k of slot j at nlist[k * n + j]
+    uint32_t *nlist;        // [list_cap][n]: entry k of slot j at nlist[k * n + j]
     uint8_t *ncount;        // [n] entries, or 0xFF: the force pass walks the cells
+    uint32_t list_cap;      // entries per slot allocated at nlist (<= kList)
 };
 
 uint32_t radix_tile_size();

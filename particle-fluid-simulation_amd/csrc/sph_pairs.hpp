@@ -537,8 +537,10 @@ __device__ __forceinline__ void force_finish(const SceneConst &c, const float4 &
 // coalesced row per entry), the few beyond are stored directly; the force pass then reads
 // its list instead of scanning 27 cells.  A particle for
 // which the cell relation fails (a rounded sample cell that is not c0 -/+ 1, a predicted
-// position outside the box), or whose list would exceed kList entries, gets the density's
-// own walk and an invalid count, and the force pass walks its cells.
+// position outside the box), or whose list would exceed the `cap` entries allocated per
+// slot (at most kList), gets an invalid count, and the force pass walks its cells; `found`
+// returns the candidates the walk recorded (0 when it did not record), so that the kernel
+// can report lists that did not fit.
 constexpr int kListLds = 20;   // entries staged in LDS (the rest are stored directly)
 
 template <int kU, bool kFast = false>
@@ -546,7 +548,8 @@ __device__ __forceinline__ float2 density_with_list(const SceneConst &c, uint32_
                                                     const float4 *__restrict__ pred_pos,
                                                     const GridRuns &runs, uint32_t *lane_list,
                                                     int stride, uint32_t *__restrict__ nlist,
-                                                    uint32_t n, uint32_t &count) {
+                                                    uint32_t n, uint32_t cap, uint32_t &count,
+                                                    uint32_t &found) {
     const AxisCells ax = sample_axis(s.x, c.minx, c.maxx, c.R);
     const AxisCells ay = sample_axis(s.y, c.miny, c.maxy, c.R);
     const AxisCells az = sample_axis(s.z, c.minz, c.maxz, c.R);
@@ -557,6 +560,7 @@ __device__ __forceinline__ float2 density_with_list(const SceneConst &c, uint32_
     if (!(same(ax) && same(ay) && same(az) && (int32_t)ax.mid == c0.x && (int32_t)ay.mid == c0.y &&
           (int32_t)az.mid == c0.z)) {
         count = kListInvalid;
+        found = 0;
         return density_walk_rows<kU, kFast>(c, s, pred_pos, DensityCellsOf<GridRuns>{runs, ax, ay, az});
     }
     const float R2 = c.R2;
@@ -587,7 +591,7 @@ __device__ __forceinline__ float2 density_with_list(const SceneConst &c, uint32_
                             const uint32_t slot = (uint32_t)(r[dx].base + k + u);
                             if (slot != j && !(d2 > R2)) {
                                 if (nl < (uint32_t)kListLds) lane_list[nl * stride] = slot;
-                                else if (nl < (uint32_t)kList) nlist[(size_t)nl * n + j] = slot;
+                                else if (nl < cap) nlist[(size_t)nl * n + j] = slot;
                                 ++nl;
                             }
                         }
@@ -595,7 +599,8 @@ __device__ __forceinline__ float2 density_with_list(const SceneConst &c, uint32_
                 }
             }
         }
-    count = nl <= (uint32_t)kList ? nl : kListInvalid;
+    count = nl <= cap ? nl : kListInvalid;
+    found = nl;
     return density_finish(c, M, Mn);
 }
 

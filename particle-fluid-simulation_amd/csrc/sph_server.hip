@@ -94,9 +94,15 @@ struct sph_server {
 
 namespace {
 
-// Device bytes a resident scene occupies (the chain's per-slot buffers and its table).
-uint64_t scene_footprint(uint32_t n, uint32_t T, uint32_t grid_cells) {
-    return (uint64_t)n * (136u + 4u * kList) + (uint64_t)T * 24u + (uint64_t)grid_cells * 8u;
+// Force-list entries per slot when the configuration leaves them at 0: room for the
+// ~100 neighbours of a breaking dam or a settled column at the target density; a denser
+// particle walks its cells instead (the same result).
+constexpr uint32_t kServerListEntries = 128;
+
+// Device bytes a resident scene occupies (the chain's per-slot buffers with force lists of
+// `list_entries` per slot, and its table and halo grid).
+uint64_t scene_footprint(uint32_t n, uint32_t T, uint32_t grid_cells, uint32_t list_entries) {
+    return (uint64_t)n * (136u + 4u * list_entries) + (uint64_t)T * 24u + (uint64_t)grid_cells * 8u;
 }
 
 bool fits(const sph_server *s, const Request *r) {
@@ -292,10 +298,13 @@ int sph_server_create(const sph_server_config *cfg, sph_server **out) {
     if (cfg->max_table_cells == 0 || cfg->max_table_cells >= (1u << 31))
         return fail(SPH_ERR_INVALID_ARG, "max_table_cells must be in [1, 2^31)");
     if (cfg->substeps == 0) return fail(SPH_ERR_INVALID_ARG, "substeps must be >= 1");
+    if (cfg->list_entries > (uint32_t)kList)
+        return fail(SPH_ERR_INVALID_ARG, "list_entries must be at most %d", kList);
     sph_server *s = new (std::nothrow) sph_server();
     if (!s) return fail(SPH_ERR_OUT_OF_MEMORY, "host allocation");
     s->cfg = *cfg;
     if (s->cfg.max_bricks == 0) s->cfg.max_bricks = 1u << 22;
+    if (s->cfg.list_entries == 0) s->cfg.list_entries = kServerListEntries;
     s->cfg_bricks = std::getenv("SPH_SERVER_BRICKS") && std::strcmp(std::getenv("SPH_SERVER_BRICKS"), "1") == 0;
     hipError_t e = hipGetDevice(&s->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
@@ -312,7 +321,7 @@ int sph_server_create(const sph_server_config *cfg, sph_server **out) {
         return fail(SPH_ERR_CAPACITY, "max_particles / max_scenes too large");
     }
     int rc = s->chain.allocate(cfg->max_particles, cfg->max_table_cells, s->cfg.max_bricks,
-                               (uint32_t)bucket_cap, (uint32_t)grid_cap);
+                               (uint32_t)bucket_cap, (uint32_t)grid_cap, s->cfg.list_entries);
     if (rc) {
         hipStreamDestroy(s->stream);
         delete s;
@@ -352,7 +361,7 @@ int sph_server_submit(sph_server *s, const sph_params *params, uint32_t n, const
     r->params.table_cells = r->T;
     const SceneConst c = make_scene_const(r->params, r->T, 0, 0, n, 0, 0, 0);
     r->bricks = scene_bricks(c);
-    r->footprint = scene_footprint(n, r->T, c.grid_cells);
+    r->footprint = scene_footprint(n, r->T, c.grid_cells, s->cfg.list_entries);
     if (r->footprint > s->cfg.hbm_budget_bytes || n > s->cfg.max_particles || r->T > s->cfg.max_table_cells ||
         (s->cfg_bricks && r->bricks > s->chain.b_cap))
         return fail(SPH_ERR_CAPACITY, "request exceeds the server's capacity");

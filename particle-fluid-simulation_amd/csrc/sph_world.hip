@@ -187,7 +187,7 @@ bool Chain::set_radix_bits(uint32_t w) {
 }
 
 int Chain::allocate(uint32_t n_, uint32_t table_total_, uint32_t brick_total_,
-                    uint32_t bucket_total_, uint32_t grid_total_) {
+                    uint32_t bucket_total_, uint32_t grid_total_, uint32_t list_entries) {
     n_cap = n_;
     t_cap = table_total_;
     b_cap = brick_total_;
@@ -219,18 +219,29 @@ int Chain::allocate(uint32_t n_, uint32_t table_total_, uint32_t brick_total_,
         {(void **)&b.run_info, (size_t)table_total * 8}, {(void **)&b.crun, (size_t)table_total * 8},
         {(void **)&b.hslot, N * 4},
         {(void **)&b.grid, (size_t)(grid_total ? grid_total : 1) * 8},
-        {(void **)&b.nlist, N * kList * 4}, {(void **)&b.ncount, N},
+        {(void **)&b.nlist, N * list_entries * 4}, {(void **)&b.ncount, N},
         {(void **)&b.counters, 8 * 4},
         {(void **)&b.brick_flag, (size_t)(brick_total ? brick_total : 1) * 4},
         {(void **)&b.queue, (size_t)(brick_total ? brick_total : 1) * 4},
     };
+    if (list_entries < 1u || list_entries > (uint32_t)kList)
+        return fail(SPH_ERR_INVALID_ARG, "force lists of %u entries (1..%d)", list_entries, kList);
+    bytes = 0;
     for (auto &r : reqs) {
         hipError_t e = hipMalloc(r.p, r.bytes);
         if (e != hipSuccess) {
             release();
             return fail(SPH_ERR_OUT_OF_MEMORY, "hipMalloc(%zu): %s", r.bytes, hipGetErrorString(e));
         }
+        bytes += r.bytes;
     }
+    b.list_cap = list_entries;
+    if (hipHostMalloc((void **)&h_list, 4, 0) != hipSuccess) {
+        h_list = nullptr;
+        release();
+        return fail(SPH_ERR_OUT_OF_MEMORY, "pinned host word");
+    }
+    *h_list = 0u;
     hipError_t e = hipMemset(b.ghist, 0, kGhistWords * 4);
     if (e == hipSuccess) e = hipMemset(b.counters, 0, 8 * 4);
     if (e == hipSuccess) e = hipMemset(b.brick_flag, 0, (size_t)(brick_total ? brick_total : 1) * 4);
@@ -257,6 +268,57 @@ int Chain::set_debug(bool on) {
     return SPH_OK;
 }
 
+int Chain::set_list_cap(uint32_t entries) {
+    if (entries < 1u || entries > (uint32_t)kList)
+        return fail(SPH_ERR_INVALID_ARG, "force lists of %u entries (1..%d)", entries, kList);
+    if (entries == b.list_cap) return SPH_OK;
+    uint32_t *fresh = nullptr;
+    const size_t want = (size_t)n_cap * entries * 4;
+    hipError_t e = hipMalloc((void **)&fresh, want);
+    if (e != hipSuccess) return fail(SPH_ERR_OUT_OF_MEMORY, "force lists of %u entries: %s", entries, hipGetErrorString(e));
+    if (b.nlist) (void)hipFree(b.nlist);
+    bytes = bytes - (size_t)n_cap * b.list_cap * 4 + want;
+    b.nlist = fresh;
+    b.list_cap = entries;
+    return SPH_OK;
+}
+
+int Chain::sync_lists(hipStream_t s, bool adapt) {
+    adapt = adapt && h_list && b.list_cap < (uint32_t)kList;
+    hipError_t e = adapt ? hipMemcpyAsync(h_list, b.counters + 4, 4, hipMemcpyDeviceToHost, s) : hipSuccess;
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    if (!adapt) return 0;
+    const uint32_t longest = *h_list;
+    if (longest <= b.list_cap) return 0;
+    *h_list = 0u;
+    // the stream is idle: clear the record for the next read
+    e = hipMemsetAsync(b.counters + 4, 0, 4, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_fail(e, "clearing the list record");
+    const uint32_t want = std::min((uint32_t)kList, std::max(2u * b.list_cap, (longest + longest / 4u + 31u) / 32u * 32u));
+    return set_list_cap(want) == SPH_OK ? 1 : 0;
+}
+
+uint32_t list_entries_for_state(const sph_params &p, uint32_t n, const float *pos4) {
+    // four times the neighbours of a uniform fill of the state's bounding box (each extent
+    // at least one smoothing radius), in steps of 32, between kListStart and kList: a dense
+    // start (a resting block, the reference's own scene) gets long lists at once instead of
+    // walking until the first synchronisation grows them
+    float lo[3] = {pos4[0], pos4[1], pos4[2]}, hi[3] = {pos4[0], pos4[1], pos4[2]};
+    for (uint32_t i = 1; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], pos4[4 * (size_t)i + a]);
+            hi[a] = std::max(hi[a], pos4[4 * (size_t)i + a]);
+        }
+    const double R = p.smoothing_radius;
+    double vol = 1.0;
+    for (int a = 0; a < 3; ++a) vol *= std::max((double)hi[a] - lo[a], R);
+    const double neighbours = (double)n / vol * (4.0 / 3.0) * M_PI * R * R * R;
+    const double want = std::ceil(4.0 * neighbours / 32.0) * 32.0;
+    return (uint32_t)std::min<double>((double)kList, std::max<double>((double)kListStart, want));
+}
+
 void Chain::release() {
     void *ptrs[] = {b.pos, b.vel, b.keys, b.sort_k[0], b.sort_v[0], b.sort_k[1], b.sort_v[1],
                     b.pred_pos, b.pos_sorted, b.vel_sorted, b.dens, b.table, b.ghist,
@@ -265,7 +327,10 @@ void Chain::release() {
                     b.run_info, b.crun, b.hslot, b.grid, b.nlist, b.ncount};
     for (void *p : ptrs)
         if (p) hipFree(p);
+    if (h_list) (void)hipHostFree(h_list);
+    h_list = nullptr;
     b = ChainBuffers{};
+    bytes = 0;
 }
 
 const uint32_t *Chain::sorted_keys() const { return passes ? b.sort_k[(passes - 1u) & 1u] : b.keys; }
@@ -396,6 +461,8 @@ struct sph_world {
     float4 *own_pos = nullptr, *own_vel = nullptr;
     bool tolerance = false;         // SPH_PRECISION_TOLERANCE: the tuner may pick the
                                     // approximate-math configurations
+    uint32_t list_pinned = 0;       // force-list entries fixed by sph_world_set_list_capacity
+                                    // (0: sized from the state and grown when lists overflow)
     // Online autotuning of an AUTO world: its first substeps run the candidate
     // configurations (two each, the second between events); all give bit-identical
     // results, so these are the world's real substeps and nothing is saved or restored.
@@ -442,6 +509,14 @@ int world_apply_code(sph_world *w, int code) {
     if (!w->chain.set_radix_bits(width ? width : 8u)) w->chain.set_radix_bits(8u);
     w->graph.drop();
     return SPH_OK;
+}
+
+// Synchronises the world's stream; an adaptive world (lists not pinned) grows its force
+// lists to hold the longest that did not fit.  Returns 1 when they grew, 0, or an error.
+int world_sync_lists(sph_world *w) {
+    const int rc = w->chain.sync_lists(w->stream, !w->list_pinned);
+    if (rc == 1) w->graph.drop();
+    return rc;
 }
 
 // Times `substeps` eager substeps of each configuration; restores the state afterwards.
@@ -548,8 +623,13 @@ int online_start(sph_world *w) {
 // the kernels, try the radix widths that need fewer passes with the fastest kernels.
 int online_finish_phase(sph_world *w) {
     auto &o = w->online;
-    hipError_t e = hipEventSynchronize(o.ev.back());
-    if (e != hipSuccess) return hip_fail(e, "online autotune");
+    const int grew = world_sync_lists(w);
+    if (grew < 0) return grew;
+    if (grew && o.phase == 0) {
+        // the list configurations ran with lists too short for this state: time them again
+        std::vector<int> codes = o.codes;
+        return online_begin_phase(w, std::move(codes));
+    }
     for (size_t i = 0; i < o.codes.size(); ++i) {
         float ms = 0.0f;
         if (hipEventElapsedTime(&ms, o.ev[2 * i], o.ev[2 * i + 1]) != hipSuccess) continue;
@@ -651,9 +731,9 @@ uint64_t sph_world_footprint(uint32_t n, uint32_t table_cells) {
     const uint64_t T = table_cells ? table_cells : n;
     const uint64_t tiles = ((uint64_t)n + radix_tile_size() - 1) / radix_tile_size();
     // state, keys, 2 x radix ping-pong, 3 sorted float4 arrays, densities, brick and
-    // cell-order slot arrays; per table cell its (start, end), run placement and cell-ordered
-    // run; buckets (at most 2n + 1024), the halo grid (at most 16n + 4096 cells) and
-    // look-back words
+    // cell-order slot arrays, force lists at their largest capacity (kList entries); per
+    // table cell its (start, end), run placement and cell-ordered run; buckets (at most
+    // 2n + 1024), the halo grid (at most 16n + 4096 cells) and look-back words
     return (uint64_t)n * (16 * 2 + 4 + 16 + 16 * 3 + 8 + 8 + 4 + 4 * kList + 1) + T * 24 +
            (2 * (uint64_t)n + 1024) * 4 + (16 * (uint64_t)n + 4096) * 8 +
            std::max<uint64_t>(4 * tiles * 256 * 4, std::max<uint64_t>(16 * (uint64_t)n, 8ull << 20)) +
@@ -683,7 +763,7 @@ int sph_world_create(const sph_params *params, uint32_t n, sph_world **out) {
     }
     w->scene.c = make_scene_const(w->params, T, 0, 0, n, 0, 0, 0);
     rc = w->chain.allocate(n, T, (uint32_t)scene_bricks(w->scene.c), scene_buckets(w->scene.c),
-                           w->scene.c.grid_cells);
+                           w->scene.c.grid_cells, kListStart);
     if (rc) {
         hipStreamDestroy(w->stream);
         delete w;
@@ -724,6 +804,10 @@ int sph_world_upload_state(sph_world *w, const float *pos4, const float *vel4) {
     if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
     if (e != hipSuccess) return hip_fail(e, "upload");
     w->stepped = false;
+    if (!w->list_pinned) {
+        const uint32_t want = list_entries_for_state(w->params, w->n, pos4);
+        if (want > w->chain.b.list_cap && w->chain.set_list_cap(want) == SPH_OK) w->graph.drop();
+    }
     return SPH_OK;
 }
 
@@ -760,8 +844,8 @@ int sph_world_step(sph_world *w, uint32_t frames) {
 
 int sph_world_sync(sph_world *w) {
     if (!w) return fail(SPH_ERR_INVALID_ARG, "world is NULL");
-    hipError_t e = hipStreamSynchronize(w->stream);
-    return e == hipSuccess ? SPH_OK : hip_fail(e, "hipStreamSynchronize");
+    const int rc = world_sync_lists(w);
+    return rc < 0 ? rc : SPH_OK;
 }
 
 int sph_world_download_state(sph_world *w, float *pos4, float *vel4, float *color4) {
@@ -776,8 +860,9 @@ int sph_world_download_state(sph_world *w, float *pos4, float *vel4, float *colo
     if (pos4) e = hipMemcpyAsync(pos4, w->chain.b.pos, (size_t)w->n * 16, hipMemcpyDeviceToHost, w->stream);
     if (e == hipSuccess && vdst)
         e = hipMemcpyAsync(vdst, w->chain.b.vel, (size_t)w->n * 16, hipMemcpyDeviceToHost, w->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
     if (e != hipSuccess) return hip_fail(e, "download");
+    const int rc = world_sync_lists(w);
+    if (rc < 0) return rc;
     if (color4) speed_ramp(w->n, vdst, color4);
     return SPH_OK;
 }
@@ -840,8 +925,15 @@ int sph_world_autotune(sph_world *w, uint32_t substeps, int *chosen, double *ms_
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
     online_cancel(w);
     int code = 0;
-    int rc = world_autotune(w, substeps, &code, ms_per_substep);
-    if (rc) return rc;
+    int rc = 0;
+    for (;;) {
+        rc = world_autotune(w, substeps, &code, ms_per_substep);
+        if (rc) return rc;
+        // the list configurations' trials overflowed lists that have now grown: again
+        const int grew = world_sync_lists(w);
+        if (grew < 0) return grew;
+        if (!grew) break;
+    }
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune_cache[tune_bucket(w->n, w->T, w->chain.brick_total, w->tolerance)] = code;
     w->variant = SPH_KERNEL_AUTO;
@@ -884,6 +976,29 @@ int sph_world_set_radix_bits(sph_world *w, int bits) {
 }
 
 int sph_lane_configs(void) { return kNumLaneConfigs; }
+
+int sph_world_set_list_capacity(sph_world *w, uint32_t entries) {
+    if (!w) return fail(SPH_ERR_INVALID_ARG, "world is NULL");
+    if (entries > (uint32_t)kList) return fail(SPH_ERR_INVALID_ARG, "force lists hold at most %d entries", kList);
+    w->list_pinned = entries;
+    if (entries == 0) return SPH_OK;
+    hipError_t e = hipStreamSynchronize(w->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    const int rc = w->chain.set_list_cap(entries);
+    if (rc) return rc;
+    w->graph.drop();
+    return SPH_OK;
+}
+
+int sph_world_list_capacity(const sph_world *w) {
+    if (!w) return fail(SPH_ERR_INVALID_ARG, "world is NULL");
+    return (int)w->chain.b.list_cap;
+}
+
+uint64_t sph_world_device_bytes(const sph_world *w) {
+    if (!w) return 0;
+    return w->chain.bytes + (w->d_snap ? (uint64_t)w->n * 32 : 0) + (uint64_t)w->map_cap * 4;
+}
 
 int sph_world_set_precision(sph_world *w, int mode) {
     if (!w) return fail(SPH_ERR_INVALID_ARG, "world is NULL");

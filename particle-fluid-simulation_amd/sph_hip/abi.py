@@ -99,6 +99,11 @@ def load_library() -> C.CDLL:
         "sph_lane_configs": (C.c_int, []),
         "sph_world_set_precision": (C.c_int, [vp, C.c_int]),
         "sph_world_set_radix_bits": (C.c_int, [vp, C.c_int]),
+        "sph_world_set_list_capacity": (C.c_int, [vp, C.c_uint32]),
+        "sph_world_list_capacity": (C.c_int, [vp]),
+        "sph_world_device_bytes": (C.c_uint64, [vp]),
+        "sph_batch_set_list_capacity": (C.c_int, [vp, C.c_uint32]),
+        "sph_batch_list_capacity": (C.c_int, [vp]),
         "sph_world_debug_dump": (C.c_int, [vp, C.c_int, vp, C.c_size_t]),
         "sph_world_pass_times": (C.c_int, [vp, C.c_uint32, P(PassTimes)]),
         "sph_world_footprint": (C.c_uint64, [C.c_uint32, C.c_uint32]),

@@ -43,6 +43,17 @@ class Batch:
     def set_kernel(self, variant: str) -> None:
         check(self._lib.sph_batch_set_kernel(self._h, KERNELS[variant]))
 
+    def set_list_capacity(self, entries: int) -> None:
+        """Fixes the force-list entries per particle (1..254); 0: adaptive sizing."""
+        check(self._lib.sph_batch_set_list_capacity(self._h, entries))
+
+    @property
+    def list_capacity(self) -> int:
+        rc = self._lib.sph_batch_list_capacity(self._h)
+        if rc < 0:
+            check(rc)
+        return rc
+
     def upload(self, scene: int, pos: np.ndarray, vel: np.ndarray) -> None:
         n = self.counts[scene]
         pos = np.ascontiguousarray(pos, np.float32).reshape(n, 4)

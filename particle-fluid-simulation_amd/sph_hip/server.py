@@ -11,7 +11,7 @@ from .abi import SphParams, check, load_library
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_scenes", C.c_uint32),
                 ("max_particles", C.c_uint32), ("max_table_cells", C.c_uint32),
-                ("max_bricks", C.c_uint32), ("substeps", C.c_uint32)]
+                ("max_bricks", C.c_uint32), ("substeps", C.c_uint32), ("list_entries", C.c_uint32)]
 
 
 class ServerStats(C.Structure):
@@ -32,11 +32,11 @@ STATES = {0: "queued", 1: "running", 2: "done", 3: "failed"}
 
 class Server:
     def __init__(self, *, hbm_budget_bytes: int, max_scenes: int, max_particles: int,
-                 max_table_cells: int, substeps: int, max_bricks: int = 0):
+                 max_table_cells: int, substeps: int, max_bricks: int = 0, list_entries: int = 0):
         self._lib = load_library()
         self._h = C.c_void_p()
         cfg = ServerConfig(hbm_budget_bytes, max_scenes, max_particles, max_table_cells,
-                           max_bricks, substeps)
+                           max_bricks, substeps, list_entries)
         check(self._lib.sph_server_create(C.byref(cfg), C.byref(self._h)))
         self._n = {}
 

@@ -143,6 +143,21 @@ class World:
         """Pins the radix digit width (8, 11 or 12 bits)."""
         check(self._lib.sph_world_set_radix_bits(self._h, bits))
 
+    def set_list_capacity(self, entries: int) -> None:
+        """Fixes the force-list entries per particle (1..254); 0: adaptive sizing."""
+        check(self._lib.sph_world_set_list_capacity(self._h, entries))
+
+    @property
+    def list_capacity(self) -> int:
+        rc = self._lib.sph_world_list_capacity(self._h)
+        if rc < 0:
+            check(rc)
+        return rc
+
+    @property
+    def device_bytes(self) -> int:
+        return int(self._lib.sph_world_device_bytes(self._h))
+
     def set_debug(self, on: bool) -> None:
         check(self._lib.sph_world_set_debug(self._h, 1 if on else 0))
 

@@ -48,8 +48,8 @@ def test_budget_admission_queues_requests(gpu):
     """A budget that holds one scene at a time still completes every request."""
     from sph_hip.workloads import mixed_scenes
     specs = mixed_scenes(count=3, seed=50, n_min=1000, n_max=1000, extent=(3.0, 4.0))
-    # 1152 B per particle (136 B of state and slots, 254 four-byte list entries)
-    one = 1000 * 1152 + 1024 * 24 + (16 * 1000 + 4096) * 8  # < two scenes at any grid size
+    # 648 B per particle (136 B of state and slots, the default 128 four-byte list entries)
+    one = 1000 * 648 + 1024 * 24 + (16 * 1000 + 4096) * 8  # < two scenes at any grid size
     with gpu.Server(hbm_budget_bytes=one, max_scenes=8, max_particles=1 << 14,
                     max_table_cells=1 << 14, substeps=3) as srv:
         ids = [srv.submit(sp.params(), *sp.state(), 2) for sp in specs]
