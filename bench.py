@@ -516,6 +516,9 @@ def main() -> int:
     lo = np.array(params.min_xyz, np.float32)
     hi = np.array(params.max_xyz, np.float32)
     in_box = bool(ok and (p_out[:, :3] >= lo).all() and (p_out[:, :3] <= hi).all())
+    # device memory the world holds at the end (its force lists sized to the state)
+    memory = {"list_entries": world.list_capacity, "device_bytes": world.device_bytes,
+              "bytes_per_particle": world.device_bytes / W.n}
     world.close()
 
     mixed = mixed_fit = None
@@ -572,6 +575,7 @@ def main() -> int:
         "serving": serve,
         "state_finite": ok,
         "state_in_box": in_box,
+        "memory": memory,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

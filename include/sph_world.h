@@ -208,11 +208,12 @@ int sph_world_set_precision(sph_world *w, int mode);
 int sph_world_set_radix_bits(sph_world *w, int bits);
 /* Force neighbour lists (recorded by the density pass for the force pass; no reference
  * counterpart: the reference walks 27 cells in both passes).  A world starts with 64
- * entries per particle, an uploaded state raises that to four times the neighbours of a
+ * entries per particle, an uploaded state raises that to five times the neighbours of a
  * uniform fill of its bounding box (a dense start such as the reference's own scene gets
- * up to 254 at once), and at sph_world_sync / download_state (and when tuning) the lists
- * grow to hold the longest that did not fit since the last check (twice the capacity, or
- * that list plus a quarter, at most 254).  A particle whose list does not fit walks its
+ * up to 254 at once), and the lists grow to hold the longest that did not fit (twice the
+ * capacity, or that list plus a quarter, at most 254): sph_world_step looks between frames
+ * without waiting (a frame or two after the overflow it synchronises once and grows), and
+ * sph_world_sync / download_state and the tuning look as well.  A particle whose list does not fit walks its
  * cells in the force pass instead: results are the same bits at every capacity, only
  * speed and memory (4 B per entry per particle) change.
  *   sph_world_set_list_capacity: entries 1..254 fixes the capacity (reallocated now);
@@ -293,8 +294,8 @@ int sph_batch_pass_times(sph_batch *b, uint32_t substeps, sph_pass_times *out);
 /* sph_world_density_map for one scene of the batch (its own box and table range). */
 int sph_batch_density_map(sph_batch *b, uint32_t scene, float voxel_size, float *out, size_t count);
 /* Force-list capacity of the batch (one for all its scenes), as sph_world_set_list_capacity /
- * sph_world_list_capacity: sized from each uploaded scene and grown at sph_batch_sync /
- * download_scene unless fixed. */
+ * sph_world_list_capacity: sized from each uploaded scene and grown during sph_batch_step,
+ * at sph_batch_sync and at download_scene unless fixed. */
 int sph_batch_set_list_capacity(sph_batch *b, uint32_t entries);
 int sph_batch_list_capacity(const sph_batch *b);
 uint32_t sph_batch_particles(const sph_batch *b);

@@ -275,8 +275,10 @@ static int batch_autotune(sph_batch *b, uint32_t substeps) {
     for (;;) {
         int rc = batch_autotune_once(b, substeps);
         if (rc) return rc;
-        const int grew = batch_sync_lists(b);
-        if (grew <= 0) return grew;
+        const uint32_t cap_before = b->chain.b.list_cap;
+        const int resized = batch_sync_lists(b);
+        if (resized < 0) return resized;
+        if (b->chain.b.list_cap <= cap_before) return SPH_OK;
     }
 }
 
@@ -319,16 +321,22 @@ int sph_batch_step(sph_batch *b, uint32_t frames) {
         int rc = batch_autotune(b, 2);
         if (rc) return rc;
     }
-    if (!b->graph.exec) {
-        int rc = b->graph.ensure_begin(b->stream);
-        if (rc) return rc;
-        hipError_t le = b->chain.enqueue(b->stream, b->policy(), nullptr);
-        rc = b->graph.end(b->stream, le);
-        if (rc) return rc;
-    }
     for (uint64_t s = 0; s < substeps; ++s) {
+        if (!b->graph.exec) {
+            int rc = b->graph.ensure_begin(b->stream);
+            if (rc) return rc;
+            hipError_t le = b->chain.enqueue(b->stream, b->policy(), nullptr);
+            rc = b->graph.end(b->stream, le);
+            if (rc) return rc;
+        }
         hipError_t e = hipGraphLaunch(b->graph.exec, b->stream);
         if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+        // once a frame, without waiting: the lists resize as the last landed record asks
+        if (!b->list_pinned && (s + 1) % b->substeps == 0) {
+            const int rc = b->chain.poll_lists_async(b->stream);
+            if (rc < 0) return rc;
+            if (rc == 1) b->graph.drop();
+        }
     }
     b->stepped = true;
     return SPH_OK;

@@ -25,8 +25,11 @@ int validate_state(const sph_params &p, uint32_t n, const float *pos4, const flo
 uint32_t key_bits_for(uint64_t table_total);
 void speed_ramp(uint32_t n, const float *vel4, float *color4);
 
-// Force-list entries a chain starts with, and the fewest an uploaded state is given.
+// Force-list entries a chain starts with, the fewest an uploaded state is given and the
+// fewest the lists shrink to; records in a row without a list above half the capacity
+// before it halves.
 constexpr uint32_t kListStart = 64;
+constexpr uint32_t kListQuiet = 8;
 // Force-list entries for an uploaded state of n particles (vec4 positions).
 uint32_t list_entries_for_state(const sph_params &p, uint32_t n, const float *pos4);
 
@@ -51,7 +54,7 @@ struct Chain {
     uint64_t lookback_cap = 0; // look-back words allocated (fixes which widths are possible)
     uint32_t n_cap = 0, t_cap = 0, b_cap = 0, u_cap = 0, g_cap = 0;  // allocated capacity
     uint64_t bytes = 0;        // device memory allocated by allocate() and set_list_cap()
-    uint32_t *h_list = nullptr;  // pinned host word: the longest list that did not fit
+    uint32_t *h_list = nullptr;  // pinned host words: [0] the step calls' copy of the list record, [1] a sync's
 
     // Allocates for the given capacity (force lists of list_entries per slot) and
     // configures the chain to use all of it.
@@ -61,12 +64,25 @@ struct Chain {
     // buffer is allocated before the old one is freed, so on failure the chain keeps the
     // old capacity.  The caller has synchronised the chain's stream.
     int set_list_cap(uint32_t entries);
-    // Synchronises `s`.  With `adapt` (and lists shorter than kList), first queues the read
-    // of the longest force list that did not fit since the last call, and afterwards grows
-    // the lists to twice their capacity or that list plus a quarter, in steps of 32 entries
-    // (at most kList).  Returns 1 when the lists grew (a captured graph must be dropped), 0,
-    // or an error.
+    // Force-list sizing from the record the density passes leave (the longest list above
+    // half the capacity since the record was last read and cleared): `list_target` gives the
+    // capacity a record asks for (0: keep), twice
+    // the capacity or the list plus a quarter (in steps of 32, at most kList) past a list that
+    // did not fit, and half the capacity (at least kListStart) after kListQuiet records in a
+    // row with no list above half of it.
+    uint32_t list_target(uint32_t record);
+    // Synchronises `s`; with `adapt`, first queues the record's copy and afterwards resizes
+    // the lists as it asks.  Returns 1 when the capacity changed (a captured graph must be
+    // dropped), 0, or an error.
     int sync_lists(hipStream_t s, bool adapt);
+    // Between the frames of a step call, without waiting: looks at the record the copy
+    // queued at the previous call left (if it has landed) and queues the next copy; when the
+    // record asks for another capacity it synchronises and resizes.  Returns 1 when the
+    // capacity changed, 0, or an error.
+    int poll_lists_async(hipStream_t s);
+    uint32_t list_quiet = 0;        // records in a row with no list above half the capacity
+    hipEvent_t list_ev = nullptr;   // marks the landing of the queued record copy
+    bool list_poll_pending = false;
     // Uses the first n slots, table_total cells, brick_total bricks, bucket_total buckets
     // and grid_total halo-grid cells (within capacity).
     void configure(uint32_t n, uint32_t table_total, uint32_t brick_total, uint32_t bucket_total,

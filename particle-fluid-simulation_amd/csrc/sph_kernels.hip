@@ -201,16 +201,16 @@ void k_density(Scenes sc, const ChainBuffers b, uint32_t n) {
     uint32_t count, found;
     const float2 d = density_with_list<kU, kF>(c, j, s, b.pred_pos, grid_runs(c, b), s_list + threadIdx.x, kB,
                                            b.nlist, n, b.list_cap, count, found);
-    // Lists that did not fit the allocated capacity (their particles walk in the force
-    // pass): the longest is recorded for the host to size the lists.  The maximum is taken
-    // over the overflowing lanes only (all active), one atomic per wave that has any and
-    // none once a longer list is on record.
-    const uint64_t over = __ballot(found > b.list_cap);
-    if (over) {
+    // The longest list above half the allocated capacity, for the host to size the lists
+    // (grow past a list that did not fit, halve when none comes near): the maximum is taken
+    // over those lanes only (all active), one atomic per wave that has any and none once a
+    // longer list is on record.  The host clears the record each time it reads it.
+    const uint64_t longer = __ballot(found > (b.list_cap >> 1));
+    if (longer) {
         uint32_t longest = 0;
-        for (uint64_t m = over; m; m &= m - 1ull)
+        for (uint64_t m = longer; m; m &= m - 1ull)
             longest = max(longest, (uint32_t)__builtin_amdgcn_readlane((int)found, __ffsll((long long)m) - 1));
-        if ((threadIdx.x & 63u) == (uint32_t)(__ffsll((long long)over) - 1) &&
+        if ((threadIdx.x & 63u) == (uint32_t)(__ffsll((long long)longer) - 1) &&
             __hip_atomic_load(&b.counters[4], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < longest)
             atomicMax(&b.counters[4], longest);
     }

@@ -43,8 +43,8 @@ struct ChainBuffers {
     uint32_t *queue;        // [brick_total] non-empty bricks of this substep
     uint32_t *stray;        // [n] slots outside every brick grid
     uint32_t *counters;     // [0] queue length [1] strays [2] density drain [3] force drain
-                            // [4] entries of the longest force list that did not fit
-                            // list_cap since the host last cleared it (0: none)
+                            // [4] entries of the longest force list above list_cap / 2
+                            // since the host last read the record (0: none)
     // Cell-ordered layout (sph_layout.hip).
     uint32_t *bcount;       // [bucket_total] particles per bucket, then the bucket's first slot
     uint32_t *blookback;    // [bucket tiles] look-back words of the bucket scan

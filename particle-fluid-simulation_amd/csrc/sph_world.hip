@@ -236,12 +236,12 @@ int Chain::allocate(uint32_t n_, uint32_t table_total_, uint32_t brick_total_,
         bytes += r.bytes;
     }
     b.list_cap = list_entries;
-    if (hipHostMalloc((void **)&h_list, 4, 0) != hipSuccess) {
+    if (hipHostMalloc((void **)&h_list, 8, 0) != hipSuccess) {
         h_list = nullptr;
         release();
-        return fail(SPH_ERR_OUT_OF_MEMORY, "pinned host word");
+        return fail(SPH_ERR_OUT_OF_MEMORY, "pinned host words");
     }
-    *h_list = 0u;
+    h_list[0] = h_list[1] = 0u;
     hipError_t e = hipMemset(b.ghist, 0, kGhistWords * 4);
     if (e == hipSuccess) e = hipMemset(b.counters, 0, 8 * 4);
     if (e == hipSuccess) e = hipMemset(b.brick_flag, 0, (size_t)(brick_total ? brick_total : 1) * 4);
@@ -283,28 +283,74 @@ int Chain::set_list_cap(uint32_t entries) {
     return SPH_OK;
 }
 
+uint32_t Chain::list_target(uint32_t record) {
+    const uint32_t cap = b.list_cap;
+    if (record > cap) {
+        list_quiet = 0;
+        if (cap >= (uint32_t)kList) return 0u;
+        return std::min((uint32_t)kList, std::max(2u * cap, (record + record / 4u + 31u) / 32u * 32u));
+    }
+    if (record != 0u) {  // a list above half the capacity: keep it
+        list_quiet = 0;
+        return 0u;
+    }
+    if (++list_quiet < kListQuiet || cap <= kListStart) return 0u;
+    list_quiet = 0;
+    return std::max(kListStart, (cap / 2u + 31u) / 32u * 32u);
+}
+
 int Chain::sync_lists(hipStream_t s, bool adapt) {
-    adapt = adapt && h_list && b.list_cap < (uint32_t)kList;
-    hipError_t e = adapt ? hipMemcpyAsync(h_list, b.counters + 4, 4, hipMemcpyDeviceToHost, s) : hipSuccess;
+    adapt = adapt && h_list;
+    // h_list[1]: this read; h_list[0]: the copy a step call may have queued (it cleared the
+    // device record after copying, so both are needed)
+    hipError_t e = adapt ? hipMemcpyAsync(h_list + 1, b.counters + 4, 4, hipMemcpyDeviceToHost, s) : hipSuccess;
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    const bool polled = list_poll_pending;
+    list_poll_pending = false;  // any queued copy has landed
     if (!adapt) return 0;
-    const uint32_t longest = *h_list;
-    if (longest <= b.list_cap) return 0;
-    *h_list = 0u;
-    // the stream is idle: clear the record for the next read
-    e = hipMemsetAsync(b.counters + 4, 0, 4, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return hip_fail(e, "clearing the list record");
-    const uint32_t want = std::min((uint32_t)kList, std::max(2u * b.list_cap, (longest + longest / 4u + 31u) / 32u * 32u));
-    return set_list_cap(want) == SPH_OK ? 1 : 0;
+    const uint32_t record = std::max(h_list[1], polled ? h_list[0] : 0u);
+    if (h_list[1]) {
+        // the stream is idle: clear the record for the next read
+        e = hipMemsetAsync(b.counters + 4, 0, 4, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return hip_fail(e, "clearing the list record");
+    }
+    const uint32_t want = list_target(record);
+    return want && set_list_cap(want) == SPH_OK ? 1 : 0;
+}
+
+int Chain::poll_lists_async(hipStream_t s) {
+    if (!h_list) return 0;
+    if (!list_ev && hipEventCreateWithFlags(&list_ev, hipEventDisableTiming) != hipSuccess) {
+        list_ev = nullptr;
+        return 0;
+    }
+    if (list_poll_pending) {
+        if (hipEventQuery(list_ev) != hipSuccess) return 0;  // not landed yet
+        list_poll_pending = false;
+        const uint32_t want = list_target(h_list[0]);
+        if (want) {
+            const hipError_t e = hipStreamSynchronize(s);  // the lists are free to reallocate
+            if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+            return set_list_cap(want) == SPH_OK ? 1 : 0;
+        }
+    }
+    // the copy, then the clear (stream-ordered: the next substeps record afresh); neither is
+    // inside a captured graph
+    if (hipMemcpyAsync(h_list, b.counters + 4, 4, hipMemcpyDeviceToHost, s) == hipSuccess &&
+        hipMemsetAsync(b.counters + 4, 0, 4, s) == hipSuccess && hipEventRecord(list_ev, s) == hipSuccess)
+        list_poll_pending = true;
+    return 0;
 }
 
 uint32_t list_entries_for_state(const sph_params &p, uint32_t n, const float *pos4) {
-    // four times the neighbours of a uniform fill of the state's bounding box (each extent
+    // five times the neighbours of a uniform fill of the state's bounding box (each extent
     // at least one smoothing radius), in steps of 32, between kListStart and kList: a dense
     // start (a resting block, the reference's own scene) gets long lists at once instead of
-    // walking until the first synchronisation grows them
+    // walking until they grow.  Measured, the most neighbours a particle reaches in the
+    // next frames (a random fill relaxing, a dam breaking, an over-dense block expanding)
+    // stay within 4.9 times the mean at the start.
     float lo[3] = {pos4[0], pos4[1], pos4[2]}, hi[3] = {pos4[0], pos4[1], pos4[2]};
     for (uint32_t i = 1; i < n; ++i)
         for (int a = 0; a < 3; ++a) {
@@ -315,7 +361,7 @@ uint32_t list_entries_for_state(const sph_params &p, uint32_t n, const float *po
     double vol = 1.0;
     for (int a = 0; a < 3; ++a) vol *= std::max((double)hi[a] - lo[a], R);
     const double neighbours = (double)n / vol * (4.0 / 3.0) * M_PI * R * R * R;
-    const double want = std::ceil(4.0 * neighbours / 32.0) * 32.0;
+    const double want = std::ceil(5.0 * neighbours / 32.0) * 32.0;
     return (uint32_t)std::min<double>((double)kList, std::max<double>((double)kListStart, want));
 }
 
@@ -329,6 +375,10 @@ void Chain::release() {
         if (p) hipFree(p);
     if (h_list) (void)hipHostFree(h_list);
     h_list = nullptr;
+    if (list_ev) (void)hipEventDestroy(list_ev);
+    list_ev = nullptr;
+    list_poll_pending = false;
+    list_quiet = 0;
     b = ChainBuffers{};
     bytes = 0;
 }
@@ -623,9 +673,10 @@ int online_start(sph_world *w) {
 // the kernels, try the radix widths that need fewer passes with the fastest kernels.
 int online_finish_phase(sph_world *w) {
     auto &o = w->online;
-    const int grew = world_sync_lists(w);
-    if (grew < 0) return grew;
-    if (grew && o.phase == 0) {
+    const uint32_t cap_before = w->chain.b.list_cap;
+    const int resized = world_sync_lists(w);
+    if (resized < 0) return resized;
+    if (w->chain.b.list_cap > cap_before && o.phase == 0) {
         // the list configurations ran with lists too short for this state: time them again
         std::vector<int> codes = o.codes;
         return online_begin_phase(w, std::move(codes));
@@ -836,6 +887,12 @@ int sph_world_step(sph_world *w, uint32_t frames) {
         }
         hipError_t e = hipGraphLaunch(w->graph.exec, w->stream);
         if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+        // once a frame, without waiting: the lists resize as the last landed record asks
+        if (!w->list_pinned && (s + 1) % w->params.substeps == 0) {
+            const int rc = w->chain.poll_lists_async(w->stream);
+            if (rc < 0) return rc;
+            if (rc == 1) w->graph.drop();
+        }
     }
     w->stepped = true;
     w->grid_current = true;
@@ -930,9 +987,10 @@ int sph_world_autotune(sph_world *w, uint32_t substeps, int *chosen, double *ms_
         rc = world_autotune(w, substeps, &code, ms_per_substep);
         if (rc) return rc;
         // the list configurations' trials overflowed lists that have now grown: again
-        const int grew = world_sync_lists(w);
-        if (grew < 0) return grew;
-        if (!grew) break;
+        const uint32_t cap_before = w->chain.b.list_cap;
+        const int resized = world_sync_lists(w);
+        if (resized < 0) return resized;
+        if (w->chain.b.list_cap <= cap_before) break;
     }
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune_cache[tune_bucket(w->n, w->T, w->chain.brick_total, w->tolerance)] = code;

@@ -1,8 +1,9 @@
 """Force-list capacity on the GPU (sph_world_set_list_capacity, the adaptive sizing, the
 batch's and the server's capacities).  A particle whose neighbours do not fit its list
 walks its cells in the force pass, so every capacity gives the same bits as the oracle;
-uploaded states size the lists from their density, and lists that overflowed grow at the
-next synchronisation, in the middle of a trajectory and during online tuning."""
+uploaded states size the lists from their density, lists that overflowed grow (between
+frames of a step call, at a synchronisation, during online tuning) and lists that no
+particle comes near shrink."""
 import numpy as np
 import pytest
 
@@ -33,12 +34,13 @@ def test_short_lists_stay_bitexact(gpu, orc, entries):
 
 
 def test_upload_sizes_the_lists(gpu):
-    """64 entries at creation; an upload raises them to four times the neighbours of a
-    uniform fill of the state's bounding box: 64 for the 1M fill (~12 neighbours), 128 for
-    the dam break's resting block (~31), 254 for the reference's own scene (~216)."""
+    """64 entries at creation; an upload raises them to five times the neighbours of a
+    uniform fill of the state's bounding box, in steps of 32: 64 for the 1M fill (~12
+    neighbours), 160 for the dam break's resting block (~31), 254 for the reference's own
+    scene (~216)."""
     from sph_hip.workloads import DAM_BREAK_16K, UNIFORM_1M
     for W, p, want in ((UNIFORM_1M, UNIFORM_1M.params(table_cells=1 << 21), 64),
-                       (DAM_BREAK_16K, DAM_BREAK_16K.params(), 128)):
+                       (DAM_BREAK_16K, DAM_BREAK_16K.params(), 160)):
         with gpu.World(p, W.n) as w:
             assert w.list_capacity == 64
             w.upload(*W.state())
@@ -56,8 +58,9 @@ def test_upload_sizes_the_lists(gpu):
 
 def test_lists_grow_at_sync_and_stay_exact(gpu, orc):
     """The dam break from 32 entries on the adaptive path: lists overflow from the start
-    (up to ~54 neighbours at rest, ~100 while breaking), each sync grows them to hold the
-    longest seen, and the 12-frame trajectory stays bit-exact across the changes."""
+    (up to ~54 neighbours at rest, ~100 while breaking); the step call itself grows them
+    between frames once it sees a list that did not fit, a sync does the same, and the
+    12-frame trajectory stays bit-exact across the changes."""
     from sph_hip.workloads import DAM_BREAK_16K as W
     params = W.params()
     pos, vel = W.state()
@@ -74,8 +77,40 @@ def test_lists_grow_at_sync_and_stay_exact(gpu, orc):
         gp, gv = w.download()
         last = w.list_capacity
     print(f"lists 32 -> {first} -> {last}", flush=True)
-    assert first > 32 and last >= first
+    assert first > 32 and last > 32
     rp, rv = orc.step(params, pos, vel, 12)
+    _assert_bitexact("pos", gp, rp)
+    _assert_bitexact("vel", gv, rv)
+
+
+def test_lists_shrink_when_no_list_comes_near(gpu, orc):
+    """A fill at the 1M scene's number density (~12 neighbours) that starts one unit inside
+    every wall, so that no particle is clamped onto a wall (the integrate margin piles
+    particles that start nearer onto planes and edges, where lists run long), from 254
+    entries on the adaptive path, synchronising after every frame: once eight records in a row
+    show no list above half the capacity it halves, down to 64, and the 30-frame trajectory
+    stays bit-exact across the reallocations."""
+    import dataclasses
+    from sph_hip.workloads import UNIFORM_1M
+    W = dataclasses.replace(UNIFORM_1M, name="inner_block", n=12480, box_max=(10.0, 5.0, 10.0),
+                            block_lo=(1.0, 1.0, 1.0), block_hi=(9.0, 4.0, 9.0))
+    p = W.params(table_cells=1 << 14)
+    pos, vel = W.state()
+    seen = []
+    with gpu.World(p, pos.shape[0]) as w:
+        w.set_lane_config(3)
+        w.upload(pos, vel)
+        w.set_list_capacity(254)
+        w.set_list_capacity(0)  # adaptive again, from 254 entries
+        for f in range(30):  # an interactive host: one frame, then a synchronisation
+            w.step(1)
+            w.sync()
+            if f % 5 == 4:
+                seen.append(w.list_capacity)
+        gp, gv = w.download()
+    print(f"list capacity every 5 frames: {seen}", flush=True)
+    assert seen[-1] == 64 and seen[0] > seen[-1]
+    rp, rv = orc.step(p, pos, vel, 30)
     _assert_bitexact("pos", gp, rp)
     _assert_bitexact("vel", gv, rv)
 
