@@ -100,6 +100,16 @@ __device__ __forceinline__ void map_walk(const SceneConst &c, const float4 *__re
 // box and span at most two cell columns, their cells are walked once for all of them;
 // otherwise (a voxel centre outside the box, or voxels spread over three columns) each is
 // walked alone.
+// The positions a per-lane substep wrote back to the hash slots, gathered into cell order
+// for the walk over the cell-ordered runs: cell slot j of the scene takes hash slot hslot[j].
+__global__ __launch_bounds__(kBlock) void k_gather_cell_positions(const float4 *__restrict__ pos,
+                                                                  const uint32_t *__restrict__ hslot,
+                                                                  float4 *__restrict__ out, uint32_t begin,
+                                                                  uint32_t end) {
+    const uint32_t j = begin + blockIdx.x * kBlock + threadIdx.x;
+    if (j < end) out[j] = pos[hslot[j]];
+}
+
 template <class Runs, int K>
 __global__ __launch_bounds__(kBlock) void k_density_map(SceneConst c, const float4 *__restrict__ pos,
                                                        Runs runs, MapGeometry g,
@@ -208,7 +218,14 @@ hipError_t launch_density_map(hipStream_t stream, const SceneConst &c, const Cha
             hipLaunchKernelGGL((k_density_map<R, 1>), dim3((uint32_t)blocks), dim3(kBlock), 0, stream, c, p, runs, g, out);
     };
     if (chain.cell_order) {
-        // hz = 0 (no grid): every lookup goes through the key's cell-ordered run
+        // the scene's new positions in cell order (the runs and grid of the last substep
+        // index cell slots), then the walk; hz = 0 (no grid): every lookup goes through the
+        // key's cell-ordered run
+        if (c.p_end > c.p_begin) {
+            const uint32_t gblocks = (c.p_end - c.p_begin + kBlock - 1u) / kBlock;
+            hipLaunchKernelGGL(k_gather_cell_positions, dim3(gblocks), dim3(kBlock), 0, stream,
+                               (const float4 *)b.pos, b.hslot, b.pos_sorted, c.p_begin, c.p_end);
+        }
         launch(GridRuns{b.grid, c.grid_base, (uint32_t)c.gx + 2u, (uint32_t)c.gy + 2u,
                         c.grid_cells && grid_current ? (uint32_t)c.gz + 2u : 0u, c.table_base, c.mask, b.crun},
                (const float4 *)b.pos_sorted);

@@ -264,9 +264,6 @@ __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4))) void k_
     }
     b.pos[o] = po;
     b.vel[o] = vo;
-    // the new position also in cell order, for a density map of this substep (no lane reads
-    // pos_sorted but its own, before this store; the next placement overwrites it)
-    b.pos_sorted[j] = po;
 }
 
 inline uint32_t blocks_for(uint32_t n) { return (n + kBlock - 1u) / kBlock; }

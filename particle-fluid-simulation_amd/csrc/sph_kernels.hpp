@@ -27,7 +27,8 @@ struct ChainBuffers {
     uint32_t *sort_v[2];
     float4 *pred_pos;    // predicted positions, sorted order (w: near density once the
                          // list-recording density pass has run, else 1)
-    float4 *pos_sorted;  // positions, sorted order (after a per-lane force pass: the new ones)
+    float4 *pos_sorted;  // positions, sorted order (a density map after a per-lane substep
+                         // gathers the new ones into it first)
     float4 *vel_sorted;  // velocities, sorted order (w: density once the list-recording
                          // density pass has run)
     float2 *dens;        // (density, near density), sorted order
