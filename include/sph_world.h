@@ -173,7 +173,8 @@ typedef enum sph_dump_stage {
 typedef enum sph_kernel_variant {
     SPH_KERNEL_AUTO = 0,
     SPH_KERNEL_DIRECT = 1,
-    SPH_KERNEL_BRICK = 2
+    SPH_KERNEL_BRICK = 2,
+    SPH_KERNEL_TILE = 3
 } sph_kernel_variant;
 int sph_world_set_kernel(sph_world *w, int variant);
 /* Runs the measurement now (`substeps` timed substeps per variant, state restored) and
@@ -189,6 +190,22 @@ int sph_world_kernel_config(const sph_world *w, char *buf, size_t bytes);
  * testing every configuration.  All configurations give bit-identical results. */
 int sph_world_set_lane_config(sph_world *w, int config);
 int sph_lane_configs(void);
+/* Selects TILE with one fixed tile configuration, 0 <= config < sph_tile_configs() (the
+ * numbering of SPH_TILE_CONFIG): 0 = 256-lane workgroups staging up to 1280 entries, 1 =
+ * 128 lanes and 768 entries, 2 = configuration 0 with the approximate square root and
+ * reciprocal (needs SPH_PRECISION_TOLERANCE).  The tiled kernels stage each tile's cells
+ * and their neighbours in LDS and give bit-identical results to every other variant; the
+ * tile shape follows the particles per cell of the last uploaded state.  Needs a box of at
+ * most 64n + 65536 cells.  No counterpart in the reference (tuning). */
+int sph_world_set_tile_config(sph_world *w, int config);
+int sph_tile_configs(void);
+/* What the last substep's tiled passes did (all 0 when it ran other kernels): out[0] tiles,
+ * out[1] strays (particles walked one lane each through global memory: those no tile owns
+ * and those of handed-over chunks), out[2] tiles split into chunks along x (their staged
+ * region exceeded the LDS capacity), out[3] chunks handed over to the strays (a single
+ * column still too large, or more particles than the workgroup takes).  Synchronises the
+ * world's stream.  No counterpart in the reference (diagnostics). */
+int sph_world_tile_stats(sph_world *w, uint32_t out[4]);
 
 /* Precision of the neighbour passes.  EXACT (the default): every float operation is the
  * reference's, correctly rounded in its order, and results are bit-identical to the FP32

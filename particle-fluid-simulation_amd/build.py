@@ -24,7 +24,7 @@ OBJDIR = os.path.join(HERE, "_build")
 LIBNAME = "libsph_hip.so"
 ARCH = "gfx950"
 
-SOURCES = ["sph_kernels.hip", "sph_brick.hip", "radix_sort.hip", "sph_layout.hip", "sph_density_map.hip",
+SOURCES = ["sph_kernels.hip", "sph_tiles.hip", "sph_brick.hip", "radix_sort.hip", "sph_layout.hip", "sph_density_map.hip",
            "sph_world.hip", "sph_batch.hip", "sph_server.hip", "sph_host.hip", "sph_noalloc.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -44,6 +44,14 @@ struct SceneConst {
     // box is large for the particle count) and every lookup goes through the hash table.
     uint32_t grid_base;
     uint32_t grid_cells;
+    // Cell tiles of the tiled neighbour passes: tiles of tw x th x td cells cover the box
+    // grid, ntx x nty x ntz of them, numbered x-fastest from tile_base.  use_tiles = 0: the
+    // scene's particles take the per-lane code inside the tiled launches.
+    uint32_t tile_base;
+    uint32_t tw, th, td;
+    uint32_t ntx, nty, ntz;
+    uint32_t use_tiles;
+    uint32_t tile_emax;  // staged entries beyond which a tile's particles become strays
 };
 
 // cellCoordToHash (updateCellEntries.comp:36-52): u32 arithmetic, wraps mod 2^32.
@@ -111,6 +119,7 @@ struct OneScene {
     __device__ __forceinline__ const SceneConst &scene_of_brick(uint32_t) const { return c; }
     __device__ __forceinline__ const SceneConst &scene_of_grid(uint32_t) const { return c; }
     __device__ __forceinline__ const SceneConst &scene_of_grid_near(uint32_t, uint32_t) const { return c; }
+    __device__ __forceinline__ const SceneConst &scene_of_tile(uint32_t) const { return c; }
 };
 
 struct ManyScenes {
@@ -134,6 +143,15 @@ struct ManyScenes {
         while (hi - lo > 1u) {
             const uint32_t mid = (lo + hi) >> 1;
             if (table[mid].grid_base <= g) lo = mid; else hi = mid;
+        }
+        return table[lo];
+    }
+    // Scene whose tile range [tile_base, next tile_base) holds global tile id `t`.
+    __device__ __forceinline__ SceneConst scene_of_tile(uint32_t t) const {
+        uint32_t lo = 0, hi = count;
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (table[mid].tile_base <= t) lo = mid; else hi = mid;
         }
         return table[lo];
     }

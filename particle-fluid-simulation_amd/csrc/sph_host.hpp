@@ -32,6 +32,11 @@ constexpr uint32_t kListStart = 64;
 constexpr uint32_t kListQuiet = 8;
 // Force-list entries for an uploaded state of n particles (vec4 positions).
 uint32_t list_entries_for_state(const sph_params &p, uint32_t n, const float *pos4);
+// Particles per cell over the bounding box of an uploaded state (sizes the cell tiles).
+double particles_per_cell_for_state(const sph_params &p, uint32_t n, const float *pos4);
+// Workgroups of a tiled launch that take the strays: `untiled` particles of scenes without
+// tiles among n.
+uint32_t stray_blocks_for(uint32_t untiled, uint32_t n, int block);
 
 // hipEvents bracketing the five passes of one substep.
 struct PassEvents {
@@ -50,6 +55,9 @@ struct Chain {
     bool bricks = false;       // density/force through the brick kernels (else per lane)
     bool cell_order = true;    // the last enqueued substep left its arrays in cell order
     int lane_cfg = kDefaultLaneConfig;  // launch configuration of the per-lane kernels
+    int tile_cfg = -1;         // configuration of the tiled kernels, -1: per-lane kernels
+    uint32_t tile_total = 0;   // tiles over all scenes (tiled kernels)
+    uint32_t stray_blocks = 8; // workgroups of the tiled launches that take the strays
     uint32_t radix_bits = 8;   // radix digit width (8, 11 or 12)
     uint64_t lookback_cap = 0; // look-back words allocated (fixes which widths are possible)
     uint32_t n_cap = 0, t_cap = 0, b_cap = 0, u_cap = 0, g_cap = 0;  // allocated capacity
@@ -97,6 +105,9 @@ struct Chain {
     void release();
     const uint32_t *sorted_keys() const;
     const uint32_t *sorted_idx() const;  // nullptr means identity (no sort pass ran)
+
+    // The neighbour passes run the tiled kernels.
+    bool tiled() const { return !bricks && tile_cfg >= 0 && tile_total > 0; }
 
     template <class Scenes>
     hipError_t enqueue(hipStream_t s, const Scenes &sc, PassEvents *ev);

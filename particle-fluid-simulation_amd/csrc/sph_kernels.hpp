@@ -46,6 +46,8 @@ struct ChainBuffers {
     uint32_t *counters;     // [0] queue length [1] strays [2] density drain [3] force drain
                             // [4] entries of the longest force list above list_cap / 2
                             // since the host last read the record (0: none)
+                            // [5] tiles split into chunks [6] chunks handed over to the strays
+                            // (the last tiled density pass)
     // Cell-ordered layout (sph_layout.hip).
     uint32_t *bcount;       // [bucket_total] particles per bucket, then the bucket's first slot
     uint32_t *blookback;    // [bucket tiles] look-back words of the bucket scan
@@ -57,7 +59,15 @@ struct ChainBuffers {
     uint32_t *nlist;        // [list_cap][n]: entry k of slot j at nlist[k * n + j]
     uint8_t *ncount;        // [n] entries, or 0xFF: the force pass walks the cells
     uint32_t list_cap;      // entries per slot allocated at nlist (<= kList)
+    // Tiled neighbour passes (sph_tiles.hip): per cell-ordered slot, the tile that owns the
+    // particle and the region cell of its home copy, or (kNoTile, 0) for a stray, which is
+    // listed at `stray` (count in counters[1]).  Owned particles' force lists hold 16-bit
+    // indices into the tile's staged region (entry k of slot j at nlist16()[k * n + j]).
+    uint2 *towner;
+    __host__ __device__ uint16_t *nlist16() const { return reinterpret_cast<uint16_t *>(nlist); }
 };
+
+constexpr uint32_t kNoTile = 0xFFFFFFFFu;
 
 uint32_t radix_tile_size();
 // tile_ctr[0..3]: radix passes; [4]: bucket scan.
@@ -96,7 +106,7 @@ hipError_t launch_bucket_scan(hipStream_t stream, const ChainBuffers &b, uint32_
 template <class Scenes>
 hipError_t launch_place(hipStream_t stream, const Scenes &sc, const ChainBuffers &b,
                         const uint32_t *sorted_keys, const uint32_t *sorted_idx, uint32_t n,
-                        uint32_t grid_total);
+                        uint32_t grid_total, bool tiles);
 
 // Launch configurations of the one-lane-per-particle density / force kernels: candidates
 // loaded per scan step, workgroup size, and whether the density pass records force
@@ -123,6 +133,30 @@ hipError_t launch_density(hipStream_t stream, const Scenes &sc, const ChainBuffe
 template <class Scenes>
 hipError_t launch_force(hipStream_t stream, const Scenes &sc, const ChainBuffers &b,
                         uint32_t n, int cfg);
+
+// Tiled neighbour passes (sph_tiles.hip).  A workgroup stages the cell runs of its tile
+// grown by one cell per side in LDS and evaluates the particles the tile owns (predicted
+// cell inside the tile, recorded by launch_place); the first `stray_blocks` workgroups walk
+// the strays through global memory.  Configurations: workgroup lanes and LDS entries.
+struct TileConfig {
+    int block;
+    int entries;  // staged entries per tile (a tile with more falls back to global walks)
+    bool tolerance;
+};
+constexpr int kNumTileConfigs = 3;
+constexpr int kNumExactTileConfigs = 2;
+extern const TileConfig kTileConfigs[kNumTileConfigs];
+// Region cells a tile may have ((tw + 2) (th + 2) (td + 2)).
+constexpr uint32_t kTileRegionMax = 768;
+// Tile dimensions for a scene whose occupied cells hold `ppc` particles on average.
+void tile_dims_for(SceneConst &c, double ppc, int cfg);
+uint32_t scene_tiles(const SceneConst &c);
+template <class Scenes>
+hipError_t launch_density_tile(hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n,
+                               uint32_t tile_total, uint32_t stray_blocks, int cfg);
+template <class Scenes>
+hipError_t launch_force_tile(hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n,
+                             uint32_t tile_total, uint32_t stray_blocks, int cfg);
 
 // Brick path (sph_brick.hip): compacts the non-empty bricks into the work queue, then
 // persistent workgroups drain it for the density and the force + integrate passes.

@@ -26,6 +26,7 @@
 // order in which the atomics were served.
 #include "sph_device.hpp"
 #include "sph_kernels.hpp"
+#include "sph_tile_owner.hpp"
 
 namespace sph {
 
@@ -128,7 +129,8 @@ __global__ __launch_bounds__(kBlock) void k_place(
     const uint32_t *__restrict__ bstart, const uint2 *__restrict__ run_info,
     float4 *__restrict__ pos_sorted, float4 *__restrict__ vel_sorted, float4 *__restrict__ pred_pos,
     float4 *__restrict__ dbg_pred_vel, float4 *__restrict__ dbg_vel_force, uint32_t *__restrict__ hslot,
-    int2 *__restrict__ crun) {
+    int2 *__restrict__ crun, uint2 *__restrict__ towner, uint32_t *__restrict__ stray,
+    uint32_t *__restrict__ counters) {
     const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
     if (j >= n) return;
     const uint32_t key = skeys[j];
@@ -151,6 +153,21 @@ __global__ __launch_bounds__(kBlock) void k_place(
     // the list-recording density pass reuses vel_sorted[].w for the density
     if (dbg_vel_force) dbg_vel_force[j2] = make_float4(0.0f, 0.0f, 0.0f, V.w);
     hslot[j2] = j;
+    if (towner) {
+        const uint2 own = tile_owner(c, make_float3(P.x, P.y, P.z), p);
+        towner[j2] = own;
+        // strays are listed for the per-lane code of the tiled launches; one atomic per wave
+        const bool is_stray = own.x == kNoTile;
+        const uint64_t m = __ballot(is_stray);
+        if (m) {
+            const int lane = threadIdx.x & 63;
+            const int leader = __ffsll((long long)m) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(&counters[1], (uint32_t)__popcll(m));
+            base = __shfl(base, leader, 64);
+            if (is_stray) stray[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = j2;
+        }
+    }
 }
 
 // The halo grid: each cell's key's cell-ordered run (one lookup per cell).
@@ -182,10 +199,11 @@ hipError_t launch_bucket_scan(hipStream_t stream, const ChainBuffers &b, uint32_
 template <class Scenes>
 hipError_t launch_place(hipStream_t stream, const Scenes &sc, const ChainBuffers &b,
                         const uint32_t *sorted_keys, const uint32_t *sorted_idx, uint32_t n,
-                        uint32_t grid_total) {
+                        uint32_t grid_total, bool tiles) {
     hipLaunchKernelGGL(k_place<Scenes>, dim3(blocks_for(n)), dim3(kBlock), 0, stream, sc, sorted_keys,
                        sorted_idx, n, b.pos, b.vel, b.table, b.bcount, b.run_info, b.pos_sorted,
-                       b.vel_sorted, b.pred_pos, b.dbg_pred_vel, b.dbg_vel_force, b.hslot, b.crun);
+                       b.vel_sorted, b.pred_pos, b.dbg_pred_vel, b.dbg_vel_force, b.hslot, b.crun,
+                       tiles ? b.towner : nullptr, b.stray, b.counters);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || grid_total == 0) return e;
     hipLaunchKernelGGL(k_grid<Scenes>, dim3(blocks_for(grid_total)), dim3(kBlock), 0, stream, sc,
@@ -195,7 +213,7 @@ hipError_t launch_place(hipStream_t stream, const Scenes &sc, const ChainBuffers
 
 #define SPH_LAYOUT_INSTANTIATE(S)                                                                \
     template hipError_t launch_place<S>(hipStream_t, const S &, const ChainBuffers &,            \
-                                        const uint32_t *, const uint32_t *, uint32_t, uint32_t);
+                                        const uint32_t *, const uint32_t *, uint32_t, uint32_t, bool);
 
 SPH_LAYOUT_INSTANTIATE(OneScene)
 SPH_LAYOUT_INSTANTIATE(ManyScenes)

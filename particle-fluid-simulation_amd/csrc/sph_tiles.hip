@@ -1,0 +1,733 @@
+// Tiled density and force passes: calculateDensities.comp:96-155 and
+// applyPressureAndViscosityForce.comp:275-337 + updateVelocities.comp:71-119 over cell tiles
+// staged in LDS.
+//
+// One workgroup per tile (sph_tile_owner.hpp).  It looks up the cell-ordered run of every
+// cell of the tile's region (the tile grown by one cell per side) in the halo grid, stages
+// the runs in LDS in region order (x fastest), and selects the particles the tile owns from
+// the staged copies.  For an owned particle the three cells (x - 1 .. x + 1) of each of its
+// nine (dz, dy) rows are consecutive region cells, so each row of the 27-cell walk is one
+// contiguous LDS range, visited in the reference's order.
+//
+// Density: a row is first scanned with the distance test only, and the candidates that
+// pass (d2 <= R2_pre) are queued per lane in LDS; the queue is drained in the order it was
+// filled (after a row once some lane holds kTFlush entries, when a lane's queue is full,
+// and at the end) with the density terms and the force-list entries.  Every particle's sums
+// thus take the reference's candidates in the reference's order, bit for bit, while the
+// in-range arithmetic runs with most lanes of the wave active.  The force-list entries are
+// 16-bit indices into the staged region; the force pass stages the same region (position
+// + near density, velocity + density per entry) and reads its neighbours from LDS.
+//
+// A tile whose region holds more than its scene's tile_emax entries (at most kE), or that
+// owns more than kOwnMax particles, hands its particles over to the stray list; the strays
+// (those and the particles no tile owns) then take one lane each in the stray kernels that
+// follow each tiled pass, walking their cells through global memory.
+#include <algorithm>
+
+#include "sph_device.hpp"
+#include "sph_kernels.hpp"
+#include "sph_pairs.hpp"
+#include "sph_tile_owner.hpp"
+
+namespace sph {
+
+namespace {
+
+constexpr int kTQ = 24;     // density candidate queue entries per lane
+constexpr int kTFlush = 16; // drain once some lane holds this many
+
+__device__ __forceinline__ GridRuns tile_grid_runs(const SceneConst &c, const ChainBuffers &b) {
+    return GridRuns{b.grid, c.grid_base, (uint32_t)c.gx + 2u, (uint32_t)c.gy + 2u,
+                    c.grid_cells ? (uint32_t)c.gz + 2u : 0u, c.table_base, c.mask, b.crun};
+}
+
+// Workgroups are handed to the 8 XCDs round-robin; this renumbering gives each XCD one
+// contiguous range of tiles (x-fastest numbering: slabs of neighbouring rows), so that the
+// runs neighbouring tiles share stay in that XCD's L2.  `b` counts from a multiple of 8.
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t nb) {
+    constexpr uint32_t kXcds = 8;
+    const uint32_t q = nb / kXcds, r = nb % kXcds, x = b % kXcds;
+    return x * q + min(x, r) + b / kXcds;
+}
+
+// Looks up the run of every region cell: s_beg[rc] = its first cell-ordered slot,
+// s_off[rc] = exclusive prefix of the run lengths in region order, s_off[cells] = total.
+// Ends with a barrier; returns the total.
+template <int kB>
+__device__ __forceinline__ uint32_t region_runs(const SceneConst &c, const ChainBuffers &b, const TileRegion &rg,
+                                                int *s_off, int *s_beg, uint32_t *s_wave) {
+    constexpr uint32_t kPer = (kTileRegionMax + kB - 1u) / kB;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+    const GridRuns runs = tile_grid_runs(c, b);
+    int cnt[kPer];
+    uint32_t sum = 0;
+    // region coordinates of the lane's first cell, then stepped x-fastest
+    uint32_t cxr = (tid * kPer) % rg.rx, yz = (tid * kPer) / rg.rx;
+    uint32_t cyr = yz % rg.ry, czr = yz / rg.ry;
+    int2 se[kPer];
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+        se[k] = make_int2(-1, -1);
+        if (tid * kPer + k < rg.cells)
+            se[k] = runs.raw(rg.ox + (int32_t)cxr, rg.oy + (int32_t)cyr, rg.oz + (int32_t)czr);
+        if (++cxr == rg.rx) {
+            cxr = 0;
+            if (++cyr == rg.ry) {
+                cyr = 0;
+                ++czr;
+            }
+        }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+        const uint32_t rc = tid * kPer + k;
+        int len = 0;
+        if (rc < rg.cells && se[k].x >= 0) {
+            len = se[k].y - se[k].x;
+            s_beg[rc] = se[k].x;
+        }
+        cnt[k] = len;
+        sum += (uint32_t)len;
+    }
+    uint32_t x = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    if (lane == 63u) s_wave[w] = x;
+    __syncthreads();
+    uint32_t wbase = 0, total = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < (uint32_t)kB / 64u; ++i) {
+        if (i < w) wbase += s_wave[i];
+        total += s_wave[i];
+    }
+    uint32_t run = wbase + x - sum;
+#pragma unroll
+    for (uint32_t k = 0; k < kPer; ++k) {
+        const uint32_t rc = tid * kPer + k;
+        if (rc < rg.cells) s_off[rc] = (int)run;
+        run += (uint32_t)cnt[k];
+    }
+    if (tid == 0) s_off[rg.cells] = (int)total;
+    __syncthreads();
+    return total;
+}
+
+// Region cell of staged entry e (s_off[rc] <= e < s_off[rc + 1]).
+__device__ __forceinline__ uint32_t region_cell_of(const int *s_off, uint32_t cells, int e) {
+    uint32_t lo = 0, hi = cells;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_off[mid] <= e) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Squared distance for the density pass's candidate filter: fused multiply-adds, within 6
+// ulp of dist2 (each of the two evaluations rounds three times, every term non-negative).
+__device__ __forceinline__ float dist2_filter(const float4 &p, const float4 &s) {
+    const float ex = p.x - s.x, ey = p.y - s.y, ez = p.z - s.z;
+    return fmaf(ez, ez, fmaf(ey, ey, ex * ex));
+}
+
+// Records the particle as owned when this staged copy is its home copy: one LDS atomic per
+// wave, so a wave's owned particles keep their region order.
+template <int kOwnMax>
+__device__ __forceinline__ void select_own(bool own, uint32_t e, uint32_t g, uint32_t *s_count,
+                                           uint16_t *s_own_e, uint32_t *s_own_g) {
+    const uint64_t m = __ballot(own);
+    if (!m) return;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1u;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(s_count, (uint32_t)__popcll(m));
+    base = __shfl(base, (int)leader, 64);
+    const uint32_t i = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    if (own && i < (uint32_t)kOwnMax) {
+        s_own_e[i] = (uint16_t)e;
+        if (s_own_g) s_own_g[i] = g;
+    }
+}
+
+// Density of slot j through global memory (strays, tiles too large for LDS); no force list.
+template <bool kF>
+__device__ __forceinline__ void density_global(const SceneConst &c, const ChainBuffers &b, uint32_t j) {
+    const float4 s = b.pred_pos[j];
+    const DensityCellsOf<GridRuns> cells{tile_grid_runs(c, b), sample_axis(s.x, c.minx, c.maxx, c.R),
+                                         sample_axis(s.y, c.miny, c.maxy, c.R),
+                                         sample_axis(s.z, c.minz, c.maxz, c.R)};
+    const float2 d = density_walk_rows<2, kF>(c, s, b.pred_pos, cells);
+    b.dens[j] = d;
+    reinterpret_cast<float *>(b.vel_sorted + j)[3] = d.x;
+    reinterpret_cast<float *>(b.pred_pos + j)[3] = d.y;
+    b.ncount[j] = kListInvalid;
+}
+
+// Force terms of slot j whose neighbours are read from global memory: the 27-cell walk
+// queueing in-range candidates in LDS (q, stride `stride`) as the per-lane kernels do, for
+// strays, tiles too large for LDS and lists that did not fit.
+struct ForceTerms {
+    float3 Fp, Fv;
+};
+template <bool kF>
+__device__ __forceinline__ ForceTerms force_global(const SceneConst &c, const ChainBuffers &b, const ForceSelf &me,
+                                                   uint32_t *q, int stride) {
+    ForceTerms t{make_float3(0.0f, 0.0f, 0.0f), make_float3(0.0f, 0.0f, 0.0f)};
+    const ForceCellsOf<GridRuns> cells{tile_grid_runs(c, b), force_cell(c, me.s)};
+    force_walk_rows<2, kF>(c, me, b.pred_pos, b.vel_sorted, b.dens, cells, q, stride, t.Fp, t.Fv);
+    return t;
+}
+
+// Integrate slot j after its force terms and store the new state at its hash slot.
+__device__ __forceinline__ void force_store(const SceneConst &c, const ChainBuffers &b, uint32_t j, const float4 &P,
+                                            const float4 &V, float density, const float3 &Fp, const float3 &Fv) {
+    float4 po, vo, vf;
+    force_finish(c, P, V, density, Fp, Fv, po, vo, vf);
+    const uint32_t o = b.hslot[j];
+    if (b.dbg_vel_force) {
+        vf.w = b.dbg_vel_force[j].w + 1.0f;
+        b.dbg_vel_force[j] = vf;
+    }
+    b.pos[o] = po;
+    b.vel[o] = vo;
+}
+
+// The longest force list above half the allocated capacity, for the host's list sizing
+// (as the per-lane density pass records it).
+__device__ __forceinline__ void record_longest(const ChainBuffers &b, uint32_t found) {
+    const uint64_t longer = __ballot(found > (b.list_cap >> 1));
+    if (!longer) return;
+    uint32_t longest = 0;
+    for (uint64_t m = longer; m; m &= m - 1ull)
+        longest = max(longest, (uint32_t)__builtin_amdgcn_readlane((int)found, __ffsll((long long)m) - 1));
+    if ((threadIdx.x & 63u) == (uint32_t)(__ffsll((long long)longer) - 1) &&
+        __hip_atomic_load(&b.counters[4], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < longest)
+        atomicMax(&b.counters[4], longest);
+}
+
+// Own-selection and staging of a tile, shared by both passes: run lookups, then (when the
+// region fits kE entries) every staged entry through `stage(e, g)` and the home copies of the
+// owned particles into s_own_e (staged index) and s_own_g (slot); else the owned slots alone
+// into s_glob.  Returns the region's entry count; *owned_out the owned particles (more than
+// kOwnMax: the tile takes the global path).
+// Chunks of a tile.  A tile whose staged region would exceed the scene's tile_emax entries
+// is split along x: chunk [a, b) owns the particles whose predicted cell has region x in
+// [a, b) and stages region columns a - 1 .. b (its cells grown by one per side), as many
+// columns as fit.  A chunk of one column that still does not fit, or that owns more than
+// kOwnMax particles, hands its particles over to the stray kernels.  Both passes plan the
+// same chunks from the same run lengths, so a force list's indices into a chunk's staging
+// stay valid.
+constexpr int kChunkMax = 66;  // tw <= 64
+constexpr int kRowMax = 16;    // (th + 2) (td + 2) <= 16
+
+struct TileLds {
+    int s_off[kTileRegionMax + 1];  // exclusive prefix of the region cells' run lengths
+    int s_beg[kTileRegionMax];      // first cell-ordered slot of each region cell's run
+    int s_col[kChunkMax];           // entries per region column (x)
+    int s_rowoff[kRowMax + 1];      // the current chunk's rows: first staged index
+    int s_delta[kRowMax];           // staged index - region index, per row
+    uint8_t s_ca[kChunkMax], s_cb[kChunkMax], s_cover[kChunkMax];
+    uint32_t s_wave[16];
+    uint32_t s_nch, s_count;
+};
+
+template <int kB>
+__device__ __forceinline__ void plan_chunks(const TileRegion &rg, int total, int emax, TileLds &L) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t rows = rg.ry * rg.rz;
+    if (total <= emax) {
+        if (tid == 0) {
+            L.s_ca[0] = 1;
+            L.s_cb[0] = (uint8_t)(rg.rx - 1u);
+            L.s_cover[0] = 0;
+            L.s_nch = 1;
+        }
+        __syncthreads();
+        return;
+    }
+    for (uint32_t x = tid; x < rg.rx; x += kB) {
+        int sum = 0;
+        for (uint32_t r = 0; r < rows; ++r) sum += L.s_off[r * rg.rx + x + 1u] - L.s_off[r * rg.rx + x];
+        L.s_col[x] = sum;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const uint32_t tw = rg.rx - 2u;
+        uint32_t nch = 0;
+        for (uint32_t a = 1; a <= tw;) {
+            int e = L.s_col[a - 1u] + L.s_col[a] + L.s_col[a + 1u];
+            uint32_t b = a + 1u;
+            while (b <= tw && e + L.s_col[b + 1u] <= emax) e += L.s_col[++b];
+            L.s_ca[nch] = (uint8_t)a;
+            L.s_cb[nch] = (uint8_t)b;
+            L.s_cover[nch] = e > emax;
+            ++nch;
+            a = b;
+        }
+        L.s_nch = nch;
+    }
+    __syncthreads();
+}
+
+// Staging layout of chunk [a, b): per region row its columns a - 1 .. b, rows in region
+// order.  Returns the chunk's entry count; ends with a barrier.
+__device__ __forceinline__ int chunk_rows(const TileRegion &rg, uint32_t a, uint32_t b, TileLds &L) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t rows = rg.ry * rg.rz;
+    if (tid < 64u) {
+        const uint32_t r = tid;
+        int len = 0, first = 0;
+        if (r < rows) {
+            first = L.s_off[r * rg.rx + a - 1u];
+            len = L.s_off[r * rg.rx + b + 1u] - first;
+        }
+        int x = len;
+#pragma unroll
+        for (int o = 1; o < 32; o <<= 1) {
+            const int y = __shfl_up(x, o, 64);
+            if ((int)r >= o) x += y;
+        }
+        if (r < rows) {
+            L.s_rowoff[r] = x - len;
+            L.s_delta[r] = x - len - first;
+            if (r == rows - 1u) L.s_rowoff[rows] = x;
+        }
+    }
+    __syncthreads();
+    return L.s_rowoff[rows];
+}
+
+// Region cell and slot of staged entry e of chunk [a, b).
+__device__ __forceinline__ void chunk_entry(const TileRegion &rg, uint32_t a, uint32_t b, const TileLds &L, int e,
+                                            uint32_t &rc, uint32_t &g) {
+    const uint32_t rows = rg.ry * rg.rz;
+    uint32_t lo = 0, hi = rows;  // s_rowoff[lo] <= e < s_rowoff[hi]
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (L.s_rowoff[mid] <= e) lo = mid; else hi = mid;
+    }
+    const int ef = e - L.s_delta[lo];
+    uint32_t c0 = lo * rg.rx + a - 1u, c1 = lo * rg.rx + b + 1u;  // s_off[c0] <= ef < s_off[c1]
+    while (c1 - c0 > 1u) {
+        const uint32_t mid = (c0 + c1) >> 1;
+        if (L.s_off[mid] <= ef) c0 = mid; else c1 = mid;
+    }
+    rc = c0;
+    g = (uint32_t)(L.s_beg[c0] + (ef - L.s_off[c0]));
+}
+
+__device__ __forceinline__ bool owns_in_chunk(const uint2 &ow, uint32_t t, uint32_t rc, uint32_t a, uint32_t b) {
+    const uint32_t px = ow.y >> 16;
+    return ow.x == t && (ow.y & 0xFFFFu) == rc && px >= a && px < b;
+}
+
+// Stages chunk [a, b) (entries [0, total_chunk) through `stage`) and selects its owned
+// particles' home copies into s_own_e / s_own_g; ends with a barrier.
+template <int kB, int kOwnMax, class Stage>
+__device__ __forceinline__ void stage_chunk(const ChainBuffers &b, uint32_t t, const TileRegion &rg, uint32_t a,
+                                            uint32_t bx, int total, TileLds &L, uint16_t *s_own_e,
+                                            uint32_t *s_own_g, Stage stage) {
+    constexpr int kPer = 5;  // a lane's loads are issued before any is used
+    const int tid = (int)threadIdx.x;
+    for (int e0 = 0; e0 < total; e0 += kPer * kB) {
+        uint32_t rc[kPer], g[kPer];
+        uint2 ow[kPer];
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int e = e0 + tid + k * kB;
+            rc[k] = 0u;
+            g[k] = 0u;
+            if (e < total) chunk_entry(rg, a, bx, L, e, rc[k], g[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int e = e0 + tid + k * kB;
+            if (e < total) {
+                stage(e, g[k]);
+                ow[k] = b.towner[g[k]];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int e = e0 + tid + k * kB;
+            if (e - tid >= total) break;  // the same for every lane of the block
+            const bool own = e < total && owns_in_chunk(ow[k], t, rc[k], a, bx);
+            select_own<kOwnMax>(own, (uint32_t)e, g[k], &L.s_count, s_own_e, s_own_g);
+        }
+    }
+    __syncthreads();
+}
+
+// Hands the owned particles of chunk [a, b) over to the stray list (its staged region does
+// not fit, or it owns too many).
+template <int kB>
+__device__ __forceinline__ void hand_over(const ChainBuffers &b, uint32_t t, const TileRegion &rg, uint32_t a,
+                                          uint32_t bx, int region_total, const TileLds &L) {
+    const uint32_t tid = threadIdx.x;
+    for (int e0 = 0; e0 < region_total; e0 += kB) {
+        const int e = e0 + (int)tid;
+        uint32_t j = kNoTile;
+        if (e < region_total) {
+            const uint32_t rc = region_cell_of(L.s_off, rg.cells, e);
+            const uint32_t g = (uint32_t)(L.s_beg[rc] + (e - L.s_off[rc]));
+            if (owns_in_chunk(b.towner[g], t, rc, a, bx)) j = g;
+        }
+        const bool mine = j != kNoTile;
+        const uint64_t m = __ballot(mine);
+        if (!m) continue;
+        const uint32_t lane = tid & 63u;
+        const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1u;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&b.counters[1], (uint32_t)__popcll(m));
+        base = __shfl(base, (int)leader, 64);
+        if (mine) b.stray[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = j;
+    }
+}
+
+template <class Scenes, int kB, int kE, bool kF>
+__global__ __launch_bounds__(kB) void k_density_tile(Scenes sc, const ChainBuffers b, uint32_t n, uint32_t tile_total) {
+    constexpr int kOwnMax = 2 * kB;
+    __shared__ float4 s_pos[kE];  // (predicted position xyz, cell-ordered slot)
+    __shared__ TileLds L;
+    __shared__ uint16_t s_own[kOwnMax];
+    __shared__ uint16_t s_q[kTQ * kB];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t t = xcd_tile(blockIdx.x, tile_total);
+    const SceneConst c = sc.scene_of_tile(t);
+    if (!c.use_tiles) return;
+    const TileRegion rg = tile_region(c, t - c.tile_base);
+    const int emax = min(kE, (int)c.tile_emax);
+    const int region_total = (int)region_runs<kB>(c, b, rg, L.s_off, L.s_beg, L.s_wave);
+    if (region_total == 0) return;
+    plan_chunks<kB>(rg, region_total, emax, L);
+    if (tid == 0 && L.s_nch > 1u) atomicAdd(&b.counters[5], 1u);
+    const uint32_t cap = b.list_cap;
+    uint16_t *__restrict__ nl16 = b.nlist16();
+    uint16_t *q = s_q + tid;
+    const float thr = c.R2_pre * (1.0f + 0x1p-20f);
+    const uint32_t nch = L.s_nch;
+    for (uint32_t ch = 0; ch < nch; ++ch) {
+        const uint32_t ca = L.s_ca[ch], cb = L.s_cb[ch];
+        if (L.s_cover[ch]) {
+            if (tid == 0) atomicAdd(&b.counters[6], 1u);
+            hand_over<kB>(b, t, rg, ca, cb, region_total, L);
+            continue;
+        }
+        if (tid == 0) L.s_count = 0u;
+        const int total = chunk_rows(rg, ca, cb, L);
+        stage_chunk<kB, kOwnMax>(b, t, rg, ca, cb, total, L, s_own, nullptr, [&](int e, uint32_t g) {
+            const float4 p = b.pred_pos[g];
+            s_pos[e] = make_float4(p.x, p.y, p.z, __uint_as_float(g));
+        });
+        const uint32_t owned = L.s_count;
+        if (owned > (uint32_t)kOwnMax) {
+            if (tid == 0) atomicAdd(&b.counters[6], 1u);
+            hand_over<kB>(b, t, rg, ca, cb, region_total, L);
+            __syncthreads();
+            continue;
+        }
+        for (uint32_t i = tid; i < owned; i += kB) {
+            const int eo = s_own[i];
+            const float4 S = s_pos[eo];
+            const uint32_t j = __float_as_uint(S.w);
+            const float4 s = make_float4(S.x, S.y, S.z, 1.0f);
+            const AxisCells ax = sample_axis(s.x, c.minx, c.maxx, c.R);
+            const AxisCells ay = sample_axis(s.y, c.miny, c.maxy, c.R);
+            const AxisCells az = sample_axis(s.z, c.minz, c.maxz, c.R);
+            // region coordinates of the predicted cell, each in [1, tile extent]
+            const uint32_t cx = (uint32_t)((int32_t)ax.mid - rg.ox);
+            const uint32_t cy = (uint32_t)((int32_t)ay.mid - rg.oy);
+            const uint32_t cz = (uint32_t)((int32_t)az.mid - rg.oz);
+            // interior: every sample point of the 27 cells lies inside the box, so every
+            // candidate counts for the density (else only those of the row's density cells)
+            const bool interior = (ax.inmask & ay.inmask & az.inmask) == 7u;
+            float M = 0.0f, Mn = 0.0f;
+            uint32_t nl = 0;
+            int qc = 0;
+            // Rows 0..8 of the walk, then a final pass (row 9, empty) that only drains.  A
+            // row's candidates are filtered with a fused-multiply-add distance (within 6 ulp
+            // of the reference's d2, against a bound 2^-20 above its cut-off, so no candidate
+            // the reference keeps is dropped) and queued without branches (the entry is
+            // written, the count advances when it passes), in chunks that fit the queue.  The
+            // drain recomputes d2 exactly and applies the reference's tests.  A wall particle
+            // drains at the end of each row, while that row's density cells are known.
+            for (uint32_t r = 0; r <= 9u; ++r) {
+                int e = 0, o3 = 0, mlo = 0, mhi = 0;
+                if (r < 9u) {
+                    const uint32_t dz = r / 3u, dy = r % 3u;
+                    const uint32_t row = (cz - 1u + dz) * rg.ry + (cy - 1u + dy);
+                    const uint32_t rc0 = row * rg.rx + (cx - 1u);
+                    const int d = L.s_delta[row];
+                    const int o0 = L.s_off[rc0] + d;
+                    o3 = L.s_off[rc0 + 3u] + d;
+                    e = o0;
+                    if (!interior) {
+                        // the three force cells whose sample points are inside the box
+                        // (the middle one always is)
+                        const bool row_in = ay.in((int)dy) && az.in((int)dz);
+                        mlo = row_in ? (ax.in(0) ? o0 : L.s_off[rc0 + 1u] + d) : o3;
+                        mhi = row_in ? (ax.in(2) ? o3 : L.s_off[rc0 + 2u] + d) : o3;
+                    }
+                }
+                for (;;) {
+                    const bool need = qc > 0 && (r == 9u || qc >= kTFlush || (!interior && e >= o3));
+                    if (__ballot(need)) {
+                        for (int k = 0; k < qc; ++k) {
+                            const int ec = q[k * kB];
+                            const float4 p = s_pos[ec];
+                            const float d2 = dist2(p, s);
+                            if (interior || (ec >= mlo && ec < mhi)) density_terms<kF>(c, d2, M, Mn);
+                            const uint32_t slot = __float_as_uint(p.w);
+                            if (slot != j && !(d2 > c.R2)) {
+                                if (nl < cap) nl16[(size_t)nl * n + j] = (uint16_t)ec;
+                                ++nl;
+                            }
+                        }
+                        qc = 0;
+                    }
+                    if (!__ballot(e < o3)) break;
+                    const int lim = min(o3, e + (kTQ - qc));
+                    for (; e + 3 < lim; e += 4) {
+                        float f[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) f[u] = dist2_filter(s_pos[e + u], s);
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            q[qc * kB] = (uint16_t)(e + u);
+                            qc += f[u] > thr ? 0 : 1;
+                        }
+                    }
+                    for (; e < lim; ++e) {
+                        q[qc * kB] = (uint16_t)e;
+                        qc += dist2_filter(s_pos[e], s) > thr ? 0 : 1;
+                    }
+                }
+            }
+            const float2 d = density_finish(c, M, Mn);
+            record_longest(b, nl);
+            b.dens[j] = d;
+            reinterpret_cast<float *>(b.vel_sorted + j)[3] = d.x;
+            reinterpret_cast<float *>(b.pred_pos + j)[3] = d.y;
+            b.ncount[j] = nl <= cap ? (uint8_t)nl : kListInvalid;
+        }
+        __syncthreads();  // the next chunk restages the LDS
+    }
+}
+
+template <class Scenes, int kB, int kE, bool kF>
+__global__ __launch_bounds__(kB) void k_force_tile(Scenes sc, const ChainBuffers b, uint32_t n, uint32_t tile_total) {
+    constexpr int kOwnMax = 2 * kB;
+    __shared__ float4 s_a[kE];  // (predicted position xyz, near density)
+    __shared__ float4 s_b[kE];  // (velocity xyz, density)
+    __shared__ TileLds L;
+    __shared__ uint16_t s_own_e[kOwnMax];
+    __shared__ uint32_t s_own_g[kOwnMax];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t t = xcd_tile(blockIdx.x, tile_total);
+    const SceneConst c = sc.scene_of_tile(t);
+    if (!c.use_tiles) return;
+    const TileRegion rg = tile_region(c, t - c.tile_base);
+    const int emax = min(kE, (int)c.tile_emax);
+    const int region_total = (int)region_runs<kB>(c, b, rg, L.s_off, L.s_beg, L.s_wave);
+    if (region_total == 0) return;
+    plan_chunks<kB>(rg, region_total, emax, L);
+    const uint16_t *__restrict__ nl16 = b.nlist16();
+    const uint32_t nch = L.s_nch;
+    for (uint32_t ch = 0; ch < nch; ++ch) {
+        if (L.s_cover[ch]) continue;  // handed over by the density pass: strays
+        const uint32_t ca = L.s_ca[ch], cb = L.s_cb[ch];
+        if (tid == 0) L.s_count = 0u;
+        const int total = chunk_rows(rg, ca, cb, L);
+        stage_chunk<kB, kOwnMax>(b, t, rg, ca, cb, total, L, s_own_e, s_own_g, [&](int e, uint32_t g) {
+            s_a[e] = b.pred_pos[g];
+            s_b[e] = b.vel_sorted[g];
+        });
+        const uint32_t owned = L.s_count;
+        if (owned > (uint32_t)kOwnMax) {
+            __syncthreads();
+            continue;  // handed over by the density pass
+        }
+        for (uint32_t i = tid; i < owned; i += kB) {
+            const uint32_t j = s_own_g[i];
+            const uint32_t cnt = b.ncount[j];
+            const int eo = s_own_e[i];
+            const float4 A = s_a[eo], V = s_b[eo];
+            const float4 s = make_float4(A.x, A.y, A.z, 1.0f);
+            const float4 P = b.pos_sorted[j];
+            const float2 myd = make_float2(V.w, A.w);
+            const ForceSelf me = force_self(c, j, s, P, V, myd);
+            float3 Fp = make_float3(0.0f, 0.0f, 0.0f), Fv = make_float3(0.0f, 0.0f, 0.0f);
+            if (cnt != kListInvalid) {
+                // the recorded neighbours, in the walk's order: eight indices loaded together,
+                // then their terms (the loads' latency is paid once per eight)
+                for (uint32_t k0 = 0; k0 < cnt; k0 += 8u) {
+                    uint32_t ix[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) ix[u] = nl16[(size_t)min(k0 + (uint32_t)u, cnt - 1u) * n + j];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        if (k0 + (uint32_t)u >= cnt) break;
+                        const float4 a = s_a[ix[u]], v = s_b[ix[u]];
+                        force_pair<kF>(c, me, make_float3(a.x, a.y, a.z), make_float3(v.x, v.y, v.z),
+                                       make_float2(v.w, a.w), Fp, Fv);
+                    }
+                }
+            } else {
+                // a list that did not fit: the 27 cells from the staged chunk
+                // (applyPressure...comp:288-325).  The particle's own copies need no slot
+                // test: at distance 0 the term is skipped (d < 1e-7).
+                const int3 c0 = force_cell(c, s);
+                const uint32_t cx = (uint32_t)(c0.x - rg.ox), cy = (uint32_t)(c0.y - rg.oy);
+                const uint32_t cz = (uint32_t)(c0.z - rg.oz);
+                for (uint32_t r = 0; r < 9u; ++r) {
+                    const uint32_t dz = r / 3u, dy = r % 3u;
+                    const uint32_t row = (cz - 1u + dz) * rg.ry + (cy - 1u + dy);
+                    const uint32_t rc0 = row * rg.rx + (cx - 1u);
+                    const int d = L.s_delta[row];
+                    const int o3 = L.s_off[rc0 + 3u] + d;
+                    for (int e = L.s_off[rc0] + d; e < o3; ++e) {
+                        const float4 a = s_a[e];
+                        if (dist2(a, s) > c.R2) continue;
+                        const float4 v = s_b[e];
+                        force_pair<kF>(c, me, make_float3(a.x, a.y, a.z), make_float3(v.x, v.y, v.z),
+                                       make_float2(v.w, a.w), Fp, Fv);
+                    }
+                }
+            }
+            force_store(c, b, j, P, V, myd.x, Fp, Fv);
+        }
+        __syncthreads();  // the next chunk restages the LDS
+    }
+}
+
+// The strays of the tiled passes (particles no tile owns, and those of tiles handed over),
+// one lane each, walking their cells through global memory.
+constexpr int kStrayBlock = 256;
+
+template <class Scenes, bool kF>
+__global__ __launch_bounds__(kStrayBlock) void k_density_stray(Scenes sc, const ChainBuffers b) {
+    const uint32_t count = b.counters[1];
+    for (uint32_t i = blockIdx.x * kStrayBlock + threadIdx.x; i < count; i += gridDim.x * kStrayBlock) {
+        const uint32_t j = b.stray[i];
+        density_global<kF>(sc.scene(j), b, j);
+    }
+}
+
+template <class Scenes, bool kF>
+__global__ __launch_bounds__(kStrayBlock) void k_force_stray(Scenes sc, const ChainBuffers b) {
+    __shared__ uint32_t s_q[kQueue * kStrayBlock];
+    const uint32_t count = b.counters[1];
+    for (uint32_t i = blockIdx.x * kStrayBlock + threadIdx.x; i < count; i += gridDim.x * kStrayBlock) {
+        const uint32_t j = b.stray[i];
+        const SceneConst c = sc.scene(j);
+        const float4 P = b.pos_sorted[j];
+        const float4 V = b.vel_sorted[j];
+        const float2 myd = b.dens[j];
+        const ForceSelf me = force_self(c, j, b.pred_pos[j], P, V, myd);
+        const ForceTerms ft = force_global<kF>(c, b, me, s_q + threadIdx.x, kStrayBlock);
+        force_store(c, b, j, P, V, myd.x, ft.Fp, ft.Fv);
+    }
+}
+
+}  // namespace
+
+const TileConfig kTileConfigs[kNumTileConfigs] = {
+    {256, 1280, false}, {128, 768, false}, {256, 1280, true},
+};
+
+uint32_t scene_tiles(const SceneConst &c) { return c.use_tiles ? c.ntx * c.nty * c.ntz : 0u; }
+
+void tile_dims_for(SceneConst &c, double ppc, int cfg) {
+    const TileConfig &tc = kTileConfigs[cfg];
+    if (!(ppc > 0.0)) ppc = 1.0;
+    // owned particles per tile about 0.85 of the workgroup, staged entries within 3/4 of
+    // the LDS capacity, region cells within kTileRegionMax
+    const double own_target = 0.85 * tc.block;
+    // a region cell stages its key's whole run, colliding cells' particles included: about
+    // one more occupied cell per T / (occupied cells) keys
+    const double occupied = std::min<double>((double)(c.p_end - c.p_begin), (double)c.gx * c.gy * c.gz);
+    const double collide = 1.0 + 1.3 * occupied / ((double)c.mask + 1.0);
+    const double stage_cap = 0.8 * tc.entries / collide;
+    static const uint32_t kShapes[3][2] = {{2, 2}, {2, 1}, {1, 1}};
+    uint32_t tw = 1, th = 1, td = 1;
+    for (const auto &sh : kShapes) {
+        th = sh[0];
+        td = sh[1];
+        double w = own_target / (ppc * th * td);
+        if (w > 64.0) w = 64.0;
+        tw = w < 1.0 ? 1u : (uint32_t)w;
+        while (tw > 1u && ((double)(tw + 2u) * (th + 2u) * (td + 2u) * ppc > stage_cap ||
+                           (tw + 2u) * (th + 2u) * (td + 2u) > kTileRegionMax))
+            --tw;
+        if (tw >= 4u) break;
+    }
+    if ((uint32_t)c.gx < tw) tw = (uint32_t)c.gx;
+    if ((uint32_t)c.gy < th) th = (uint32_t)c.gy;
+    if ((uint32_t)c.gz < td) td = (uint32_t)c.gz;
+    // the kernels' LDS tables hold at most 64 columns (chunks), 16 rows and kTileRegionMax cells
+    if (tw > (uint32_t)kChunkMax - 2u) tw = (uint32_t)kChunkMax - 2u;
+    c.tw = tw;
+    c.th = th;
+    c.td = td;
+    // staged entries beyond which a tile is split into chunks along x: the LDS capacity
+    c.tile_emax = (uint32_t)tc.entries;
+    c.ntx = ((uint32_t)c.gx + tw - 1u) / tw;
+    c.nty = ((uint32_t)c.gy + th - 1u) / th;
+    c.ntz = ((uint32_t)c.gz + td - 1u) / td;
+}
+
+template <class Scenes, int kB, int kE, bool kF>
+hipError_t launch_tile_pair(bool force, hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n,
+                            uint32_t tile_total, uint32_t stray_blocks) {
+    if (force) {
+        hipLaunchKernelGGL((k_force_tile<Scenes, kB, kE, kF>), dim3(tile_total), dim3(kB), 0, stream, sc, b, n,
+                           tile_total);
+        if (hipPeekAtLastError() != hipSuccess) return hipGetLastError();
+        hipLaunchKernelGGL((k_force_stray<Scenes, kF>), dim3(stray_blocks), dim3(kStrayBlock), 0, stream, sc, b);
+    } else {
+        hipLaunchKernelGGL((k_density_tile<Scenes, kB, kE, kF>), dim3(tile_total), dim3(kB), 0, stream, sc, b, n,
+                           tile_total);
+        if (hipPeekAtLastError() != hipSuccess) return hipGetLastError();
+        hipLaunchKernelGGL((k_density_stray<Scenes, kF>), dim3(stray_blocks), dim3(kStrayBlock), 0, stream, sc, b);
+    }
+    return hipGetLastError();
+}
+
+template <class Scenes>
+hipError_t launch_tile(bool force, hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n,
+                       uint32_t tile_total, uint32_t stray_blocks, int cfg) {
+    switch (cfg) {
+        case 1: return launch_tile_pair<Scenes, 128, 768, false>(force, stream, sc, b, n, tile_total, stray_blocks);
+        case 2:
+            if constexpr (std::is_same_v<Scenes, OneScene>)
+                return launch_tile_pair<Scenes, 256, 1280, true>(force, stream, sc, b, n, tile_total, stray_blocks);
+            else
+                return launch_tile_pair<Scenes, 256, 1280, false>(force, stream, sc, b, n, tile_total, stray_blocks);
+        default: return launch_tile_pair<Scenes, 256, 1280, false>(force, stream, sc, b, n, tile_total, stray_blocks);
+    }
+}
+
+template <class Scenes>
+hipError_t launch_density_tile(hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n,
+                               uint32_t tile_total, uint32_t stray_blocks, int cfg) {
+    return launch_tile(false, stream, sc, b, n, tile_total, stray_blocks, cfg);
+}
+
+template <class Scenes>
+hipError_t launch_force_tile(hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n,
+                             uint32_t tile_total, uint32_t stray_blocks, int cfg) {
+    return launch_tile(true, stream, sc, b, n, tile_total, stray_blocks, cfg);
+}
+
+#define SPH_TILE_INSTANTIATE(S)                                                                         \
+    template hipError_t launch_density_tile<S>(hipStream_t, const S &, const ChainBuffers &, uint32_t, \
+                                               uint32_t, uint32_t, int);                               \
+    template hipError_t launch_force_tile<S>(hipStream_t, const S &, const ChainBuffers &, uint32_t,   \
+                                             uint32_t, uint32_t, int);
+
+SPH_TILE_INSTANTIATE(OneScene)
+SPH_TILE_INSTANTIATE(ManyScenes)
+
+}  // namespace sph

@@ -219,7 +219,7 @@ int Chain::allocate(uint32_t n_, uint32_t table_total_, uint32_t brick_total_,
         {(void **)&b.run_info, (size_t)table_total * 8}, {(void **)&b.crun, (size_t)table_total * 8},
         {(void **)&b.hslot, N * 4},
         {(void **)&b.grid, (size_t)(grid_total ? grid_total : 1) * 8},
-        {(void **)&b.nlist, N * list_entries * 4}, {(void **)&b.ncount, N},
+        {(void **)&b.nlist, N * list_entries * 4}, {(void **)&b.ncount, N}, {(void **)&b.towner, N * 8},
         {(void **)&b.counters, 8 * 4},
         {(void **)&b.brick_flag, (size_t)(brick_total ? brick_total : 1) * 4},
         {(void **)&b.queue, (size_t)(brick_total ? brick_total : 1) * 4},
@@ -365,12 +365,34 @@ uint32_t list_entries_for_state(const sph_params &p, uint32_t n, const float *po
     return (uint32_t)std::min<double>((double)kList, std::max<double>((double)kListStart, want));
 }
 
+double particles_per_cell_for_state(const sph_params &p, uint32_t n, const float *pos4) {
+    // the mean over the cells of the state's bounding box, the box's grid at most
+    float lo[3] = {pos4[0], pos4[1], pos4[2]}, hi[3] = {pos4[0], pos4[1], pos4[2]};
+    for (uint32_t i = 1; i < n; ++i)
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::min(lo[a], pos4[4 * (size_t)i + a]);
+            hi[a] = std::max(hi[a], pos4[4 * (size_t)i + a]);
+        }
+    const double R = p.smoothing_radius;
+    double cells = 1.0;
+    for (int a = 0; a < 3; ++a) cells *= std::floor(((double)hi[a] - lo[a]) / R) + 1.0;
+    return (double)n / cells;
+}
+
+uint32_t stray_blocks_for(uint32_t untiled, uint32_t n, int block) {
+    // every particle of scenes without tiles, plus an eighth of the rest, in one sweep of
+    // the stray kernels (more strays take more sweeps)
+    const uint64_t lanes = (uint64_t)untiled + ((uint64_t)n - untiled) / 8u;
+    const uint64_t blocks = (lanes + (uint64_t)block - 1u) / (uint64_t)block;
+    return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(8u, blocks), 65535u);
+}
+
 void Chain::release() {
     void *ptrs[] = {b.pos, b.vel, b.keys, b.sort_k[0], b.sort_v[0], b.sort_k[1], b.sort_v[1],
                     b.pred_pos, b.pos_sorted, b.vel_sorted, b.dens, b.table, b.ghist,
                     b.lookback, b.tile_ctr, b.dbg_pred_vel, b.dbg_vel_force, b.hfull,
                     b.brick_flag, b.queue, b.stray, b.counters, b.bcount, b.blookback,
-                    b.run_info, b.crun, b.hslot, b.grid, b.nlist, b.ncount};
+                    b.run_info, b.crun, b.hslot, b.grid, b.nlist, b.ncount, b.towner};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (h_list) (void)hipHostFree(h_list);
@@ -410,13 +432,17 @@ hipError_t Chain::enqueue(hipStream_t s, const Scenes &sc, PassEvents *ev) {
         if ((e = launch_queue(s, b, brick_total)) != hipSuccess) return e;
     } else {
         if ((e = launch_bucket_scan(s, b, bucket_total)) != hipSuccess) return e;
-        if ((e = launch_place(s, sc, b, sorted_keys(), sorted_idx(), n, grid_total)) != hipSuccess) return e;
+        if ((e = launch_place(s, sc, b, sorted_keys(), sorted_idx(), n, grid_total, tiled())) != hipSuccess) return e;
     }
     if (ev) hipEventRecord(ev->ev[3], s);
-    e = bricks ? launch_density_brick(s, sc, b, n, brick_total) : launch_density(s, sc, b, n, lane_cfg);
+    e = bricks ? launch_density_brick(s, sc, b, n, brick_total)
+        : tiled() ? launch_density_tile(s, sc, b, n, tile_total, stray_blocks, tile_cfg)
+                  : launch_density(s, sc, b, n, lane_cfg);
     if (e != hipSuccess) return e;
     if (ev) hipEventRecord(ev->ev[4], s);
-    e = bricks ? launch_force_brick(s, sc, b, n, brick_total) : launch_force(s, sc, b, n, lane_cfg);
+    e = bricks ? launch_force_brick(s, sc, b, n, brick_total)
+        : tiled() ? launch_force_tile(s, sc, b, n, tile_total, stray_blocks, tile_cfg)
+                  : launch_force(s, sc, b, n, lane_cfg);
     if (e != hipSuccess) return e;
     if (ev) hipEventRecord(ev->ev[5], s);
     return hipSuccess;
@@ -511,6 +537,8 @@ struct sph_world {
     float4 *own_pos = nullptr, *own_vel = nullptr;
     bool tolerance = false;         // SPH_PRECISION_TOLERANCE: the tuner may pick the
                                     // approximate-math configurations
+    double ppc = 0.0;               // particles per occupied cell of the last uploaded state
+                                    // (0: not known; the box's mean is used)
     uint32_t list_pinned = 0;       // force-list entries fixed by sph_world_set_list_capacity
                                     // (0: sized from the state and grown when lists overflow)
     // Online autotuning of an AUTO world: its first substeps run the candidate
@@ -549,12 +577,44 @@ int lane_configs_for(const sph_world *w) { return w->tolerance ? kNumLaneConfigs
 // that launch configuration, kCodeBrick = the brick kernels; bits 8 and up carry the radix
 // digit width (0: 8 bits).
 constexpr int kCodeBrick = 100;
+constexpr int kCodeTile = 200;  // kCodeTile + c: the tiled kernels at tile configuration c
 constexpr int kCodeMask = 0xFF;
+
+int tile_configs_for(const sph_world *w) { return w->tolerance ? kNumTileConfigs : kNumExactTileConfigs; }
+
+// Tiles of the world's scene at tile configuration cfg (-1: the per-lane kernels), sized
+// for the particles per occupied cell of the last uploaded state.
+bool world_tiles_possible(const sph_world *w);
+
+void world_set_tiles(sph_world *w, int cfg) {
+    SceneConst &c = w->scene.c;
+    if (cfg < 0 || !world_tiles_possible(w)) {
+        c.use_tiles = 0u;
+        w->chain.tile_cfg = -1;
+        w->chain.tile_total = 0;
+        return;
+    }
+    const double ppc = w->ppc > 0.0 ? w->ppc : (double)w->n / ((double)c.gx * c.gy * c.gz);
+    tile_dims_for(c, ppc, cfg);
+    c.use_tiles = 1u;
+    c.tile_base = 0u;
+    w->chain.tile_cfg = cfg;
+    w->chain.tile_total = scene_tiles(c);
+    w->chain.stray_blocks = stray_blocks_for(0, w->n, 256);
+}
+
+// Tiles cover the box's cell grid: at most 64 cells per particle (+ 65536), so that empty
+// tiles stay cheap; cells without a halo grid are looked up through the hash table.
+bool world_tiles_possible(const sph_world *w) {
+    const SceneConst &c = w->scene.c;
+    return (double)c.gx * c.gy * c.gz <= 64.0 * w->n + 65536.0;
+}
 
 int world_apply_code(sph_world *w, int code) {
     const int kernels = code & kCodeMask;
     w->chain.bricks = kernels == kCodeBrick && w->chain.brick_total > 0;
     if (kernels >= 0 && kernels < kNumLaneConfigs) w->chain.lane_cfg = kernels;
+    world_set_tiles(w, kernels >= kCodeTile && kernels < kCodeTile + kNumTileConfigs ? kernels - kCodeTile : -1);
     const uint32_t width = (uint32_t)code >> 8;
     if (!w->chain.set_radix_bits(width ? width : 8u)) w->chain.set_radix_bits(8u);
     w->graph.drop();
@@ -585,6 +645,8 @@ int world_autotune(sph_world *w, uint32_t substeps, int *chosen, double *ms_out)
     std::vector<int> cands;
     for (int c = 0; c < lane_configs_for(w); ++c) cands.push_back(c);
     if (w->chain.brick_total) cands.push_back(kCodeBrick);
+    if (world_tiles_possible(w))
+        for (int c = 0; c < tile_configs_for(w); ++c) cands.push_back(kCodeTile + c);
     int best = kDefaultLaneConfig;
     double best_ms = 1e30;
     hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -664,6 +726,8 @@ int online_start(sph_world *w) {
     std::vector<int> codes;
     for (int c = 0; c < lane_configs_for(w); ++c) codes.push_back(c);
     if (w->chain.brick_total) codes.push_back(kCodeBrick);
+    if (world_tiles_possible(w))
+        for (int c = 0; c < tile_configs_for(w); ++c) codes.push_back(kCodeTile + c);
     w->online = sph_world::OnlineTune{};
     w->online.active = true;
     return online_begin_phase(w, std::move(codes));
@@ -748,8 +812,12 @@ int world_resolve_variant(sph_world *w, bool online = false) {
     if (code < 0) {
         const char *off = std::getenv("SPH_AUTOTUNE");
         const char *pin = std::getenv("SPH_LANE_CONFIG");
+        const char *tpin = std::getenv("SPH_TILE_CONFIG");
         const int pinned = pin ? std::atoi(pin) : -1;
-        if (pin && *pin >= '0' && *pin <= '9' && pinned >= 0 && pinned < lane_configs_for(w)) {
+        const int tpinned = tpin ? std::atoi(tpin) : -1;
+        if (tpin && *tpin >= '0' && *tpin <= '9' && tpinned < tile_configs_for(w) && world_tiles_possible(w)) {
+            code = kCodeTile + tpinned;  // a fixed tiled configuration (profiling)
+        } else if (pin && *pin >= '0' && *pin <= '9' && pinned >= 0 && pinned < lane_configs_for(w)) {
             code = pinned;  // a fixed per-lane launch configuration (profiling)
         } else if (off && std::strcmp(off, "0") == 0) {
             code = kDefaultLaneConfig;
@@ -841,6 +909,7 @@ int sph_world_set_params(sph_world *w, const sph_params *params) {
     w->params = *params;
     w->params.table_cells = T;
     w->scene.c = nc;
+    world_set_tiles(w, w->chain.tile_cfg);
     w->graph.drop();
     return SPH_OK;
 }
@@ -858,6 +927,12 @@ int sph_world_upload_state(sph_world *w, const float *pos4, const float *vel4) {
     if (!w->list_pinned) {
         const uint32_t want = list_entries_for_state(w->params, w->n, pos4);
         if (want > w->chain.b.list_cap && w->chain.set_list_cap(want) == SPH_OK) w->graph.drop();
+    }
+    w->ppc = particles_per_cell_for_state(w->params, w->n, pos4);
+    if (w->chain.tile_cfg >= 0) {
+        const SceneConst before = w->scene.c;
+        world_set_tiles(w, w->chain.tile_cfg);
+        if (before.tw != w->scene.c.tw || before.th != w->scene.c.th || before.td != w->scene.c.td) w->graph.drop();
     }
     return SPH_OK;
 }
@@ -961,17 +1036,53 @@ int sph_world_snapshot_async(sph_world *w, float *pos4, float *vel4) {
 
 int sph_world_set_kernel(sph_world *w, int variant) {
     if (!w) return fail(SPH_ERR_INVALID_ARG, "world is NULL");
-    if (variant < SPH_KERNEL_AUTO || variant > SPH_KERNEL_BRICK)
+    if (variant < SPH_KERNEL_AUTO || variant > SPH_KERNEL_TILE)
         return fail(SPH_ERR_INVALID_ARG, "unknown kernel variant %d", variant);
     if (variant == SPH_KERNEL_BRICK && w->chain.brick_total == 0)
         return fail(SPH_ERR_INVALID_ARG, "brick kernels need a grid of at most 2^28 bricks");
+    if (variant == SPH_KERNEL_TILE && !world_tiles_possible(w))
+        return fail(SPH_ERR_INVALID_ARG, "tiled kernels need a box of at most 64n + 65536 cells");
     hipError_t e = hipStreamSynchronize(w->stream);
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
     online_cancel(w);
     w->variant = variant;
     w->tuned = variant != SPH_KERNEL_AUTO;
-    world_apply_code(w, (variant == SPH_KERNEL_BRICK ? kCodeBrick : w->chain.lane_cfg) |
-                            (int)(w->chain.radix_bits == 8u ? 0u : w->chain.radix_bits << 8));
+    const int kernels = variant == SPH_KERNEL_BRICK ? kCodeBrick
+                      : variant == SPH_KERNEL_TILE   ? kCodeTile + (w->chain.tile_cfg >= 0 ? w->chain.tile_cfg : 0)
+                                                     : w->chain.lane_cfg;
+    world_apply_code(w, kernels | (int)(w->chain.radix_bits == 8u ? 0u : w->chain.radix_bits << 8));
+    return SPH_OK;
+}
+
+int sph_world_set_tile_config(sph_world *w, int config) {
+    if (!w) return fail(SPH_ERR_INVALID_ARG, "world is NULL");
+    if (config < 0 || config >= kNumTileConfigs) return fail(SPH_ERR_INVALID_ARG, "tile configuration %d", config);
+    if (kTileConfigs[config].tolerance && !w->tolerance)
+        return fail(SPH_ERR_INVALID_ARG, "tile configuration %d needs SPH_PRECISION_TOLERANCE", config);
+    if (!world_tiles_possible(w))
+        return fail(SPH_ERR_INVALID_ARG, "tiled kernels need a box of at most 64n + 65536 cells");
+    hipError_t e = hipStreamSynchronize(w->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    online_cancel(w);
+    w->variant = SPH_KERNEL_TILE;
+    w->tuned = true;
+    world_apply_code(w, (kCodeTile + config) | (int)(w->chain.radix_bits == 8u ? 0u : w->chain.radix_bits << 8));
+    return SPH_OK;
+}
+
+int sph_tile_configs(void) { return kNumTileConfigs; }
+
+int sph_world_tile_stats(sph_world *w, uint32_t out[4]) {
+    if (!w || !out) return fail(SPH_ERR_INVALID_ARG, "NULL argument");
+    uint32_t c[8];
+    hipError_t e = hipMemcpyAsync(c, w->chain.b.counters, sizeof c, hipMemcpyDeviceToHost, w->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(w->stream);
+    if (e != hipSuccess) return hip_fail(e, "tile statistics");
+    const bool tiled = w->chain.tiled() && w->stepped;
+    out[0] = tiled ? w->chain.tile_total : 0u;
+    out[1] = tiled ? c[1] : 0u;
+    out[2] = tiled ? c[5] : 0u;
+    out[3] = tiled ? c[6] : 0u;
     return SPH_OK;
 }
 
@@ -996,7 +1107,9 @@ int sph_world_autotune(sph_world *w, uint32_t substeps, int *chosen, double *ms_
     g_tune_cache[tune_bucket(w->n, w->T, w->chain.brick_total, w->tolerance)] = code;
     w->variant = SPH_KERNEL_AUTO;
     w->tuned = true;
-    *chosen = code == kCodeBrick ? SPH_KERNEL_BRICK : SPH_KERNEL_DIRECT;
+    *chosen = (code & kCodeMask) == kCodeBrick ? SPH_KERNEL_BRICK
+            : (code & kCodeMask) >= kCodeTile  ? SPH_KERNEL_TILE
+                                               : SPH_KERNEL_DIRECT;
     return SPH_OK;
 }
 
@@ -1076,7 +1189,7 @@ int sph_world_set_precision(sph_world *w, int mode) {
 
 int sph_world_kernel(const sph_world *w) {
     if (!w) return fail(SPH_ERR_INVALID_ARG, "world is NULL");
-    return w->chain.bricks ? SPH_KERNEL_BRICK : SPH_KERNEL_DIRECT;
+    return w->chain.bricks ? SPH_KERNEL_BRICK : w->chain.tiled() ? SPH_KERNEL_TILE : SPH_KERNEL_DIRECT;
 }
 
 int sph_world_kernel_config(const sph_world *w, char *buf, size_t bytes) {
@@ -1085,6 +1198,11 @@ int sph_world_kernel_config(const sph_world *w, char *buf, size_t bytes) {
     snprintf(radix, sizeof radix, "%u-bit radix digits (%u passes)", w->chain.radix_bits, w->chain.passes);
     if (w->chain.bricks)
         snprintf(buf, bytes, "brick %dx%dx%d cells, persistent queue, %s", kBrick, kBrick, kBrick, radix);
+    else if (w->chain.tiled())
+        snprintf(buf, bytes, "tiles of %ux%ux%u cells staged in LDS (%d-lane workgroups, %d entries), %s%s",
+                 w->scene.c.tw, w->scene.c.th, w->scene.c.td, kTileConfigs[w->chain.tile_cfg].block,
+                 kTileConfigs[w->chain.tile_cfg].entries, radix,
+                 kTileConfigs[w->chain.tile_cfg].tolerance ? ", approximate square root and reciprocal" : "");
     else
         snprintf(buf, bytes, "per-lane, %d candidates per scan step, %d-lane workgroups, %s, %s%s",
                  kLaneConfigs[w->chain.lane_cfg].unroll, kLaneConfigs[w->chain.lane_cfg].block,

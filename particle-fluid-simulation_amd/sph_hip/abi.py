@@ -15,7 +15,7 @@ STATUS_NAMES = {
     -4: "SPH_ERR_STATE", -5: "SPH_ERR_CAPACITY",
 }
 
-KERNELS = {"auto": 0, "direct": 1, "brick": 2}
+KERNELS = {"auto": 0, "direct": 1, "brick": 2, "tile": 3}
 
 DUMP_STAGES = {
     "cell_keys": 0, "sorted_keys": 1, "sorted_index": 2, "start": 3, "end": 4,
@@ -97,6 +97,9 @@ def load_library() -> C.CDLL:
         "sph_world_kernel_config": (C.c_int, [vp, C.c_char_p, C.c_size_t]),
         "sph_world_set_lane_config": (C.c_int, [vp, C.c_int]),
         "sph_lane_configs": (C.c_int, []),
+        "sph_world_set_tile_config": (C.c_int, [vp, C.c_int]),
+        "sph_tile_configs": (C.c_int, []),
+        "sph_world_tile_stats": (C.c_int, [vp, C.POINTER(C.c_uint32)]),
         "sph_world_set_precision": (C.c_int, [vp, C.c_int]),
         "sph_world_set_radix_bits": (C.c_int, [vp, C.c_int]),
         "sph_world_set_list_capacity": (C.c_int, [vp, C.c_uint32]),

@@ -102,7 +102,8 @@ class World:
         self._snap = None
 
     def set_kernel(self, variant: str) -> None:
-        """Neighbour-pass kernels: "auto", "direct" (one lane per particle) or "brick"."""
+        """Neighbour-pass kernels: "auto", "direct" (one lane per particle), "brick" or
+        "tile" (cell tiles staged in LDS)."""
         check(self._lib.sph_world_set_kernel(self._h, KERNELS[variant]))
 
     def autotune(self, substeps: int = 3):
@@ -118,7 +119,7 @@ class World:
     def kernel(self) -> str:
         r = self._lib.sph_world_kernel(self._h)
         check(min(r, 0))
-        return {1: "direct", 2: "brick"}[r]
+        return {1: "direct", 2: "brick", 3: "tile"}[r]
 
     @property
     def kernel_config(self) -> str:
@@ -133,6 +134,21 @@ class World:
     @staticmethod
     def lane_configs() -> int:
         return int(load_library().sph_lane_configs())
+
+    def set_tile_config(self, config: int) -> None:
+        """TILE with one fixed tile configuration (see tile_configs())."""
+        check(self._lib.sph_world_set_tile_config(self._h, config))
+
+    @staticmethod
+    def tile_configs() -> int:
+        return int(load_library().sph_tile_configs())
+
+    def tile_stats(self) -> dict:
+        """What the last substep's tiled passes did: tiles, strays, chunked tiles and chunks
+        handed over to the strays (all 0 after other kernels)."""
+        out = (C.c_uint32 * 4)()
+        check(self._lib.sph_world_tile_stats(self._h, out))
+        return dict(zip(("tiles", "strays", "chunked", "handed_over"), (int(v) for v in out)))
 
     def set_precision(self, mode: str) -> None:
         """"exact" (bit-identical to the reference's float order, the default) or

@@ -70,7 +70,7 @@ def test_near_pressure_every_lane_config(gpu, orc, config):
     _assert_bitexact("vel", gv, rv)
 
 
-@pytest.mark.parametrize("kernel", ["direct", "brick"])
+@pytest.mark.parametrize("kernel", ["direct", "brick", "tile"])
 def test_near_pressure_dam_break_ten_frames(gpu, orc, kernel):
     from sph_hip.workloads import DAM_BREAK_16K
     W = _near(DAM_BREAK_16K)

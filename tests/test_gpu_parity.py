@@ -41,7 +41,7 @@ def _assert_bitexact(name, got, ref):
                            f"gpu {got.ravel()[bad[:5]]} ref {ref.ravel()[bad[:5]]}")
 
 
-def _check_substep(sph_hip, orc, params, pos, vel, kernels=("direct", "brick")):
+def _check_substep(sph_hip, orc, params, pos, vel, kernels=("direct", "brick", "tile")):
     for kernel in kernels:
         got, ref = _check_substep_one(sph_hip, orc, params, pos, vel, kernel)
     return got, ref
@@ -150,7 +150,7 @@ def test_uniform_32m_one_substep(gpu, orc):
     p1.substeps = 1
     print(f"oracle n={W.n}", flush=True)
     rpos, rvel, ref = orc.substep(p1, pos, vel)
-    for kernel in ("direct", "brick"):
+    for kernel in ("direct", "brick", "tile"):
         print(f"case n={W.n} T={W.n} kernel={kernel}", flush=True)
         got = _run_one_substep(gpu, params, pos, vel, kernel)
         for s in STAGES_INT + STAGES_F32:
@@ -160,7 +160,7 @@ def test_uniform_32m_one_substep(gpu, orc):
         del got
 
 
-@pytest.mark.parametrize("kernel", ["direct", "brick"])
+@pytest.mark.parametrize("kernel", ["direct", "brick", "tile"])
 def test_dam_break_multi_frame_bitexact(gpu, orc, kernel):
     """10 frames x 3 substeps: the whole trajectory matches the oracle bit for bit."""
     from sph_hip.workloads import DAM_BREAK_16K as W
@@ -237,7 +237,7 @@ def test_tiny_box_strays(gpu, orc):
     # every particle left the cell grid (y clamped to 0.8 or to -0.3) after one substep
     assert ((rp1[:, 1] > 0.5) | (rp1[:, 1] < 0.0)).all()
     rp, rv = orc.step(params, pos, vel, 2)
-    for kernel in ("direct", "brick"):
+    for kernel in ("direct", "brick", "tile"):
         with gpu.World(params, 3000) as w:
             w.set_kernel(kernel)
             w.upload(pos, vel)
@@ -257,7 +257,7 @@ def test_box_thinner_than_particle_radius(gpu, orc):
                                 substeps=1, pressure_multiplier=500, target_density=55)
     pos, vel = gpu.fill_block(2000, 6, (0, 0, 0), (10, 0.2, 6), 1.0)
     rp, rv = orc.step(params, pos, vel, 2)
-    for kernel in ("direct", "brick"):
+    for kernel in ("direct", "brick", "tile"):
         print(f"case thin box kernel={kernel}", flush=True)
         with gpu.World(params, 2000) as w:
             w.set_kernel(kernel)

@@ -26,7 +26,7 @@ def _scene(gpu, n=N):
     return p, pos, vel
 
 
-@pytest.mark.parametrize("kernel", ["direct", "brick"])
+@pytest.mark.parametrize("kernel", ["direct", "brick", "tile"])
 def test_reference_scene_one_substep_every_stage(gpu, orc, kernel):
     p, pos, vel = _scene(gpu)
     assert p.min_xyz[0] == -140.0 and p.table_cells == 0
