@@ -251,6 +251,7 @@ def mixed_batch(steps: int, warmup: int, count: int = 256, fitting_only: bool = 
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
     bricks = sum(b.uses_bricks(i) for i in range(count))
+    batch_kernels = b.kernel
     b.close()
 
     worlds = []
@@ -279,7 +280,8 @@ def mixed_batch(steps: int, warmup: int, count: int = 256, fitting_only: bool = 
             "particles": total, "steps": steps,
             "batched_particle_steps_per_s": batched, "plain_particle_steps_per_s": plain,
             "batched_over_plain": batched / plain, "batched_ms_per_step": t_batch * 1e3 / steps,
-            "plain_ms_per_step": t_plain * 1e3 / steps, "scenes_on_brick_kernels": bricks}
+            "plain_ms_per_step": t_plain * 1e3 / steps, "scenes_on_brick_kernels": bricks,
+            "batched_kernels": batch_kernels}
 
 
 def request_stream(count: int, seed: int, rate_hz: float, n_range=(4096, 262144),

@@ -307,6 +307,15 @@ int sph_batch_download_scene(sph_batch *b, uint32_t scene, float *pos_vec4, floa
 int sph_batch_set_debug(sph_batch *b, int enable);
 int sph_batch_debug_dump(sph_batch *b, uint32_t scene, int stage, void *dst, size_t bytes);
 int sph_batch_scene_uses_bricks(sph_batch *b, uint32_t scene);  /* 1, 0, or < 0 on error */
+/* TILE at one fixed exact tile configuration (0 or 1, see sph_world_set_tile_config); every
+ * scene whose box has at most 64 cells per particle (+ 65536) gets tiles sized for its last
+ * uploaded state, the others' particles take the tiled launches' stray kernels.  AUTO
+ * measures the tile configurations beside the per-lane and brick kernels.  No counterpart in
+ * the reference (tuning). */
+int sph_batch_set_tile_config(sph_batch *b, int config);
+/* The kernels the next step uses once tuned (DIRECT, BRICK or TILE; AUTO before its
+ * measurement), or < 0 on error. */
+int sph_batch_kernel(const sph_batch *b);
 int sph_batch_pass_times(sph_batch *b, uint32_t substeps, sph_pass_times *out);
 /* sph_world_density_map for one scene of the batch (its own box and table range). */
 int sph_batch_density_map(sph_batch *b, uint32_t scene, float voxel_size, float *out, size_t count);

@@ -44,6 +44,8 @@ struct sph_batch {
     float *d_map = nullptr;  // density map staging for host copies
     size_t map_cap = 0;
     uint32_t list_pinned = 0;  // force-list entries fixed by sph_batch_set_list_capacity (0: adaptive)
+    std::vector<double> ppc;   // per scene: particles per cell of its last uploaded state (0: unknown)
+    int tile_cfg = 0;          // tile configuration TILE uses
     ManyScenes policy() const { return ManyScenes{d_scenes, d_scene_of, count}; }
 };
 
@@ -74,6 +76,8 @@ hipError_t launch_fill_scene_of(hipStream_t stream, const SceneConst *table, uin
 }  // namespace sph
 
 namespace {
+
+constexpr int kBatchCodeTile = 200;  // autotuner code of tile configuration c: kBatchCodeTile + c
 
 // A scene runs through the brick kernels when its particles are dense enough for a brick's
 // halo staging to be reused: at least 0.25 particles per cell of the box its particles
@@ -151,6 +155,7 @@ int sph_batch_create(uint32_t count, const sph_params *params, const uint32_t *c
     b->t_total = (uint32_t)t_total;
     b->b_total = (uint32_t)b_total;
     b->brick_choice.assign(count, 0);
+    b->ppc.assign(count, 0.0);
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete b;
@@ -182,10 +187,36 @@ int sph_batch_create(uint32_t count, const sph_params *params, const uint32_t *c
     return SPH_OK;
 }
 
+// Tiles for every scene whose box has at most 64 cells per particle (+ 65536), sized for its
+// last uploaded state, at tile configuration cfg (-1: none); the other scenes' particles are
+// the tiled launches' strays.
+static void batch_set_tiles(sph_batch *b, int cfg) {
+    uint32_t tile_base = 0, untiled = 0;
+    for (uint32_t s = 0; s < b->count; ++s) {
+        SceneConst &c = b->scenes[s];
+        const double cells = (double)c.gx * c.gy * c.gz;
+        c.use_tiles = 0u;
+        c.tile_base = tile_base;
+        if (cfg >= 0 && cells <= 64.0 * b->n[s] + 65536.0) {
+            tile_dims_for(c, b->ppc[s] > 0.0 ? b->ppc[s] : (double)b->n[s] / cells, cfg);
+            const uint64_t t = (uint64_t)c.ntx * c.nty * c.ntz;
+            if ((uint64_t)tile_base + t < (1ull << 31)) {
+                c.use_tiles = 1u;
+                tile_base += (uint32_t)t;
+            }
+        }
+        if (!c.use_tiles) untiled += b->n[s];
+    }
+    b->chain.tile_cfg = cfg >= 0 && tile_base > 0 ? cfg : -1;
+    b->chain.tile_total = tile_base;
+    b->chain.stray_blocks = stray_blocks_for(untiled, b->n_total, 256);
+}
+
 // DIRECT: every scene on the per-lane kernels.  BRICK: the queue kernels, with bricks for
 // the scenes whose brick grid exists and whose density (AUTO's per-scene choice, or all of
-// them when `all` is set) makes staging pay; the others ride along as lane chunks.
-static int batch_apply(sph_batch *b, int v, bool all) {
+// them when `all` is set) makes staging pay; the others ride along as lane chunks.  TILE:
+// the tiled kernels at tile configuration `tile_cfg`.
+static int batch_apply(sph_batch *b, int v, bool all, int tile_cfg = 0) {
     for (uint32_t s = 0; s < b->count; ++s) {
         uint32_t ub = 0u;
         if (v == SPH_KERNEL_BRICK && b->b_total && scene_bricks(b->scenes[s]))
@@ -193,6 +224,7 @@ static int batch_apply(sph_batch *b, int v, bool all) {
         b->scenes[s].use_bricks = ub;
     }
     b->chain.bricks = v == SPH_KERNEL_BRICK && b->b_total > 0;
+    batch_set_tiles(b, v == SPH_KERNEL_TILE ? tile_cfg : -1);
     b->graph.drop();
     return upload_scene_table(b);
 }
@@ -226,6 +258,7 @@ static int batch_autotune_once(sph_batch *b, uint32_t substeps) {
     std::vector<int> cands;
     for (int c = 0; c < kNumExactLaneConfigs; ++c) cands.push_back(c);
     if (b->b_total) cands.push_back(-1);
+    for (int c = 0; c < kNumExactTileConfigs; ++c) cands.push_back(kBatchCodeTile + c);
     b->chain.set_radix_bits(8u);
     const size_t n_kernel_cands = cands.size();
     for (size_t ci = 0;; ++ci) {
@@ -240,8 +273,9 @@ static int batch_autotune_once(sph_batch *b, uint32_t substeps) {
         const int code = cands[ci];
         const int v = code < 0 ? -1 : (code & 0xFF);
         b->chain.set_radix_bits(code < 0 || (code >> 8) == 0 ? 8u : (uint32_t)(code >> 8));
-        if (v >= 0) b->chain.lane_cfg = v;
-        int rc = batch_apply(b, v < 0 ? SPH_KERNEL_BRICK : SPH_KERNEL_DIRECT, false);
+        if (v >= 0 && v < kBatchCodeTile) b->chain.lane_cfg = v;
+        int rc = batch_apply(b, v < 0 ? SPH_KERNEL_BRICK : v >= kBatchCodeTile ? SPH_KERNEL_TILE : SPH_KERNEL_DIRECT,
+                             false, v - kBatchCodeTile);
         if (rc) { e = hipErrorUnknown; break; }
         e = hipMemcpyAsync(b->chain.b.pos, sp, bytes, hipMemcpyDeviceToDevice, b->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(b->chain.b.vel, sv, bytes, hipMemcpyDeviceToDevice, b->stream);
@@ -263,10 +297,12 @@ static int batch_autotune_once(sph_batch *b, uint32_t substeps) {
     if (sv) (void)hipFree(sv);
     if (e != hipSuccess) return hip_fail(e, "batch autotune");
     b->stepped = false;  // the table is the last trial's, not the restored state's
-    b->tuned_variant = best < 0 ? SPH_KERNEL_BRICK : SPH_KERNEL_DIRECT;
-    if (best >= 0) b->chain.lane_cfg = best & 0xFF;
+    const int bv = best < 0 ? -1 : (best & 0xFF);
+    b->tuned_variant = bv < 0 ? SPH_KERNEL_BRICK : bv >= kBatchCodeTile ? SPH_KERNEL_TILE : SPH_KERNEL_DIRECT;
+    if (bv >= 0 && bv < kBatchCodeTile) b->chain.lane_cfg = bv;
+    b->tile_cfg = bv >= kBatchCodeTile ? bv - kBatchCodeTile : b->tile_cfg;
     b->chain.set_radix_bits(best < 0 || (best >> 8) == 0 ? 8u : (uint32_t)(best >> 8));
-    return batch_apply(b, b->tuned_variant, false);
+    return batch_apply(b, b->tuned_variant, false, b->tile_cfg);
 }
 
 // The measurement again while the trials grew the force lists (their list configurations
@@ -284,13 +320,26 @@ static int batch_autotune(sph_batch *b, uint32_t substeps) {
 
 int sph_batch_set_kernel(sph_batch *b, int variant) {
     if (!b) return fail(SPH_ERR_INVALID_ARG, "batch is NULL");
-    if (variant < SPH_KERNEL_AUTO || variant > SPH_KERNEL_BRICK)
+    if (variant < SPH_KERNEL_AUTO || variant > SPH_KERNEL_TILE)
         return fail(SPH_ERR_INVALID_ARG, "unknown kernel variant %d", variant);
     hipError_t e = hipStreamSynchronize(b->stream);
     if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
     b->variant = variant;
     b->tuned_variant = variant == SPH_KERNEL_AUTO ? -1 : variant;
-    return batch_apply(b, variant == SPH_KERNEL_AUTO ? SPH_KERNEL_DIRECT : variant, variant == SPH_KERNEL_BRICK);
+    return batch_apply(b, variant == SPH_KERNEL_AUTO ? SPH_KERNEL_DIRECT : variant, variant == SPH_KERNEL_BRICK,
+                       b->tile_cfg);
+}
+
+int sph_batch_set_tile_config(sph_batch *b, int config) {
+    if (!b) return fail(SPH_ERR_INVALID_ARG, "batch is NULL");
+    if (config < 0 || config >= kNumExactTileConfigs)
+        return fail(SPH_ERR_INVALID_ARG, "tile configuration %d (a batch takes the exact ones)", config);
+    hipError_t e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+    b->variant = SPH_KERNEL_TILE;
+    b->tuned_variant = SPH_KERNEL_TILE;
+    b->tile_cfg = config;
+    return batch_apply(b, SPH_KERNEL_TILE, false, config);
 }
 
 int sph_batch_upload_scene(sph_batch *b, uint32_t scene, const float *pos4, const float *vel4) {
@@ -304,6 +353,12 @@ int sph_batch_upload_scene(sph_batch *b, uint32_t scene, const float *pos4, cons
     if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
     if (e != hipSuccess) return hip_fail(e, "upload");
     b->brick_choice[scene] = (uint8_t)choose_bricks(b->scenes[scene], b->n[scene], pos4, SPH_KERNEL_AUTO);
+    b->ppc[scene] = particles_per_cell_for_state(b->params[scene], b->n[scene], pos4);
+    if (b->chain.tile_cfg >= 0) {
+        batch_set_tiles(b, b->chain.tile_cfg);  // the scene's tiles follow its new density
+        b->graph.drop();
+        if ((rc = upload_scene_table(b))) return rc;
+    }
     if (!b->list_pinned) {
         const uint32_t want = list_entries_for_state(b->params[scene], b->n[scene], pos4);
         if (want > b->chain.b.list_cap && b->chain.set_list_cap(want) == SPH_OK) b->graph.drop();
@@ -403,6 +458,11 @@ int sph_batch_debug_dump(sph_batch *b, uint32_t scene, int stage, void *dst, siz
     if (!b->stepped) return fail(SPH_ERR_STATE, "no substep has run since the last upload");
     return dump_stage(b->chain, b->stream, stage, b->p_begin[scene], b->p_begin[scene] + b->n[scene],
                       b->t_begin[scene], b->t_begin[scene] + b->T[scene], dst, bytes);
+}
+
+int sph_batch_kernel(const sph_batch *b) {
+    if (!b) return fail(SPH_ERR_INVALID_ARG, "batch is NULL");
+    return b->tuned_variant < 0 ? SPH_KERNEL_AUTO : b->tuned_variant;
 }
 
 int sph_batch_scene_uses_bricks(sph_batch *b, uint32_t scene) {

@@ -121,6 +121,8 @@ def load_library() -> C.CDLL:
         "sph_last_error": (C.c_char_p, []),
         "sph_batch_create": (C.c_int, [C.c_uint32, P(SphParams), P(C.c_uint32), P(vp)]),
         "sph_batch_set_kernel": (C.c_int, [vp, C.c_int]),
+        "sph_batch_set_tile_config": (C.c_int, [vp, C.c_int]),
+        "sph_batch_kernel": (C.c_int, [vp]),
         "sph_batch_upload_scene": (C.c_int, [vp, C.c_uint32, vp, vp]),
         "sph_batch_step": (C.c_int, [vp, C.c_uint32]),
         "sph_batch_sync": (C.c_int, [vp]),

@@ -43,6 +43,17 @@ class Batch:
     def set_kernel(self, variant: str) -> None:
         check(self._lib.sph_batch_set_kernel(self._h, KERNELS[variant]))
 
+    def set_tile_config(self, config: int) -> None:
+        """TILE at one fixed exact tile configuration (0 or 1)."""
+        check(self._lib.sph_batch_set_tile_config(self._h, config))
+
+    @property
+    def kernel(self) -> str:
+        """The kernels the next step uses once tuned ("auto" before the measurement)."""
+        r = self._lib.sph_batch_kernel(self._h)
+        check(min(r, 0))
+        return {0: "auto", 1: "direct", 2: "brick", 3: "tile"}[r]
+
     def set_list_capacity(self, entries: int) -> None:
         """Fixes the force-list entries per particle (1..254); 0: adaptive sizing."""
         check(self._lib.sph_batch_set_list_capacity(self._h, entries))

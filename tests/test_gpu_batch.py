@@ -18,7 +18,7 @@ def _specs():
     return mixed_scenes(count=10, seed=100, n_min=300, n_max=6000, extent=(2.0, 12.0))
 
 
-@pytest.mark.parametrize("kernel", ["auto", "direct", "brick"])
+@pytest.mark.parametrize("kernel", ["auto", "direct", "brick", "tile"])
 def test_batch_matches_oracle_per_scene(gpu, orc, kernel):
     specs = _specs()
     params = [s.params() for s in specs]
@@ -81,19 +81,25 @@ def test_scene_alone_equals_scene_in_mix(gpu):
     _bitexact("vel", mixed[1], alone[1])
 
 
-def test_full_mixed_batch_one_frame(gpu, orc):
+@pytest.mark.parametrize("kernel", ["auto", "tile"])
+def test_full_mixed_batch_one_frame(gpu, orc, kernel):
     """Full size of the mixed workload: its 256 scenes (5.4M particles, 47 of them packed
     above their target density, whose particles record up to ~200 neighbours) stepped one
-    frame together; every scene's state is bit-exact against the oracle run on it alone."""
+    frame together, through the autotuned kernels and through tiles (every scene tiled); every
+    scene's state is bit-exact against the oracle run on it alone."""
     from sph_hip.workloads import mixed_scenes
     specs = mixed_scenes()
     params = [s.params() for s in specs]
     states = [s.state() for s in specs]
-    print(f"full mixed batch: {len(specs)} scenes, {sum(s.n for s in specs)} particles", flush=True)
+    print(f"full mixed batch: {len(specs)} scenes, {sum(s.n for s in specs)} particles, kernel={kernel}",
+          flush=True)
     with gpu.Batch(params, [s.n for s in specs]) as b:
+        if kernel == "tile":
+            b.set_tile_config(0)
         for i, (p, v) in enumerate(states):
             b.upload(i, p, v)
         b.step(1)
+        print("batch kernels:", b.kernel, flush=True)
         got = [b.download(i) for i in range(len(specs))]
     print("oracle", flush=True)
     for i, (p, v) in enumerate(states):
@@ -121,7 +127,7 @@ def shifted(gpu, params, pos, offset):
 OFFSETS = [(-7.31, 2.77, -0.123), (13.5, -41.05, 0.6), (-0.35, -0.7, -3.3), (1.0e3, 5.25, -2.0e2)]
 
 
-@pytest.mark.parametrize("kernel", ["direct", "brick"])
+@pytest.mark.parametrize("kernel", ["direct", "brick", "tile"])
 def test_batch_scenes_with_offset_boxes(gpu, orc, kernel):
     """Scenes whose boxes start away from the origin (each its own asymmetric offset), two
     frames in one batch: every scene bit-exact against the oracle run on it alone."""
