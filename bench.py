@@ -395,6 +395,11 @@ def main() -> int:
     ap.add_argument("--lane-config", type=int, default=None,
                     help="pin one per-lane launch configuration (0-9) instead of autotuning; "
                          "for hardware-counter runs, where counters distort the tuner's timings")
+    ap.add_argument("--tile-config", type=int, default=None,
+                    help="pin one tiled configuration (0-1: cell tiles staged in LDS) instead of "
+                         "autotuning; for hardware-counter runs of the tiled kernels")
+    ap.add_argument("--no-tiled-leg", dest="tiled_leg", action="store_false",
+                    help="skip timing the tiled kernels (tile configuration 0) beside the tuned step")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--batch-steps", type=int, default=100,
                     help="steps of the mixed 256-scene batched-vs-plain comparison (0 = skip)")
@@ -430,7 +435,11 @@ def main() -> int:
     # restores that state afterwards), and the last warmup step builds the tuned graph.
     world.step(max(0, args.warmup - 1))
     world.sync()
-    if args.lane_config is None:
+    if args.tile_config is not None:
+        world.set_tile_config(args.tile_config)
+        chosen = world.kernel_config
+        print(f"[bench] pinned: {chosen}", file=sys.stderr, flush=True)
+    elif args.lane_config is None:
         chosen, tuned_ms = world.autotune(5)
         chosen = world.kernel_config
         print(f"[bench] autotune: {chosen}, {tuned_ms:.3f} ms/substep", file=sys.stderr, flush=True)
@@ -454,8 +463,9 @@ def main() -> int:
     # Dominant kernel, timed with HIP events recorded on the library's stream around every
     # launch of 30 more substeps continuing the timed run.
     pt = world.pass_times(30)
+    tile_stats = world.tile_stats()
     variants = {}
-    for v in ("brick", "direct"):  # direct last: keeps the tuned per-lane configuration
+    for v in ("brick", "tile", "direct"):  # direct last: keeps the tuned per-lane configuration
         world.set_kernel(v)
         variants[v] = world.pass_times(10)
     dom = "density" if pt["density_ms"] > pt["force_ms"] else "force"
@@ -502,6 +512,27 @@ def main() -> int:
                "over_exact": (W.n * args.steps / t_tol) / value}
         wt.close()
         print(f"[bench] tolerance path: {tol['ms_per_step']:.3f} ms/step ({tol['kernels']})", file=sys.stderr,
+              flush=True)
+
+    # The same timed steps through the tiled kernels (tile configuration 0: cell tiles
+    # staged in LDS), from the same start after the same warmup; never the bench's value.
+    tiled = None
+    if args.tiled_leg and args.tile_config is None:
+        wtl = sph_hip.World(params, W.n)
+        wtl.set_tile_config(0)
+        wtl.upload(pos, vel)
+        wtl.step(args.warmup)
+        wtl.sync()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        wtl.step(args.steps)
+        wtl.sync()
+        t_tl = time.perf_counter() - t1
+        tiled = {"value": W.n * args.steps / t_tl, "ms_per_step": t_tl * 1e3 / args.steps,
+                 "kernels": wtl.kernel_config, "pass_ms": wtl.pass_times(30), "stats": wtl.tile_stats(),
+                 "over_tuned": (W.n * args.steps / t_tl) / value}
+        wtl.close()
+        print(f"[bench] tiled path: {tiled['ms_per_step']:.3f} ms/step ({tiled['kernels']})", file=sys.stderr,
               flush=True)
 
     # updateDensityMap of the last substep at a voxel of R/3.5 (16M voxels for uniform_1m,
@@ -570,6 +601,8 @@ def main() -> int:
         "pass_ms": pt,
         "pass_ms_by_kernel": variants,
         "tolerance_path": tol,
+        "tiled_path": tiled,
+        "tile_stats": tile_stats,
         "density_map": dmap,
         "pcie_inclusive": pcie,
         "mixed_batch": mixed,
