@@ -15,7 +15,7 @@ def test_autotune_keeps_state_and_parity(gpu, orc):
         before = w.download()
         chosen, ms = w.autotune(2)
         after = w.download()
-        assert chosen in ("direct", "brick") and ms > 0
+        assert chosen in ("direct", "brick", "tile") and ms > 0
         assert w.kernel == chosen
         assert np.array_equal(before[0].view(np.uint32), after[0].view(np.uint32))
         assert np.array_equal(before[1].view(np.uint32), after[1].view(np.uint32))
