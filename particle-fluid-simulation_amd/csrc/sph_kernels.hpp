@@ -47,7 +47,8 @@ struct ChainBuffers {
                             // [4] entries of the longest force list above list_cap / 2
                             // since the host last read the record (0: none)
                             // [5] tiles split into chunks [6] chunks handed over to the strays
-                            // (the last tiled density pass)
+                            // [7] their particles, listed from the top of `stray` (the last
+                            // tiled density pass)
     // Cell-ordered layout (sph_layout.hip).
     uint32_t *bcount;       // [bucket_total] particles per bucket, then the bucket's first slot
     uint32_t *blookback;    // [bucket tiles] look-back words of the bucket scan

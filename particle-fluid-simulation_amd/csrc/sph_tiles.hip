@@ -18,10 +18,10 @@
 // 16-bit indices into the staged region; the force pass stages the same region (position
 // + near density, velocity + density per entry) and reads its neighbours from LDS.
 //
-// A tile whose region holds more than its scene's tile_emax entries (at most kE), or that
-// owns more than kOwnMax particles, hands its particles over to the stray list; the strays
-// (those and the particles no tile owns) then take one lane each in the stray kernels that
-// follow each tiled pass, walking their cells through global memory.
+// The particles no tile owns (the placement's strays) take one lane each in the first
+// workgroups of each tiled launch, walking their cells through global memory beside the
+// tiles.  A chunk too large for LDS (see plan_chunks) hands its particles over to the top of
+// the stray list, which a stray kernel after each tiled pass walks the same way.
 #include <algorithm>
 
 #include "sph_device.hpp"
@@ -363,8 +363,8 @@ __device__ __forceinline__ void stage_chunk(const ChainBuffers &b, uint32_t t, c
 // Hands the owned particles of chunk [a, b) over to the stray list (its staged region does
 // not fit, or it owns too many).
 template <int kB>
-__device__ __forceinline__ void hand_over(const ChainBuffers &b, uint32_t t, const TileRegion &rg, uint32_t a,
-                                          uint32_t bx, int region_total, const TileLds &L) {
+__device__ __forceinline__ void hand_over(const ChainBuffers &b, uint32_t n, uint32_t t, const TileRegion &rg,
+                                          uint32_t a, uint32_t bx, int region_total, const TileLds &L) {
     const uint32_t tid = threadIdx.x;
     for (int e0 = 0; e0 < region_total; e0 += kB) {
         const int e = e0 + (int)tid;
@@ -380,21 +380,32 @@ __device__ __forceinline__ void hand_over(const ChainBuffers &b, uint32_t t, con
         const uint32_t lane = tid & 63u;
         const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1u;
         uint32_t base = 0;
-        if (lane == leader) base = atomicAdd(&b.counters[1], (uint32_t)__popcll(m));
+        if (lane == leader) base = atomicAdd(&b.counters[7], (uint32_t)__popcll(m));
         base = __shfl(base, (int)leader, 64);
-        if (mine) b.stray[base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = j;
+        // from the top of the stray list (the placement's strays fill it from the bottom)
+        if (mine) b.stray[n - 1u - (base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull)))] = j;
     }
 }
 
 template <class Scenes, int kB, int kE, bool kF>
-__global__ __launch_bounds__(kB) void k_density_tile(Scenes sc, const ChainBuffers b, uint32_t n, uint32_t tile_total) {
+__global__ __launch_bounds__(kB) void k_density_tile(Scenes sc, const ChainBuffers b, uint32_t n, uint32_t lead,
+                                                     uint32_t tile_total) {
     constexpr int kOwnMax = 2 * kB;
     __shared__ float4 s_pos[kE];  // (predicted position xyz, cell-ordered slot)
     __shared__ TileLds L;
     __shared__ uint16_t s_own[kOwnMax];
     __shared__ uint16_t s_q[kTQ * kB];
     const uint32_t tid = threadIdx.x;
-    const uint32_t t = xcd_tile(blockIdx.x, tile_total);
+    if (blockIdx.x < lead) {
+        // the placement's strays, beside the tiles (their walks' latency overlaps them)
+        const uint32_t count = b.counters[1];
+        for (uint32_t i = blockIdx.x * kB + tid; i < count; i += lead * kB) {
+            const uint32_t j = b.stray[i];
+            density_global<kF>(sc.scene(j), b, j);
+        }
+        return;
+    }
+    const uint32_t t = xcd_tile(blockIdx.x - lead, tile_total);
     const SceneConst c = sc.scene_of_tile(t);
     if (!c.use_tiles) return;
     const TileRegion rg = tile_region(c, t - c.tile_base);
@@ -412,7 +423,7 @@ __global__ __launch_bounds__(kB) void k_density_tile(Scenes sc, const ChainBuffe
         const uint32_t ca = L.s_ca[ch], cb = L.s_cb[ch];
         if (L.s_cover[ch]) {
             if (tid == 0) atomicAdd(&b.counters[6], 1u);
-            hand_over<kB>(b, t, rg, ca, cb, region_total, L);
+            hand_over<kB>(b, n, t, rg, ca, cb, region_total, L);
             continue;
         }
         if (tid == 0) L.s_count = 0u;
@@ -424,7 +435,7 @@ __global__ __launch_bounds__(kB) void k_density_tile(Scenes sc, const ChainBuffe
         const uint32_t owned = L.s_count;
         if (owned > (uint32_t)kOwnMax) {
             if (tid == 0) atomicAdd(&b.counters[6], 1u);
-            hand_over<kB>(b, t, rg, ca, cb, region_total, L);
+            hand_over<kB>(b, n, t, rg, ca, cb, region_total, L);
             __syncthreads();
             continue;
         }
@@ -517,15 +528,32 @@ __global__ __launch_bounds__(kB) void k_density_tile(Scenes sc, const ChainBuffe
 }
 
 template <class Scenes, int kB, int kE, bool kF>
-__global__ __launch_bounds__(kB) void k_force_tile(Scenes sc, const ChainBuffers b, uint32_t n, uint32_t tile_total) {
+__global__ __launch_bounds__(kB) void k_force_tile(Scenes sc, const ChainBuffers b, uint32_t n, uint32_t lead,
+                                                   uint32_t tile_total) {
     constexpr int kOwnMax = 2 * kB;
+    static_assert(kE * 4 >= kQueue * kB, "the strays' walk queue lives in s_b");
     __shared__ float4 s_a[kE];  // (predicted position xyz, near density)
     __shared__ float4 s_b[kE];  // (velocity xyz, density)
     __shared__ TileLds L;
     __shared__ uint16_t s_own_e[kOwnMax];
     __shared__ uint32_t s_own_g[kOwnMax];
     const uint32_t tid = threadIdx.x;
-    const uint32_t t = xcd_tile(blockIdx.x, tile_total);
+    if (blockIdx.x < lead) {
+        // the placement's strays, beside the tiles; their walks queue candidates in s_b
+        const uint32_t count = b.counters[1];
+        for (uint32_t i = blockIdx.x * kB + tid; i < count; i += lead * kB) {
+            const uint32_t j = b.stray[i];
+            const SceneConst cj = sc.scene(j);
+            const float4 P = b.pos_sorted[j];
+            const float4 V = b.vel_sorted[j];
+            const float2 myd = b.dens[j];
+            const ForceSelf me = force_self(cj, j, b.pred_pos[j], P, V, myd);
+            const ForceTerms ft = force_global<kF>(cj, b, me, reinterpret_cast<uint32_t *>(s_b) + tid, kB);
+            force_store(cj, b, j, P, V, myd.x, ft.Fp, ft.Fv);
+        }
+        return;
+    }
+    const uint32_t t = xcd_tile(blockIdx.x - lead, tile_total);
     const SceneConst c = sc.scene_of_tile(t);
     if (!c.use_tiles) return;
     const TileRegion rg = tile_region(c, t - c.tile_base);
@@ -602,25 +630,25 @@ __global__ __launch_bounds__(kB) void k_force_tile(Scenes sc, const ChainBuffers
     }
 }
 
-// The strays of the tiled passes (particles no tile owns, and those of tiles handed over),
-// one lane each, walking their cells through global memory.
+// The particles of chunks handed over by the tiled density pass, one lane each, walking
+// their cells through global memory after each tiled pass.
 constexpr int kStrayBlock = 256;
 
 template <class Scenes, bool kF>
-__global__ __launch_bounds__(kStrayBlock) void k_density_stray(Scenes sc, const ChainBuffers b) {
-    const uint32_t count = b.counters[1];
+__global__ __launch_bounds__(kStrayBlock) void k_density_stray(Scenes sc, const ChainBuffers b, uint32_t n) {
+    const uint32_t count = b.counters[7];
     for (uint32_t i = blockIdx.x * kStrayBlock + threadIdx.x; i < count; i += gridDim.x * kStrayBlock) {
-        const uint32_t j = b.stray[i];
+        const uint32_t j = b.stray[n - 1u - i];
         density_global<kF>(sc.scene(j), b, j);
     }
 }
 
 template <class Scenes, bool kF>
-__global__ __launch_bounds__(kStrayBlock) void k_force_stray(Scenes sc, const ChainBuffers b) {
+__global__ __launch_bounds__(kStrayBlock) void k_force_stray(Scenes sc, const ChainBuffers b, uint32_t n) {
     __shared__ uint32_t s_q[kQueue * kStrayBlock];
-    const uint32_t count = b.counters[1];
+    const uint32_t count = b.counters[7];
     for (uint32_t i = blockIdx.x * kStrayBlock + threadIdx.x; i < count; i += gridDim.x * kStrayBlock) {
-        const uint32_t j = b.stray[i];
+        const uint32_t j = b.stray[n - 1u - i];
         const SceneConst c = sc.scene(j);
         const float4 P = b.pos_sorted[j];
         const float4 V = b.vel_sorted[j];
@@ -681,16 +709,19 @@ void tile_dims_for(SceneConst &c, double ppc, int cfg) {
 template <class Scenes, int kB, int kE, bool kF>
 hipError_t launch_tile_pair(bool force, hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n,
                             uint32_t tile_total, uint32_t stray_blocks) {
+    // leading stray workgroups (stray_blocks counts kStrayBlock lanes each): a multiple of 8
+    // keeps the tiles' XCD numbering
+    const uint32_t lead = (stray_blocks * (uint32_t)(kStrayBlock / kB) + 7u) / 8u * 8u;
     if (force) {
-        hipLaunchKernelGGL((k_force_tile<Scenes, kB, kE, kF>), dim3(tile_total), dim3(kB), 0, stream, sc, b, n,
-                           tile_total);
+        hipLaunchKernelGGL((k_force_tile<Scenes, kB, kE, kF>), dim3(lead + tile_total), dim3(kB), 0, stream, sc, b,
+                           n, lead, tile_total);
         if (hipPeekAtLastError() != hipSuccess) return hipGetLastError();
-        hipLaunchKernelGGL((k_force_stray<Scenes, kF>), dim3(stray_blocks), dim3(kStrayBlock), 0, stream, sc, b);
+        hipLaunchKernelGGL((k_force_stray<Scenes, kF>), dim3(stray_blocks), dim3(kStrayBlock), 0, stream, sc, b, n);
     } else {
-        hipLaunchKernelGGL((k_density_tile<Scenes, kB, kE, kF>), dim3(tile_total), dim3(kB), 0, stream, sc, b, n,
-                           tile_total);
+        hipLaunchKernelGGL((k_density_tile<Scenes, kB, kE, kF>), dim3(lead + tile_total), dim3(kB), 0, stream, sc,
+                           b, n, lead, tile_total);
         if (hipPeekAtLastError() != hipSuccess) return hipGetLastError();
-        hipLaunchKernelGGL((k_density_stray<Scenes, kF>), dim3(stray_blocks), dim3(kStrayBlock), 0, stream, sc, b);
+        hipLaunchKernelGGL((k_density_stray<Scenes, kF>), dim3(stray_blocks), dim3(kStrayBlock), 0, stream, sc, b, n);
     }
     return hipGetLastError();
 }

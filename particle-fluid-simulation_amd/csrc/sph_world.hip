@@ -380,9 +380,9 @@ double particles_per_cell_for_state(const sph_params &p, uint32_t n, const float
 }
 
 uint32_t stray_blocks_for(uint32_t untiled, uint32_t n, int block) {
-    // every particle of scenes without tiles, plus an eighth of the rest, in one sweep of
-    // the stray kernels (more strays take more sweeps)
-    const uint64_t lanes = (uint64_t)untiled + ((uint64_t)n - untiled) / 8u;
+    // every particle of scenes without tiles, plus a sixteenth of the rest, in one sweep of
+    // the stray workgroups (more strays take more sweeps)
+    const uint64_t lanes = (uint64_t)untiled + ((uint64_t)n - untiled) / 16u;
     const uint64_t blocks = (lanes + (uint64_t)block - 1u) / (uint64_t)block;
     return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(8u, blocks), 65535u);
 }
@@ -1080,7 +1080,7 @@ int sph_world_tile_stats(sph_world *w, uint32_t out[4]) {
     if (e != hipSuccess) return hip_fail(e, "tile statistics");
     const bool tiled = w->chain.tiled() && w->stepped;
     out[0] = tiled ? w->chain.tile_total : 0u;
-    out[1] = tiled ? c[1] : 0u;
+    out[1] = tiled ? c[1] + c[7] : 0u;
     out[2] = tiled ? c[5] : 0u;
     out[3] = tiled ? c[6] : 0u;
     return SPH_OK;
