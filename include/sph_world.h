@@ -192,11 +192,13 @@ int sph_world_set_lane_config(sph_world *w, int config);
 int sph_lane_configs(void);
 /* Selects TILE with one fixed tile configuration, 0 <= config < sph_tile_configs() (the
  * numbering of SPH_TILE_CONFIG): 0 = 256-lane workgroups staging up to 1280 entries, 1 =
- * 128 lanes and 768 entries, 2 = configuration 0 with the approximate square root and
- * reciprocal (needs SPH_PRECISION_TOLERANCE).  The tiled kernels stage each tile's cells
- * and their neighbours in LDS and give bit-identical results to every other variant; the
- * tile shape follows the particles per cell of the last uploaded state.  Needs a box of at
- * most 64n + 65536 cells.  No counterpart in the reference (tuning). */
+ * 128 lanes and 768 entries, both followed by the per-lane force pass over the neighbour
+ * lists the tiled density pass records; 2 and 3 = 0 and 1 with the force pass tiled too;
+ * 4 = configuration 0 with the approximate square root and reciprocal (needs
+ * SPH_PRECISION_TOLERANCE).  The tiled kernels stage each tile's cells and their
+ * neighbours in LDS and give bit-identical results to every other variant; the tile shape
+ * follows the particles per cell of the last uploaded state.  Needs a box of at most
+ * 64n + 65536 cells.  No counterpart in the reference (tuning). */
 int sph_world_set_tile_config(sph_world *w, int config);
 int sph_tile_configs(void);
 /* What the last substep's tiled passes did (all 0 when it ran other kernels): out[0] tiles,
@@ -307,7 +309,7 @@ int sph_batch_download_scene(sph_batch *b, uint32_t scene, float *pos_vec4, floa
 int sph_batch_set_debug(sph_batch *b, int enable);
 int sph_batch_debug_dump(sph_batch *b, uint32_t scene, int stage, void *dst, size_t bytes);
 int sph_batch_scene_uses_bricks(sph_batch *b, uint32_t scene);  /* 1, 0, or < 0 on error */
-/* TILE at one fixed exact tile configuration (0 or 1, see sph_world_set_tile_config); every
+/* TILE at one fixed exact tile configuration (0 to 3, see sph_world_set_tile_config); every
  * scene whose box has at most 64 cells per particle (+ 65536) gets tiles sized for its last
  * uploaded state, the others' particles take the tiled launches' stray kernels.  AUTO
  * measures the tile configurations beside the per-lane and brick kernels.  No counterpart in

@@ -138,14 +138,17 @@ hipError_t launch_force(hipStream_t stream, const Scenes &sc, const ChainBuffers
 // Tiled neighbour passes (sph_tiles.hip).  A workgroup stages the cell runs of its tile
 // grown by one cell per side in LDS and evaluates the particles the tile owns (predicted
 // cell inside the tile, recorded by launch_place); the first `stray_blocks` workgroups walk
-// the strays through global memory.  Configurations: workgroup lanes and LDS entries.
+// the strays through global memory.  Configurations: workgroup lanes and LDS entries, and
+// the force pass: tiled (lists of 16-bit staged indices), or per-lane configuration
+// `force_lane` over lists of cell-ordered slots.  The exact configurations come first.
 struct TileConfig {
     int block;
     int entries;  // staged entries per tile (a tile with more falls back to global walks)
     bool tolerance;
+    int force_lane;  // per-lane force configuration (kLaneConfigs), -1: the tiled force pass
 };
-constexpr int kNumTileConfigs = 3;
-constexpr int kNumExactTileConfigs = 2;
+constexpr int kNumTileConfigs = 5;
+constexpr int kNumExactTileConfigs = 4;
 extern const TileConfig kTileConfigs[kNumTileConfigs];
 // Region cells a tile may have ((tw + 2) (th + 2) (td + 2)).
 constexpr uint32_t kTileRegionMax = 768;

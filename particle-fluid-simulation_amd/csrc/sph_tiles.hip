@@ -387,7 +387,9 @@ __device__ __forceinline__ void hand_over(const ChainBuffers &b, uint32_t n, uin
     }
 }
 
-template <class Scenes, int kB, int kE, bool kF>
+// kS: force lists of cell-ordered slots (for the per-lane force pass), else of 16-bit
+// staged indices (for the tiled one).
+template <class Scenes, int kB, int kE, bool kF, bool kS>
 __global__ __launch_bounds__(kB) void k_density_tile(Scenes sc, const ChainBuffers b, uint32_t n, uint32_t lead,
                                                      uint32_t tile_total) {
     constexpr int kOwnMax = 2 * kB;
@@ -492,7 +494,10 @@ __global__ __launch_bounds__(kB) void k_density_tile(Scenes sc, const ChainBuffe
                             if (interior || (ec >= mlo && ec < mhi)) density_terms<kF>(c, d2, M, Mn);
                             const uint32_t slot = __float_as_uint(p.w);
                             if (slot != j && !(d2 > c.R2)) {
-                                if (nl < cap) nl16[(size_t)nl * n + j] = (uint16_t)ec;
+                                if (nl < cap) {
+                                    if (kS) b.nlist[(size_t)nl * n + j] = slot;
+                                    else nl16[(size_t)nl * n + j] = (uint16_t)ec;
+                                }
                                 ++nl;
                             }
                         }
@@ -661,8 +666,11 @@ __global__ __launch_bounds__(kStrayBlock) void k_force_stray(Scenes sc, const Ch
 
 }  // namespace
 
+// Per-lane force over slot lists (lane configuration 3: 4 candidates per scan step, 128
+// lanes; 10 its tolerance variant) outruns the tiled force pass wherever both were measured.
 const TileConfig kTileConfigs[kNumTileConfigs] = {
-    {256, 1280, false}, {128, 768, false}, {256, 1280, true},
+    {256, 1280, false, 3}, {128, 768, false, 3}, {256, 1280, false, -1}, {128, 768, false, -1},
+    {256, 1280, true, 10},
 };
 
 uint32_t scene_tiles(const SceneConst &c) { return c.use_tiles ? c.ntx * c.nty * c.ntz : 0u; }
@@ -706,50 +714,59 @@ void tile_dims_for(SceneConst &c, double ppc, int cfg) {
     c.ntz = ((uint32_t)c.gz + td - 1u) / td;
 }
 
-template <class Scenes, int kB, int kE, bool kF>
-hipError_t launch_tile_pair(bool force, hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n,
-                            uint32_t tile_total, uint32_t stray_blocks) {
-    // leading stray workgroups (stray_blocks counts kStrayBlock lanes each): a multiple of 8
-    // keeps the tiles' XCD numbering
-    const uint32_t lead = (stray_blocks * (uint32_t)(kStrayBlock / kB) + 7u) / 8u * 8u;
-    if (force) {
-        hipLaunchKernelGGL((k_force_tile<Scenes, kB, kE, kF>), dim3(lead + tile_total), dim3(kB), 0, stream, sc, b,
-                           n, lead, tile_total);
-        if (hipPeekAtLastError() != hipSuccess) return hipGetLastError();
-        hipLaunchKernelGGL((k_force_stray<Scenes, kF>), dim3(stray_blocks), dim3(kStrayBlock), 0, stream, sc, b, n);
-    } else {
-        hipLaunchKernelGGL((k_density_tile<Scenes, kB, kE, kF>), dim3(lead + tile_total), dim3(kB), 0, stream, sc,
-                           b, n, lead, tile_total);
-        if (hipPeekAtLastError() != hipSuccess) return hipGetLastError();
-        hipLaunchKernelGGL((k_density_stray<Scenes, kF>), dim3(stray_blocks), dim3(kStrayBlock), 0, stream, sc, b, n);
-    }
+namespace {
+
+// Leading stray workgroups of a tiled launch (stray_blocks counts kStrayBlock lanes each): a
+// multiple of 8 keeps the tiles' XCD numbering.
+template <int kB>
+uint32_t lead_blocks(uint32_t stray_blocks) {
+    return (stray_blocks * (uint32_t)(kStrayBlock / kB) + 7u) / 8u * 8u;
+}
+
+template <class Scenes, int kB, int kE, bool kF, bool kS>
+hipError_t launch_density_pair(hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n,
+                               uint32_t tile_total, uint32_t stray_blocks) {
+    const uint32_t lead = lead_blocks<kB>(stray_blocks);
+    hipLaunchKernelGGL((k_density_tile<Scenes, kB, kE, kF, kS>), dim3(lead + tile_total), dim3(kB), 0, stream, sc, b,
+                       n, lead, tile_total);
+    if (hipPeekAtLastError() != hipSuccess) return hipGetLastError();
+    hipLaunchKernelGGL((k_density_stray<Scenes, kF>), dim3(stray_blocks), dim3(kStrayBlock), 0, stream, sc, b, n);
     return hipGetLastError();
 }
 
-template <class Scenes>
-hipError_t launch_tile(bool force, hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n,
-                       uint32_t tile_total, uint32_t stray_blocks, int cfg) {
-    switch (cfg) {
-        case 1: return launch_tile_pair<Scenes, 128, 768, false>(force, stream, sc, b, n, tile_total, stray_blocks);
-        case 2:
-            if constexpr (std::is_same_v<Scenes, OneScene>)
-                return launch_tile_pair<Scenes, 256, 1280, true>(force, stream, sc, b, n, tile_total, stray_blocks);
-            else
-                return launch_tile_pair<Scenes, 256, 1280, false>(force, stream, sc, b, n, tile_total, stray_blocks);
-        default: return launch_tile_pair<Scenes, 256, 1280, false>(force, stream, sc, b, n, tile_total, stray_blocks);
-    }
+template <class Scenes, int kB, int kE>
+hipError_t launch_force_pair(hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n,
+                             uint32_t tile_total, uint32_t stray_blocks) {
+    const uint32_t lead = lead_blocks<kB>(stray_blocks);
+    hipLaunchKernelGGL((k_force_tile<Scenes, kB, kE, false>), dim3(lead + tile_total), dim3(kB), 0, stream, sc, b, n,
+                       lead, tile_total);
+    if (hipPeekAtLastError() != hipSuccess) return hipGetLastError();
+    hipLaunchKernelGGL((k_force_stray<Scenes, false>), dim3(stray_blocks), dim3(kStrayBlock), 0, stream, sc, b, n);
+    return hipGetLastError();
 }
+
+}  // namespace
 
 template <class Scenes>
 hipError_t launch_density_tile(hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n,
                                uint32_t tile_total, uint32_t stray_blocks, int cfg) {
-    return launch_tile(false, stream, sc, b, n, tile_total, stray_blocks, cfg);
+    constexpr bool kOne = std::is_same_v<Scenes, OneScene>;
+    switch (cfg) {
+        case 1: return launch_density_pair<Scenes, 128, 768, false, true>(stream, sc, b, n, tile_total, stray_blocks);
+        case 2: return launch_density_pair<Scenes, 256, 1280, false, false>(stream, sc, b, n, tile_total, stray_blocks);
+        case 3: return launch_density_pair<Scenes, 128, 768, false, false>(stream, sc, b, n, tile_total, stray_blocks);
+        case 4: return launch_density_pair<Scenes, 256, 1280, kOne, true>(stream, sc, b, n, tile_total, stray_blocks);
+        default: return launch_density_pair<Scenes, 256, 1280, false, true>(stream, sc, b, n, tile_total, stray_blocks);
+    }
 }
 
 template <class Scenes>
 hipError_t launch_force_tile(hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n,
                              uint32_t tile_total, uint32_t stray_blocks, int cfg) {
-    return launch_tile(true, stream, sc, b, n, tile_total, stray_blocks, cfg);
+    if (kTileConfigs[cfg].force_lane >= 0) return launch_force(stream, sc, b, n, kTileConfigs[cfg].force_lane);
+    if (kTileConfigs[cfg].block == 128)
+        return launch_force_pair<Scenes, 128, 768>(stream, sc, b, n, tile_total, stray_blocks);
+    return launch_force_pair<Scenes, 256, 1280>(stream, sc, b, n, tile_total, stray_blocks);
 }
 
 #define SPH_TILE_INSTANTIATE(S)                                                                         \

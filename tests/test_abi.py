@@ -132,7 +132,7 @@ def test_lane_configs_without_a_device():
     lib = sph_hip.load_library()
     assert lib.sph_lane_configs() == 12
     assert lib.sph_world_set_lane_config(None, 0) == -1
-    assert lib.sph_tile_configs() == 3
+    assert lib.sph_tile_configs() == 5
     assert lib.sph_world_set_tile_config(None, 0) == -1
     assert lib.sph_world_tile_stats(None, None) == -1
 

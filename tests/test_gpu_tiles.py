@@ -2,10 +2,12 @@
 against the FP32 oracle, bit for bit: every exact tile configuration on the dam break (one
 substep, every stage, and ten frames), the 1M scene over three frames (its first clamp
 piles particles onto the margin planes, so tiles split into chunks), lists pinned far
-below the neighbour counts (the force pass walks the staged chunk instead), scenes whose
-local density is far above the mean the tiles were sized for (chunks along x; a single
-column too dense for LDS hands its particles to the stray kernels), the near-pressure term,
-and the tolerance configuration within 1e-5 of the FP64 oracle."""
+below the neighbour counts (the force pass walks the cells instead), scenes whose local
+density is far above the mean the tiles were sized for (chunks along x; a single column
+too dense for LDS hands its particles to the stray kernels), the near-pressure term, and
+the tolerance configuration within 1e-5 of the FP64 oracle.  Configurations 0 and 1 pair
+the tiled density pass with the per-lane force pass over slot lists, 2 and 3 with the tiled
+force pass over staged-index lists; the cases that stress the force pass run both."""
 import numpy as np
 import pytest
 
@@ -22,7 +24,7 @@ def _tile_world(gpu, params, n, config):
     return w
 
 
-@pytest.mark.parametrize("config", [0, 1])
+@pytest.mark.parametrize("config", [0, 1, 2, 3])
 def test_tiles_dam_break_every_stage_and_ten_frames(gpu, orc, config):
     from sph_hip.workloads import DAM_BREAK_16K as W
     params = W.params()
@@ -52,14 +54,15 @@ def test_tiles_dam_break_every_stage_and_ten_frames(gpu, orc, config):
     _assert_bitexact("vel", gv, rv)
 
 
-def test_tiles_uniform_1m_three_frames(gpu, orc):
+@pytest.mark.parametrize("config", [0, 2])
+def test_tiles_uniform_1m_three_frames(gpu, orc, config):
     """Three frames of the 1M scene (T = 2^21) on tiles: the margin piles the first clamp
     forms overflow their tiles' LDS, which then split into chunks; bit-exact throughout."""
     from sph_hip.workloads import UNIFORM_1M as W
     params = W.params(table_cells=1 << 21)
     pos, vel = W.state()
-    print(f"case n={W.n} frames=3 tiles", flush=True)
-    with _tile_world(gpu, params, W.n, 0) as w:
+    print(f"case n={W.n} frames=3 tile config {config}", flush=True)
+    with _tile_world(gpu, params, W.n, config) as w:
         w.upload(pos, vel)
         w.step(3)
         gp, gv = w.download()
@@ -71,14 +74,17 @@ def test_tiles_uniform_1m_three_frames(gpu, orc):
     _assert_bitexact("vel", gv, rv)
 
 
+@pytest.mark.parametrize("config", [0, 2])
 @pytest.mark.parametrize("entries", [1, 16])
-def test_tiles_short_lists(gpu, orc, entries):
+def test_tiles_short_lists(gpu, orc, entries, config):
     """Lists pinned far below the dam break's neighbour counts: the force pass walks the
-    27 cells of the staged chunk instead; ten frames bit-exact."""
+    27 cells instead (through global memory, or of the staged chunk); ten frames
+    bit-exact."""
     from sph_hip.workloads import DAM_BREAK_16K as W
     params = W.params()
     pos, vel = W.state()
-    with _tile_world(gpu, params, W.n, 0) as w:
+    print(f"case short lists {entries}, tile config {config}", flush=True)
+    with _tile_world(gpu, params, W.n, config) as w:
         w.set_list_capacity(entries)
         w.upload(pos, vel)
         w.step(10)
@@ -98,8 +104,9 @@ def _background_and_block(gpu, n_bg, n_block, lo, hi):
     return params, np.concatenate([p0, p1]), np.concatenate([v0, v1])
 
 
+@pytest.mark.parametrize("config", [0, 2])
 @pytest.mark.parametrize("case", ["chunks", "hand_over"])
-def test_tiles_dense_blocks(gpu, orc, case):
+def test_tiles_dense_blocks(gpu, orc, case, config):
     """A uniform background plus a block far denser than the mean the tiles are sized for:
     ~6 particles per cell (tiles crossing it split into chunks along x) or ~60 (one column's
     region exceeds the LDS: its particles are handed over to the stray kernels).  Two
@@ -109,8 +116,8 @@ def test_tiles_dense_blocks(gpu, orc, case):
     else:
         params, pos, vel = _background_and_block(gpu, 8000, 12000, (4, 2, 1), (5.5, 3.5, 2.5))
     n = pos.shape[0]
-    print(f"case dense block {case} n={n}", flush=True)
-    with _tile_world(gpu, params, n, 0) as w:
+    print(f"case dense block {case} n={n}, tile config {config}", flush=True)
+    with _tile_world(gpu, params, n, config) as w:
         w.upload(pos, vel)
         w.step(1)
         stats = w.tile_stats()
@@ -148,7 +155,7 @@ def test_tiles_near_pressure(gpu, orc):
 
 
 def test_tiles_tolerance(gpu, orc):
-    """Tile configuration 2 (approximate square root and reciprocal, opt-in): integer stages
+    """Tile configuration 4 (approximate square root and reciprocal, opt-in): integer stages
     bit-exact, one substep within 1e-5 of the FP64 oracle."""
     from sph_hip.workloads import DAM_BREAK_16K as W
     params = W.params()
@@ -157,9 +164,9 @@ def test_tiles_tolerance(gpu, orc):
     p1.substeps = 1
     with gpu.World(p1, W.n) as w:
         with pytest.raises(gpu.SphError):
-            w.set_tile_config(2)  # needs the tolerance precision
+            w.set_tile_config(4)  # needs the tolerance precision
         w.set_precision("tolerance")
-        w.set_tile_config(2)
+        w.set_tile_config(4)
         assert "approximate" in w.kernel_config
         w.set_debug(True)
         w.upload(pos, vel)
