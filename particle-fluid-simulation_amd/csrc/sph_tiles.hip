@@ -165,6 +165,39 @@ __device__ __forceinline__ void density_global(const SceneConst &c, const ChainB
     b.ncount[j] = kListInvalid;
 }
 
+// The longest force list above half the allocated capacity, for the host's list sizing
+// (as the per-lane density pass records it).
+__device__ __forceinline__ void record_longest(const ChainBuffers &b, uint32_t found) {
+    const uint64_t longer = __ballot(found > (b.list_cap >> 1));
+    if (!longer) return;
+    uint32_t longest = 0;
+    for (uint64_t m = longer; m; m &= m - 1ull)
+        longest = max(longest, (uint32_t)__builtin_amdgcn_readlane((int)found, __ffsll((long long)m) - 1));
+    if ((threadIdx.x & 63u) == (uint32_t)(__ffsll((long long)longer) - 1) &&
+        __hip_atomic_load(&b.counters[4], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < longest)
+        atomicMax(&b.counters[4], longest);
+}
+
+// Density of slot j through global memory, recording its force list of cell-ordered slots
+// as the per-lane density pass does (lane_list: kListLds staged entries at `stride`), for
+// the strays of launches whose force pass reads slot lists.
+template <bool kF>
+__device__ __forceinline__ void density_listed(const SceneConst &c, const ChainBuffers &b, uint32_t n, uint32_t j,
+                                               uint32_t *lane_list, int stride) {
+    const float4 s = b.pred_pos[j];
+    uint32_t count, found;
+    const float2 d = density_with_list<4, kF>(c, j, s, b.pred_pos, tile_grid_runs(c, b), lane_list, stride, b.nlist,
+                                              n, b.list_cap, count, found);
+    record_longest(b, found);
+    b.dens[j] = d;
+    reinterpret_cast<float *>(b.vel_sorted + j)[3] = d.x;
+    reinterpret_cast<float *>(b.pred_pos + j)[3] = d.y;
+    b.ncount[j] = (uint8_t)count;
+    const uint32_t m = count == kListInvalid ? 0u : min(count, (uint32_t)kListLds);
+    for (uint32_t k = 0; __ballot(k < m); ++k)
+        if (k < m) b.nlist[(size_t)k * n + j] = lane_list[k * stride];
+}
+
 // Force terms of slot j whose neighbours are read from global memory: the 27-cell walk
 // queueing in-range candidates in LDS (q, stride `stride`) as the per-lane kernels do, for
 // strays, tiles too large for LDS and lists that did not fit.
@@ -192,19 +225,6 @@ __device__ __forceinline__ void force_store(const SceneConst &c, const ChainBuff
     }
     b.pos[o] = po;
     b.vel[o] = vo;
-}
-
-// The longest force list above half the allocated capacity, for the host's list sizing
-// (as the per-lane density pass records it).
-__device__ __forceinline__ void record_longest(const ChainBuffers &b, uint32_t found) {
-    const uint64_t longer = __ballot(found > (b.list_cap >> 1));
-    if (!longer) return;
-    uint32_t longest = 0;
-    for (uint64_t m = longer; m; m &= m - 1ull)
-        longest = max(longest, (uint32_t)__builtin_amdgcn_readlane((int)found, __ffsll((long long)m) - 1));
-    if ((threadIdx.x & 63u) == (uint32_t)(__ffsll((long long)longer) - 1) &&
-        __hip_atomic_load(&b.counters[4], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < longest)
-        atomicMax(&b.counters[4], longest);
 }
 
 // Own-selection and staging of a tile, shared by both passes: run lookups, then (when the
@@ -399,11 +419,14 @@ __global__ __launch_bounds__(kB) void k_density_tile(Scenes sc, const ChainBuffe
     __shared__ uint16_t s_q[kTQ * kB];
     const uint32_t tid = threadIdx.x;
     if (blockIdx.x < lead) {
-        // the placement's strays, beside the tiles (their walks' latency overlaps them)
+        // the placement's strays, beside the tiles (their walks' latency overlaps them); with
+        // slot lists they record theirs, their list entries staged in s_pos
+        static_assert(!kS || kE * 4 >= kListLds * kB, "the strays' list staging lives in s_pos");
         const uint32_t count = b.counters[1];
         for (uint32_t i = blockIdx.x * kB + tid; i < count; i += lead * kB) {
             const uint32_t j = b.stray[i];
-            density_global<kF>(sc.scene(j), b, j);
+            if (kS) density_listed<kF>(sc.scene(j), b, n, j, reinterpret_cast<uint32_t *>(s_pos) + tid, kB);
+            else density_global<kF>(sc.scene(j), b, j);
         }
         return;
     }
@@ -636,15 +659,18 @@ __global__ __launch_bounds__(kB) void k_force_tile(Scenes sc, const ChainBuffers
 }
 
 // The particles of chunks handed over by the tiled density pass, one lane each, walking
-// their cells through global memory after each tiled pass.
+// their cells through global memory after each tiled pass (recording slot lists when the
+// force pass reads them).
 constexpr int kStrayBlock = 256;
 
-template <class Scenes, bool kF>
+template <class Scenes, bool kF, bool kS>
 __global__ __launch_bounds__(kStrayBlock) void k_density_stray(Scenes sc, const ChainBuffers b, uint32_t n) {
+    __shared__ uint32_t s_list[kS ? kListLds * kStrayBlock : 1];
     const uint32_t count = b.counters[7];
     for (uint32_t i = blockIdx.x * kStrayBlock + threadIdx.x; i < count; i += gridDim.x * kStrayBlock) {
         const uint32_t j = b.stray[n - 1u - i];
-        density_global<kF>(sc.scene(j), b, j);
+        if (kS) density_listed<kF>(sc.scene(j), b, n, j, s_list + threadIdx.x, kStrayBlock);
+        else density_global<kF>(sc.scene(j), b, j);
     }
 }
 
@@ -730,7 +756,7 @@ hipError_t launch_density_pair(hipStream_t stream, const Scenes &sc, const Chain
     hipLaunchKernelGGL((k_density_tile<Scenes, kB, kE, kF, kS>), dim3(lead + tile_total), dim3(kB), 0, stream, sc, b,
                        n, lead, tile_total);
     if (hipPeekAtLastError() != hipSuccess) return hipGetLastError();
-    hipLaunchKernelGGL((k_density_stray<Scenes, kF>), dim3(stray_blocks), dim3(kStrayBlock), 0, stream, sc, b, n);
+    hipLaunchKernelGGL((k_density_stray<Scenes, kF, kS>), dim3(stray_blocks), dim3(kStrayBlock), 0, stream, sc, b, n);
     return hipGetLastError();
 }
 
