@@ -203,9 +203,10 @@ int sph_world_set_tile_config(sph_world *w, int config);
 int sph_tile_configs(void);
 /* What the last substep's tiled passes did (all 0 when it ran other kernels): out[0] tiles,
  * out[1] strays (particles walked one lane each through global memory: those no tile owns
- * and those of handed-over chunks), out[2] tiles split into chunks along x (their staged
- * region exceeded the LDS capacity), out[3] chunks handed over to the strays (a single
- * column still too large, or more particles than the workgroup takes).  Synchronises the
+ * and those of handed-over tiles), out[2] tiles whose region streamed through LDS in more
+ * than one group of rows (it exceeded the LDS capacity), out[3] tiles handed over to the
+ * strays (a single row of the region too large, more particles than the workgroup takes,
+ * or, with the tiled force pass, a region larger than the LDS capacity).  Synchronises the
  * world's stream.  No counterpart in the reference (diagnostics). */
 int sph_world_tile_stats(sph_world *w, uint32_t out[4]);
 

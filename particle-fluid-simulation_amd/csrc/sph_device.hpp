@@ -51,7 +51,7 @@ struct SceneConst {
     uint32_t tw, th, td;
     uint32_t ntx, nty, ntz;
     uint32_t use_tiles;
-    uint32_t tile_emax;  // staged entries beyond which a tile's particles become strays
+    uint32_t tile_emax;  // staged entries beyond which a tile's region streams in row groups
 };
 
 // cellCoordToHash (updateCellEntries.comp:36-52): u32 arithmetic, wraps mod 2^32.

@@ -57,7 +57,7 @@ __global__ __launch_bounds__(kBlock) void k_hash(
         const uint32_t b = blockIdx.x - hash_blocks;
         const uint32_t nb = gridDim.x - hash_blocks;
         if (b == 0 && tid < kTileCounters) tile_ctr[tid] = 0u;
-        // [0] queue, [1] strays, [5] chunked tiles, [6] handed-over chunks, [7] their
+        // [0] queue, [1] strays, [5] streamed tiles, [6] handed-over tiles, [7] their
         // particles (not [4]: the host clears the list record when it reads it)
         if (b == 0 && counters && (tid < 2 || (tid >= 5 && tid < 8))) counters[tid] = 0u;
         for (uint32_t i = b * kBlock + tid; i < table_total; i += nb * kBlock) {

@@ -46,9 +46,9 @@ struct ChainBuffers {
     uint32_t *counters;     // [0] queue length [1] strays [2] density drain [3] force drain
                             // [4] entries of the longest force list above list_cap / 2
                             // since the host last read the record (0: none)
-                            // [5] tiles split into chunks [6] chunks handed over to the strays
-                            // [7] their particles, listed from the top of `stray` (the last
-                            // tiled density pass)
+                            // [5] tiles streamed in more than one row group [6] tiles handed
+                            // over to the strays [7] their particles, listed from the top of
+                            // `stray` (the last tiled density pass)
     // Cell-ordered layout (sph_layout.hip).
     uint32_t *bcount;       // [bucket_total] particles per bucket, then the bucket's first slot
     uint32_t *blookback;    // [bucket tiles] look-back words of the bucket scan

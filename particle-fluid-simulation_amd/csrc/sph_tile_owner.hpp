@@ -42,8 +42,8 @@ __device__ __forceinline__ TileRegion tile_region(const SceneConst &c, uint32_t 
     return r;
 }
 
-// (owning tile, region cell of the home copy | region x of the predicted cell << 16) of a
-// particle at current position P with predicted position s, or (kNoTile, 0).
+// (owning tile, region cell of the home copy) of a particle at current position P with
+// predicted position s, or (kNoTile, 0).
 __device__ __forceinline__ uint2 tile_owner(const SceneConst &c, const float3 &P, const float3 &s) {
     if (!c.use_tiles) return make_uint2(kNoTile, 0u);
     const AxisCells ax = sample_axis(s.x, c.minx, c.maxx, c.R);
@@ -65,10 +65,7 @@ __device__ __forceinline__ uint2 tile_owner(const SceneConst &c, const float3 &P
     const uint32_t lx = (uint32_t)(x0 - (int32_t)(tx * c.tw) + 1 + dx);
     const uint32_t ly = (uint32_t)(y0 - (int32_t)(ty * c.th) + 1 + dy);
     const uint32_t lz = (uint32_t)(z0 - (int32_t)(tz * c.td) + 1 + dz);
-    // low 16 bits: the home copy's region cell; high 16: the predicted cell's region x
-    const uint32_t home = lx + (c.tw + 2u) * (ly + (c.th + 2u) * lz);
-    const uint32_t px = (uint32_t)(x0 - (int32_t)(tx * c.tw) + 1);
-    return make_uint2(t, home | (px << 16));
+    return make_uint2(t, lx + (c.tw + 2u) * (ly + (c.th + 2u) * lz));
 }
 
 }  // namespace sph

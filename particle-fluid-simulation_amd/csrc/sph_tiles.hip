@@ -7,7 +7,8 @@
 // the runs in LDS in region order (x fastest), and selects the particles the tile owns from
 // the staged copies.  For an owned particle the three cells (x - 1 .. x + 1) of each of its
 // nine (dz, dy) rows are consecutive region cells, so each row of the 27-cell walk is one
-// contiguous LDS range, visited in the reference's order.
+// contiguous LDS range, visited in the reference's order.  A region larger than the LDS
+// capacity streams through it in groups of rows (see plan_groups).
 //
 // Density: a row is first scanned with the distance test only, and the candidates that
 // pass (d2 <= R2_pre) are queued per lane in LDS; the queue is drained in the order it was
@@ -15,13 +16,14 @@
 // and at the end) with the density terms and the force-list entries.  Every particle's sums
 // thus take the reference's candidates in the reference's order, bit for bit, while the
 // in-range arithmetic runs with most lanes of the wave active.  The force-list entries are
-// 16-bit indices into the staged region; the force pass stages the same region (position
-// + near density, velocity + density per entry) and reads its neighbours from LDS.
+// either cell-ordered slots, read by the per-lane force pass, or 16-bit indices into the
+// staged region, read by the tiled force pass, which stages the same region (position +
+// near density, velocity + density per entry) and reads its neighbours from LDS.
 //
 // The particles no tile owns (the placement's strays) take one lane each in the first
 // workgroups of each tiled launch, walking their cells through global memory beside the
-// tiles.  A chunk too large for LDS (see plan_chunks) hands its particles over to the top of
-// the stray list, which a stray kernel after each tiled pass walks the same way.
+// tiles.  A tile that cannot be staged (see plan_groups) hands its particles over to the top
+// of the stray list, which a stray kernel after each tiled pass walks the same way.
 #include <algorithm>
 
 #include "sph_device.hpp"
@@ -113,16 +115,6 @@ __device__ __forceinline__ uint32_t region_runs(const SceneConst &c, const Chain
     if (tid == 0) s_off[rg.cells] = (int)total;
     __syncthreads();
     return total;
-}
-
-// Region cell of staged entry e (s_off[rc] <= e < s_off[rc + 1]).
-__device__ __forceinline__ uint32_t region_cell_of(const int *s_off, uint32_t cells, int e) {
-    uint32_t lo = 0, hi = cells;
-    while (hi - lo > 1u) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (s_off[mid] <= e) lo = mid; else hi = mid;
-    }
-    return lo;
 }
 
 // Squared distance for the density pass's candidate filter: fused multiply-adds, within 6
@@ -227,172 +219,126 @@ __device__ __forceinline__ void force_store(const SceneConst &c, const ChainBuff
     b.vel[o] = vo;
 }
 
-// Own-selection and staging of a tile, shared by both passes: run lookups, then (when the
-// region fits kE entries) every staged entry through `stage(e, g)` and the home copies of the
-// owned particles into s_own_e (staged index) and s_own_g (slot); else the owned slots alone
-// into s_glob.  Returns the region's entry count; *owned_out the owned particles (more than
-// kOwnMax: the tile takes the global path).
-// Chunks of a tile.  A tile whose staged region would exceed the scene's tile_emax entries
-// is split along x: chunk [a, b) owns the particles whose predicted cell has region x in
-// [a, b) and stages region columns a - 1 .. b (its cells grown by one per side), as many
-// columns as fit.  A chunk of one column that still does not fit, or that owns more than
-// kOwnMax particles, hands its particles over to the stray kernels.  Both passes plan the
-// same chunks from the same run lengths, so a force list's indices into a chunk's staging
-// stay valid.
-constexpr int kChunkMax = 66;  // tw <= 64
-constexpr int kRowMax = 16;    // (th + 2) (td + 2) <= 16
+// Row groups of a tile.  The region is staged in LDS one group of consecutive region rows
+// (a row: the rx cells of one (y, z), x fastest; rows numbered y fastest) at a time, as
+// many rows as fit the scene's tile_emax entries.  A particle's nine walk rows have
+// increasing region-row indices in the walk's order ((cz - 1 + dz) ry + (cy - 1 + dy), with
+// ry >= 3), so walking each group's rows in turn, the sums carried from group to group,
+// keeps the reference's order while every owned particle walks every group.  A row that
+// alone does not fit, more owned particles than kOwnMax, or more than one group where the
+// force lists index the staging hands the tile's particles over to the stray list.
+constexpr int kGroupMax = 64;  // region rows (th + 2) (td + 2) a tile may have
 
 struct TileLds {
     int s_off[kTileRegionMax + 1];  // exclusive prefix of the region cells' run lengths
     int s_beg[kTileRegionMax];      // first cell-ordered slot of each region cell's run
-    int s_col[kChunkMax];           // entries per region column (x)
-    int s_rowoff[kRowMax + 1];      // the current chunk's rows: first staged index
-    int s_delta[kRowMax];           // staged index - region index, per row
-    uint8_t s_ca[kChunkMax], s_cb[kChunkMax], s_cover[kChunkMax];
+    uint8_t s_g0[kGroupMax + 1];    // row groups: group k holds rows [s_g0[k], s_g0[k + 1])
     uint32_t s_wave[16];
-    uint32_t s_nch, s_count;
+    uint32_t s_ng, s_count;
 };
 
+// Row groups of a region of `total` entries (s_off filled): their count, 0 when the tile is
+// handed over (a row alone exceeds emax, or `single` and the region does not fit at once).
+// Clears the owned count; ends with a barrier.
 template <int kB>
-__device__ __forceinline__ void plan_chunks(const TileRegion &rg, int total, int emax, TileLds &L) {
-    const uint32_t tid = threadIdx.x;
-    const uint32_t rows = rg.ry * rg.rz;
-    if (total <= emax) {
-        if (tid == 0) {
-            L.s_ca[0] = 1;
-            L.s_cb[0] = (uint8_t)(rg.rx - 1u);
-            L.s_cover[0] = 0;
-            L.s_nch = 1;
+__device__ __forceinline__ uint32_t plan_groups(const TileRegion &rg, int total, int emax, bool single, TileLds &L) {
+    if (threadIdx.x == 0) {
+        L.s_count = 0u;
+        const uint32_t rows = rg.ry * rg.rz;
+        uint32_t ng = 0;
+        L.s_g0[0] = 0;
+        if (total <= emax) {
+            ng = 1;
+        } else if (!single) {
+            int base = 0;  // first entry of the open group
+            for (uint32_t r = 0; r < rows; ++r) {
+                const int lo = L.s_off[r * rg.rx], hi = L.s_off[(r + 1u) * rg.rx];
+                if (hi - lo > emax) {
+                    ng = 0;
+                    break;
+                }
+                if (hi - base > emax) {  // row r opens the next group
+                    L.s_g0[++ng] = (uint8_t)r;
+                    base = lo;
+                }
+                if (r + 1u == rows) ++ng;
+            }
         }
-        __syncthreads();
-        return;
-    }
-    for (uint32_t x = tid; x < rg.rx; x += kB) {
-        int sum = 0;
-        for (uint32_t r = 0; r < rows; ++r) sum += L.s_off[r * rg.rx + x + 1u] - L.s_off[r * rg.rx + x];
-        L.s_col[x] = sum;
+        if (ng) L.s_g0[ng] = (uint8_t)rows;
+        L.s_ng = ng;
     }
     __syncthreads();
-    if (tid == 0) {
-        const uint32_t tw = rg.rx - 2u;
-        uint32_t nch = 0;
-        for (uint32_t a = 1; a <= tw;) {
-            int e = L.s_col[a - 1u] + L.s_col[a] + L.s_col[a + 1u];
-            uint32_t b = a + 1u;
-            while (b <= tw && e + L.s_col[b + 1u] <= emax) e += L.s_col[++b];
-            L.s_ca[nch] = (uint8_t)a;
-            L.s_cb[nch] = (uint8_t)b;
-            L.s_cover[nch] = e > emax;
-            ++nch;
-            a = b;
-        }
-        L.s_nch = nch;
-    }
-    __syncthreads();
+    return L.s_ng;
 }
 
-// Staging layout of chunk [a, b): per region row its columns a - 1 .. b, rows in region
-// order.  Returns the chunk's entry count; ends with a barrier.
-__device__ __forceinline__ int chunk_rows(const TileRegion &rg, uint32_t a, uint32_t b, TileLds &L) {
-    const uint32_t tid = threadIdx.x;
-    const uint32_t rows = rg.ry * rg.rz;
-    if (tid < 64u) {
-        const uint32_t r = tid;
-        int len = 0, first = 0;
-        if (r < rows) {
-            first = L.s_off[r * rg.rx + a - 1u];
-            len = L.s_off[r * rg.rx + b + 1u] - first;
-        }
-        int x = len;
-#pragma unroll
-        for (int o = 1; o < 32; o <<= 1) {
-            const int y = __shfl_up(x, o, 64);
-            if ((int)r >= o) x += y;
-        }
-        if (r < rows) {
-            L.s_rowoff[r] = x - len;
-            L.s_delta[r] = x - len - first;
-            if (r == rows - 1u) L.s_rowoff[rows] = x;
-        }
-    }
-    __syncthreads();
-    return L.s_rowoff[rows];
-}
-
-// Region cell and slot of staged entry e of chunk [a, b).
-__device__ __forceinline__ void chunk_entry(const TileRegion &rg, uint32_t a, uint32_t b, const TileLds &L, int e,
-                                            uint32_t &rc, uint32_t &g) {
-    const uint32_t rows = rg.ry * rg.rz;
-    uint32_t lo = 0, hi = rows;  // s_rowoff[lo] <= e < s_rowoff[hi]
-    while (hi - lo > 1u) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (L.s_rowoff[mid] <= e) lo = mid; else hi = mid;
-    }
-    const int ef = e - L.s_delta[lo];
-    uint32_t c0 = lo * rg.rx + a - 1u, c1 = lo * rg.rx + b + 1u;  // s_off[c0] <= ef < s_off[c1]
+// Region cell of region entry e among cells [c0, c1) (s_off[c0] <= e < s_off[c1]).
+__device__ __forceinline__ uint32_t region_cell_in(const int *s_off, uint32_t c0, uint32_t c1, int e) {
     while (c1 - c0 > 1u) {
         const uint32_t mid = (c0 + c1) >> 1;
-        if (L.s_off[mid] <= ef) c0 = mid; else c1 = mid;
+        if (s_off[mid] <= e) c0 = mid; else c1 = mid;
     }
-    rc = c0;
-    g = (uint32_t)(L.s_beg[c0] + (ef - L.s_off[c0]));
+    return c0;
 }
 
-__device__ __forceinline__ bool owns_in_chunk(const uint2 &ow, uint32_t t, uint32_t rc, uint32_t a, uint32_t b) {
-    const uint32_t px = ow.y >> 16;
-    return ow.x == t && (ow.y & 0xFFFFu) == rc && px >= a && px < b;
-}
-
-// Stages chunk [a, b) (entries [0, total_chunk) through `stage`) and selects its owned
-// particles' home copies into s_own_e / s_own_g; ends with a barrier.
-template <int kB, int kOwnMax, class Stage>
-__device__ __forceinline__ void stage_chunk(const ChainBuffers &b, uint32_t t, const TileRegion &rg, uint32_t a,
-                                            uint32_t bx, int total, TileLds &L, uint16_t *s_own_e,
-                                            uint32_t *s_own_g, Stage stage) {
+// Stages region entries [e0, e1) (those of cells [c0, c1)) through `stage(k, g)` at staged
+// index k = e - e0 (g: the entry's cell-ordered slot).  With kSelect, also records the home
+// copies of the tile's owned particles: s_own_e their region entry, s_own_g (if given) their
+// slot.  Ends with a barrier.
+template <int kB, int kOwnMax, bool kSelect, class Stage>
+__device__ __forceinline__ void stage_group(const ChainBuffers &b, uint32_t t, uint32_t c0, uint32_t c1, int e0,
+                                            int e1, TileLds &L, uint16_t *s_own_e, uint32_t *s_own_g,
+                                            Stage stage) {
     constexpr int kPer = 5;  // a lane's loads are issued before any is used
     const int tid = (int)threadIdx.x;
-    for (int e0 = 0; e0 < total; e0 += kPer * kB) {
+    const int cnt = e1 - e0;
+    for (int k0 = 0; k0 < cnt; k0 += kPer * kB) {
         uint32_t rc[kPer], g[kPer];
         uint2 ow[kPer];
 #pragma unroll
-        for (int k = 0; k < kPer; ++k) {
-            const int e = e0 + tid + k * kB;
-            rc[k] = 0u;
-            g[k] = 0u;
-            if (e < total) chunk_entry(rg, a, bx, L, e, rc[k], g[k]);
-        }
-#pragma unroll
-        for (int k = 0; k < kPer; ++k) {
-            const int e = e0 + tid + k * kB;
-            if (e < total) {
-                stage(e, g[k]);
-                ow[k] = b.towner[g[k]];
+        for (int u = 0; u < kPer; ++u) {
+            const int k = k0 + tid + u * kB;
+            rc[u] = 0u;
+            g[u] = 0u;
+            if (k < cnt) {
+                rc[u] = region_cell_in(L.s_off, c0, c1, e0 + k);
+                g[u] = (uint32_t)(L.s_beg[rc[u]] + (e0 + k - L.s_off[rc[u]]));
             }
         }
 #pragma unroll
-        for (int k = 0; k < kPer; ++k) {
-            const int e = e0 + tid + k * kB;
-            if (e - tid >= total) break;  // the same for every lane of the block
-            const bool own = e < total && owns_in_chunk(ow[k], t, rc[k], a, bx);
-            select_own<kOwnMax>(own, (uint32_t)e, g[k], &L.s_count, s_own_e, s_own_g);
+        for (int u = 0; u < kPer; ++u) {
+            const int k = k0 + tid + u * kB;
+            if (k < cnt) {
+                stage(k, g[u]);
+                if (kSelect) ow[u] = b.towner[g[u]];
+            }
+        }
+        if (kSelect) {
+#pragma unroll
+            for (int u = 0; u < kPer; ++u) {
+                const int k = k0 + tid + u * kB;
+                if (k - tid >= cnt) break;  // the same for every lane of the block
+                const bool own = k < cnt && ow[u].x == t && ow[u].y == rc[u];
+                select_own<kOwnMax>(own, (uint32_t)(e0 + k), g[u], &L.s_count, s_own_e, s_own_g);
+            }
         }
     }
     __syncthreads();
 }
 
-// Hands the owned particles of chunk [a, b) over to the stray list (its staged region does
-// not fit, or it owns too many).
+// Hands the tile's owned particles over to the stray list, from its top (the placement's
+// strays fill it from the bottom).
 template <int kB>
 __device__ __forceinline__ void hand_over(const ChainBuffers &b, uint32_t n, uint32_t t, const TileRegion &rg,
-                                          uint32_t a, uint32_t bx, int region_total, const TileLds &L) {
+                                          int total, const TileLds &L) {
     const uint32_t tid = threadIdx.x;
-    for (int e0 = 0; e0 < region_total; e0 += kB) {
+    for (int e0 = 0; e0 < total; e0 += kB) {
         const int e = e0 + (int)tid;
         uint32_t j = kNoTile;
-        if (e < region_total) {
-            const uint32_t rc = region_cell_of(L.s_off, rg.cells, e);
+        if (e < total) {
+            const uint32_t rc = region_cell_in(L.s_off, 0u, rg.cells, e);
             const uint32_t g = (uint32_t)(L.s_beg[rc] + (e - L.s_off[rc]));
-            if (owns_in_chunk(b.towner[g], t, rc, a, bx)) j = g;
+            const uint2 ow = b.towner[g];
+            if (ow.x == t && ow.y == rc) j = g;
         }
         const bool mine = j != kNoTile;
         const uint64_t m = __ballot(mine);
@@ -402,17 +348,108 @@ __device__ __forceinline__ void hand_over(const ChainBuffers &b, uint32_t n, uin
         uint32_t base = 0;
         if (lane == leader) base = atomicAdd(&b.counters[7], (uint32_t)__popcll(m));
         base = __shfl(base, (int)leader, 64);
-        // from the top of the stray list (the placement's strays fill it from the bottom)
         if (mine) b.stray[n - 1u - (base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull)))] = j;
     }
 }
 
+// The density walk of one owned particle over the rows of group [r0, r1), staged from
+// region entry e0: S = (predicted position xyz, slot); M, Mn, nl carry its sums and list
+// length from group to group.
+//
+// Rows 0..8 of the walk, then a final pass (row 9, empty) that only drains; rows outside the
+// group are empty.  A row's candidates are filtered with a fused-multiply-add distance
+// (within 6 ulp of the reference's d2, against a bound 2^-20 above its cut-off, so no
+// candidate the reference keeps is dropped) and queued without branches (the entry is
+// written, the count advances when it passes), in chunks that fit the queue.  The drain
+// recomputes d2 exactly and applies the reference's tests.  A wall particle drains at the
+// end of each row, while that row's density cells are known.
+template <int kB, bool kF, bool kS>
+__device__ __forceinline__ void walk_group(const SceneConst &c, const ChainBuffers &b, uint32_t n,
+                                           const TileRegion &rg, const TileLds &L, const float4 *s_pos, uint16_t *q,
+                                           const float4 &S, uint32_t r0, uint32_t r1, int e0, float &M, float &Mn,
+                                           uint32_t &nl) {
+    const uint32_t cap = b.list_cap;
+    uint16_t *__restrict__ nl16 = b.nlist16();
+    const float thr = c.R2_pre * (1.0f + 0x1p-20f);
+    const uint32_t j = __float_as_uint(S.w);
+    const float4 s = make_float4(S.x, S.y, S.z, 1.0f);
+    const AxisCells ax = sample_axis(s.x, c.minx, c.maxx, c.R);
+    const AxisCells ay = sample_axis(s.y, c.miny, c.maxy, c.R);
+    const AxisCells az = sample_axis(s.z, c.minz, c.maxz, c.R);
+    // region coordinates of the predicted cell, each in [1, tile extent]
+    const uint32_t cx = (uint32_t)((int32_t)ax.mid - rg.ox);
+    const uint32_t cy = (uint32_t)((int32_t)ay.mid - rg.oy);
+    const uint32_t cz = (uint32_t)((int32_t)az.mid - rg.oz);
+    // interior: every sample point of the 27 cells lies inside the box, so every candidate
+    // counts for the density (else only those of the row's density cells)
+    const bool interior = (ax.inmask & ay.inmask & az.inmask) == 7u;
+    int qc = 0;
+    for (uint32_t r = 0; r <= 9u; ++r) {
+        int e = 0, o3 = 0, mlo = 0, mhi = 0;
+        if (r < 9u) {
+            const uint32_t dz = r / 3u, dy = r % 3u;
+            const uint32_t row = (cz - 1u + dz) * rg.ry + (cy - 1u + dy);
+            if (row >= r0 && row < r1) {
+                const uint32_t rc0 = row * rg.rx + (cx - 1u);
+                const int o0 = L.s_off[rc0] - e0;
+                o3 = L.s_off[rc0 + 3u] - e0;
+                e = o0;
+                if (!interior) {
+                    // the three force cells whose sample points are inside the box (the
+                    // middle one always is)
+                    const bool row_in = ay.in((int)dy) && az.in((int)dz);
+                    mlo = row_in ? (ax.in(0) ? o0 : L.s_off[rc0 + 1u] - e0) : o3;
+                    mhi = row_in ? (ax.in(2) ? o3 : L.s_off[rc0 + 2u] - e0) : o3;
+                }
+            }
+        }
+        for (;;) {
+            const bool need = qc > 0 && (r == 9u || qc >= kTFlush || (!interior && e >= o3));
+            if (__ballot(need)) {
+                for (int k = 0; k < qc; ++k) {
+                    const int ec = q[k * kB];
+                    const float4 p = s_pos[ec];
+                    const float d2 = dist2(p, s);
+                    if (interior || (ec >= mlo && ec < mhi)) density_terms<kF>(c, d2, M, Mn);
+                    const uint32_t slot = __float_as_uint(p.w);
+                    if (slot != j && !(d2 > c.R2)) {
+                        if (nl < cap) {
+                            // slot lists, or 16-bit region entries (one group: e0 = 0)
+                            if (kS) b.nlist[(size_t)nl * n + j] = slot;
+                            else nl16[(size_t)nl * n + j] = (uint16_t)ec;
+                        }
+                        ++nl;
+                    }
+                }
+                qc = 0;
+            }
+            if (!__ballot(e < o3)) break;
+            const int lim = min(o3, e + (kTQ - qc));
+            for (; e + 3 < lim; e += 4) {
+                float f[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) f[u] = dist2_filter(s_pos[e + u], s);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    q[qc * kB] = (uint16_t)(e + u);
+                    qc += f[u] > thr ? 0 : 1;
+                }
+            }
+            for (; e < lim; ++e) {
+                q[qc * kB] = (uint16_t)e;
+                qc += dist2_filter(s_pos[e], s) > thr ? 0 : 1;
+            }
+        }
+    }
+}
+
 // kS: force lists of cell-ordered slots (for the per-lane force pass), else of 16-bit
-// staged indices (for the tiled one).
+// region entries (for the tiled one).
 template <class Scenes, int kB, int kE, bool kF, bool kS>
 __global__ __launch_bounds__(kB) void k_density_tile(Scenes sc, const ChainBuffers b, uint32_t n, uint32_t lead,
                                                      uint32_t tile_total) {
     constexpr int kOwnMax = 2 * kB;
+    constexpr int kSlots = kOwnMax / kB;  // owned particles per lane
     __shared__ float4 s_pos[kE];  // (predicted position xyz, cell-ordered slot)
     __shared__ TileLds L;
     __shared__ uint16_t s_own[kOwnMax];
@@ -435,123 +472,80 @@ __global__ __launch_bounds__(kB) void k_density_tile(Scenes sc, const ChainBuffe
     if (!c.use_tiles) return;
     const TileRegion rg = tile_region(c, t - c.tile_base);
     const int emax = min(kE, (int)c.tile_emax);
-    const int region_total = (int)region_runs<kB>(c, b, rg, L.s_off, L.s_beg, L.s_wave);
-    if (region_total == 0) return;
-    plan_chunks<kB>(rg, region_total, emax, L);
-    if (tid == 0 && L.s_nch > 1u) atomicAdd(&b.counters[5], 1u);
-    const uint32_t cap = b.list_cap;
-    uint16_t *__restrict__ nl16 = b.nlist16();
-    uint16_t *q = s_q + tid;
-    const float thr = c.R2_pre * (1.0f + 0x1p-20f);
-    const uint32_t nch = L.s_nch;
-    for (uint32_t ch = 0; ch < nch; ++ch) {
-        const uint32_t ca = L.s_ca[ch], cb = L.s_cb[ch];
-        if (L.s_cover[ch]) {
-            if (tid == 0) atomicAdd(&b.counters[6], 1u);
-            hand_over<kB>(b, n, t, rg, ca, cb, region_total, L);
-            continue;
-        }
-        if (tid == 0) L.s_count = 0u;
-        const int total = chunk_rows(rg, ca, cb, L);
-        stage_chunk<kB, kOwnMax>(b, t, rg, ca, cb, total, L, s_own, nullptr, [&](int e, uint32_t g) {
-            const float4 p = b.pred_pos[g];
-            s_pos[e] = make_float4(p.x, p.y, p.z, __uint_as_float(g));
-        });
-        const uint32_t owned = L.s_count;
-        if (owned > (uint32_t)kOwnMax) {
-            if (tid == 0) atomicAdd(&b.counters[6], 1u);
-            hand_over<kB>(b, n, t, rg, ca, cb, region_total, L);
-            __syncthreads();
-            continue;
-        }
-        for (uint32_t i = tid; i < owned; i += kB) {
-            const int eo = s_own[i];
-            const float4 S = s_pos[eo];
-            const uint32_t j = __float_as_uint(S.w);
-            const float4 s = make_float4(S.x, S.y, S.z, 1.0f);
-            const AxisCells ax = sample_axis(s.x, c.minx, c.maxx, c.R);
-            const AxisCells ay = sample_axis(s.y, c.miny, c.maxy, c.R);
-            const AxisCells az = sample_axis(s.z, c.minz, c.maxz, c.R);
-            // region coordinates of the predicted cell, each in [1, tile extent]
-            const uint32_t cx = (uint32_t)((int32_t)ax.mid - rg.ox);
-            const uint32_t cy = (uint32_t)((int32_t)ay.mid - rg.oy);
-            const uint32_t cz = (uint32_t)((int32_t)az.mid - rg.oz);
-            // interior: every sample point of the 27 cells lies inside the box, so every
-            // candidate counts for the density (else only those of the row's density cells)
-            const bool interior = (ax.inmask & ay.inmask & az.inmask) == 7u;
-            float M = 0.0f, Mn = 0.0f;
-            uint32_t nl = 0;
-            int qc = 0;
-            // Rows 0..8 of the walk, then a final pass (row 9, empty) that only drains.  A
-            // row's candidates are filtered with a fused-multiply-add distance (within 6 ulp
-            // of the reference's d2, against a bound 2^-20 above its cut-off, so no candidate
-            // the reference keeps is dropped) and queued without branches (the entry is
-            // written, the count advances when it passes), in chunks that fit the queue.  The
-            // drain recomputes d2 exactly and applies the reference's tests.  A wall particle
-            // drains at the end of each row, while that row's density cells are known.
-            for (uint32_t r = 0; r <= 9u; ++r) {
-                int e = 0, o3 = 0, mlo = 0, mhi = 0;
-                if (r < 9u) {
-                    const uint32_t dz = r / 3u, dy = r % 3u;
-                    const uint32_t row = (cz - 1u + dz) * rg.ry + (cy - 1u + dy);
-                    const uint32_t rc0 = row * rg.rx + (cx - 1u);
-                    const int d = L.s_delta[row];
-                    const int o0 = L.s_off[rc0] + d;
-                    o3 = L.s_off[rc0 + 3u] + d;
-                    e = o0;
-                    if (!interior) {
-                        // the three force cells whose sample points are inside the box
-                        // (the middle one always is)
-                        const bool row_in = ay.in((int)dy) && az.in((int)dz);
-                        mlo = row_in ? (ax.in(0) ? o0 : L.s_off[rc0 + 1u] + d) : o3;
-                        mhi = row_in ? (ax.in(2) ? o3 : L.s_off[rc0 + 2u] + d) : o3;
-                    }
-                }
-                for (;;) {
-                    const bool need = qc > 0 && (r == 9u || qc >= kTFlush || (!interior && e >= o3));
-                    if (__ballot(need)) {
-                        for (int k = 0; k < qc; ++k) {
-                            const int ec = q[k * kB];
-                            const float4 p = s_pos[ec];
-                            const float d2 = dist2(p, s);
-                            if (interior || (ec >= mlo && ec < mhi)) density_terms<kF>(c, d2, M, Mn);
-                            const uint32_t slot = __float_as_uint(p.w);
-                            if (slot != j && !(d2 > c.R2)) {
-                                if (nl < cap) {
-                                    if (kS) b.nlist[(size_t)nl * n + j] = slot;
-                                    else nl16[(size_t)nl * n + j] = (uint16_t)ec;
-                                }
-                                ++nl;
-                            }
-                        }
-                        qc = 0;
-                    }
-                    if (!__ballot(e < o3)) break;
-                    const int lim = min(o3, e + (kTQ - qc));
-                    for (; e + 3 < lim; e += 4) {
-                        float f[4];
+    const int total = (int)region_runs<kB>(c, b, rg, L.s_off, L.s_beg, L.s_wave);
+    if (total == 0) return;
+    // 16-bit entries (s_own, the queue, 16-bit lists) cover the region; 16-bit lists index
+    // a single staging
+    const uint32_t ng = total > 65535 ? 0u : plan_groups<kB>(rg, total, emax, !kS, L);
+    if (ng == 0u) {
+        if (tid == 0) atomicAdd(&b.counters[6], 1u);
+        hand_over<kB>(b, n, t, rg, total, L);
+        return;
+    }
+    if (tid == 0 && ng > 1u) atomicAdd(&b.counters[5], 1u);
+    auto stage_pos = [&](int k, uint32_t g) {
+        const float4 p = b.pred_pos[g];
+        s_pos[k] = make_float4(p.x, p.y, p.z, __uint_as_float(g));
+    };
+    // one group: staged with the owned particles' selection; else the selection alone
+    if (ng == 1u)
+        stage_group<kB, kOwnMax, true>(b, t, 0u, rg.cells, 0, total, L, s_own, nullptr, stage_pos);
+    else
+        stage_group<kB, kOwnMax, true>(b, t, 0u, rg.cells, 0, total, L, s_own, nullptr, [](int, uint32_t) {});
+    const uint32_t owned = L.s_count;
+    if (owned > (uint32_t)kOwnMax) {
+        if (tid == 0) atomicAdd(&b.counters[6], 1u);
+        hand_over<kB>(b, n, t, rg, total, L);
+        return;
+    }
+    // the lane's owned particles tid + q kB: (predicted position, slot) and running sums
+    float4 S[kSlots];
+    float M[kSlots], Mn[kSlots];
+    uint32_t nl[kSlots];
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) f[u] = dist2_filter(s_pos[e + u], s);
-#pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            q[qc * kB] = (uint16_t)(e + u);
-                            qc += f[u] > thr ? 0 : 1;
-                        }
-                    }
-                    for (; e < lim; ++e) {
-                        q[qc * kB] = (uint16_t)e;
-                        qc += dist2_filter(s_pos[e], s) > thr ? 0 : 1;
-                    }
-                }
+    for (int q = 0; q < kSlots; ++q) {
+        M[q] = 0.0f;
+        Mn[q] = 0.0f;
+        nl[q] = 0u;
+        S[q] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const uint32_t i = tid + (uint32_t)q * kB;
+        if (i < owned) {
+            const int e = s_own[i];
+            if (ng == 1u) {
+                S[q] = s_pos[e];
+            } else {
+                const uint32_t rc = region_cell_in(L.s_off, 0u, rg.cells, e);
+                const uint32_t g = (uint32_t)(L.s_beg[rc] + (e - L.s_off[rc]));
+                const float4 p = b.pred_pos[g];
+                S[q] = make_float4(p.x, p.y, p.z, __uint_as_float(g));
             }
-            const float2 d = density_finish(c, M, Mn);
-            record_longest(b, nl);
-            b.dens[j] = d;
-            reinterpret_cast<float *>(b.vel_sorted + j)[3] = d.x;
-            reinterpret_cast<float *>(b.pred_pos + j)[3] = d.y;
-            b.ncount[j] = nl <= cap ? (uint8_t)nl : kListInvalid;
         }
-        __syncthreads();  // the next chunk restages the LDS
+    }
+    for (uint32_t gi = 0; gi < ng; ++gi) {
+        const uint32_t r0 = L.s_g0[gi], r1 = L.s_g0[gi + 1u];
+        const int e0 = L.s_off[r0 * rg.rx];
+        if (ng > 1u) {
+            if (gi > 0u) __syncthreads();  // the previous group's walks are done with s_pos
+            stage_group<kB, kOwnMax, false>(b, t, r0 * rg.rx, r1 * rg.rx, e0, L.s_off[r1 * rg.rx], L, nullptr,
+                                            nullptr, stage_pos);
+        }
+#pragma unroll
+        for (int q = 0; q < kSlots; ++q) {
+            if (tid + (uint32_t)q * kB >= owned) break;
+            walk_group<kB, kF, kS>(c, b, n, rg, L, s_pos, s_q + tid, S[q], r0, r1, e0, M[q], Mn[q], nl[q]);
+        }
+    }
+    const uint32_t cap = b.list_cap;
+#pragma unroll
+    for (int q = 0; q < kSlots; ++q) {
+        if (tid + (uint32_t)q * kB >= owned) break;
+        const uint32_t j = __float_as_uint(S[q].w);
+        const float2 d = density_finish(c, M[q], Mn[q]);
+        record_longest(b, nl[q]);
+        b.dens[j] = d;
+        reinterpret_cast<float *>(b.vel_sorted + j)[3] = d.x;
+        reinterpret_cast<float *>(b.pred_pos + j)[3] = d.y;
+        b.ncount[j] = nl[q] <= cap ? (uint8_t)nl[q] : kListInvalid;
     }
 }
 
@@ -586,79 +580,67 @@ __global__ __launch_bounds__(kB) void k_force_tile(Scenes sc, const ChainBuffers
     if (!c.use_tiles) return;
     const TileRegion rg = tile_region(c, t - c.tile_base);
     const int emax = min(kE, (int)c.tile_emax);
-    const int region_total = (int)region_runs<kB>(c, b, rg, L.s_off, L.s_beg, L.s_wave);
-    if (region_total == 0) return;
-    plan_chunks<kB>(rg, region_total, emax, L);
+    const int total = (int)region_runs<kB>(c, b, rg, L.s_off, L.s_beg, L.s_wave);
+    if (total == 0) return;
+    // the density pass's plan: a region that does not fit at once was handed over
+    if (total > 65535 || plan_groups<kB>(rg, total, emax, true, L) != 1u) return;
+    stage_group<kB, kOwnMax, true>(b, t, 0u, rg.cells, 0, total, L, s_own_e, s_own_g, [&](int k, uint32_t g) {
+        s_a[k] = b.pred_pos[g];
+        s_b[k] = b.vel_sorted[g];
+    });
+    const uint32_t owned = L.s_count;
+    if (owned > (uint32_t)kOwnMax) return;  // handed over by the density pass
     const uint16_t *__restrict__ nl16 = b.nlist16();
-    const uint32_t nch = L.s_nch;
-    for (uint32_t ch = 0; ch < nch; ++ch) {
-        if (L.s_cover[ch]) continue;  // handed over by the density pass: strays
-        const uint32_t ca = L.s_ca[ch], cb = L.s_cb[ch];
-        if (tid == 0) L.s_count = 0u;
-        const int total = chunk_rows(rg, ca, cb, L);
-        stage_chunk<kB, kOwnMax>(b, t, rg, ca, cb, total, L, s_own_e, s_own_g, [&](int e, uint32_t g) {
-            s_a[e] = b.pred_pos[g];
-            s_b[e] = b.vel_sorted[g];
-        });
-        const uint32_t owned = L.s_count;
-        if (owned > (uint32_t)kOwnMax) {
-            __syncthreads();
-            continue;  // handed over by the density pass
-        }
-        for (uint32_t i = tid; i < owned; i += kB) {
-            const uint32_t j = s_own_g[i];
-            const uint32_t cnt = b.ncount[j];
-            const int eo = s_own_e[i];
-            const float4 A = s_a[eo], V = s_b[eo];
-            const float4 s = make_float4(A.x, A.y, A.z, 1.0f);
-            const float4 P = b.pos_sorted[j];
-            const float2 myd = make_float2(V.w, A.w);
-            const ForceSelf me = force_self(c, j, s, P, V, myd);
-            float3 Fp = make_float3(0.0f, 0.0f, 0.0f), Fv = make_float3(0.0f, 0.0f, 0.0f);
-            if (cnt != kListInvalid) {
-                // the recorded neighbours, in the walk's order: eight indices loaded together,
-                // then their terms (the loads' latency is paid once per eight)
-                for (uint32_t k0 = 0; k0 < cnt; k0 += 8u) {
-                    uint32_t ix[8];
+    for (uint32_t i = tid; i < owned; i += kB) {
+        const uint32_t j = s_own_g[i];
+        const uint32_t cnt = b.ncount[j];
+        const int eo = s_own_e[i];
+        const float4 A = s_a[eo], V = s_b[eo];
+        const float4 s = make_float4(A.x, A.y, A.z, 1.0f);
+        const float4 P = b.pos_sorted[j];
+        const float2 myd = make_float2(V.w, A.w);
+        const ForceSelf me = force_self(c, j, s, P, V, myd);
+        float3 Fp = make_float3(0.0f, 0.0f, 0.0f), Fv = make_float3(0.0f, 0.0f, 0.0f);
+        if (cnt != kListInvalid) {
+            // the recorded neighbours, in the walk's order: eight indices loaded together,
+            // then their terms (the loads' latency is paid once per eight)
+            for (uint32_t k0 = 0; k0 < cnt; k0 += 8u) {
+                uint32_t ix[8];
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) ix[u] = nl16[(size_t)min(k0 + (uint32_t)u, cnt - 1u) * n + j];
+                for (int u = 0; u < 8; ++u) ix[u] = nl16[(size_t)min(k0 + (uint32_t)u, cnt - 1u) * n + j];
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        if (k0 + (uint32_t)u >= cnt) break;
-                        const float4 a = s_a[ix[u]], v = s_b[ix[u]];
-                        force_pair<kF>(c, me, make_float3(a.x, a.y, a.z), make_float3(v.x, v.y, v.z),
-                                       make_float2(v.w, a.w), Fp, Fv);
-                    }
-                }
-            } else {
-                // a list that did not fit: the 27 cells from the staged chunk
-                // (applyPressure...comp:288-325).  The particle's own copies need no slot
-                // test: at distance 0 the term is skipped (d < 1e-7).
-                const int3 c0 = force_cell(c, s);
-                const uint32_t cx = (uint32_t)(c0.x - rg.ox), cy = (uint32_t)(c0.y - rg.oy);
-                const uint32_t cz = (uint32_t)(c0.z - rg.oz);
-                for (uint32_t r = 0; r < 9u; ++r) {
-                    const uint32_t dz = r / 3u, dy = r % 3u;
-                    const uint32_t row = (cz - 1u + dz) * rg.ry + (cy - 1u + dy);
-                    const uint32_t rc0 = row * rg.rx + (cx - 1u);
-                    const int d = L.s_delta[row];
-                    const int o3 = L.s_off[rc0 + 3u] + d;
-                    for (int e = L.s_off[rc0] + d; e < o3; ++e) {
-                        const float4 a = s_a[e];
-                        if (dist2(a, s) > c.R2) continue;
-                        const float4 v = s_b[e];
-                        force_pair<kF>(c, me, make_float3(a.x, a.y, a.z), make_float3(v.x, v.y, v.z),
-                                       make_float2(v.w, a.w), Fp, Fv);
-                    }
+                for (int u = 0; u < 8; ++u) {
+                    if (k0 + (uint32_t)u >= cnt) break;
+                    const float4 a = s_a[ix[u]], v = s_b[ix[u]];
+                    force_pair<kF>(c, me, make_float3(a.x, a.y, a.z), make_float3(v.x, v.y, v.z),
+                                   make_float2(v.w, a.w), Fp, Fv);
                 }
             }
-            force_store(c, b, j, P, V, myd.x, Fp, Fv);
+        } else {
+            // a list that did not fit: the 27 cells from the staged region
+            // (applyPressure...comp:288-325).  The particle's own copies need no slot test: at
+            // distance 0 the term is skipped (d < 1e-7).
+            const int3 c0 = force_cell(c, s);
+            const uint32_t cx = (uint32_t)(c0.x - rg.ox), cy = (uint32_t)(c0.y - rg.oy);
+            const uint32_t cz = (uint32_t)(c0.z - rg.oz);
+            for (uint32_t r = 0; r < 9u; ++r) {
+                const uint32_t dz = r / 3u, dy = r % 3u;
+                const uint32_t rc0 = ((cz - 1u + dz) * rg.ry + (cy - 1u + dy)) * rg.rx + (cx - 1u);
+                const int o3 = L.s_off[rc0 + 3u];
+                for (int e = L.s_off[rc0]; e < o3; ++e) {
+                    const float4 a = s_a[e];
+                    if (dist2(a, s) > c.R2) continue;
+                    const float4 v = s_b[e];
+                    force_pair<kF>(c, me, make_float3(a.x, a.y, a.z), make_float3(v.x, v.y, v.z),
+                                   make_float2(v.w, a.w), Fp, Fv);
+                }
+            }
         }
-        __syncthreads();  // the next chunk restages the LDS
+        force_store(c, b, j, P, V, myd.x, Fp, Fv);
     }
 }
 
-// The particles of chunks handed over by the tiled density pass, one lane each, walking
+// The particles of tiles handed over by the tiled density pass, one lane each, walking
 // their cells through global memory after each tiled pass (recording slot lists when the
 // force pass reads them).
 constexpr int kStrayBlock = 256;
@@ -728,12 +710,12 @@ void tile_dims_for(SceneConst &c, double ppc, int cfg) {
     if ((uint32_t)c.gx < tw) tw = (uint32_t)c.gx;
     if ((uint32_t)c.gy < th) th = (uint32_t)c.gy;
     if ((uint32_t)c.gz < td) td = (uint32_t)c.gz;
-    // the kernels' LDS tables hold at most 64 columns (chunks), 16 rows and kTileRegionMax cells
-    if (tw > (uint32_t)kChunkMax - 2u) tw = (uint32_t)kChunkMax - 2u;
+    // the kernels' LDS tables hold at most kGroupMax rows and kTileRegionMax cells
+    if (tw > 64u) tw = 64u;
     c.tw = tw;
     c.th = th;
     c.td = td;
-    // staged entries beyond which a tile is split into chunks along x: the LDS capacity
+    // staged entries beyond which a tile's region streams in row groups: the LDS capacity
     c.tile_emax = (uint32_t)tc.entries;
     c.ntx = ((uint32_t)c.gx + tw - 1u) / tw;
     c.nty = ((uint32_t)c.gy + th - 1u) / th;

@@ -144,11 +144,12 @@ class World:
         return int(load_library().sph_tile_configs())
 
     def tile_stats(self) -> dict:
-        """What the last substep's tiled passes did: tiles, strays, chunked tiles and chunks
-        handed over to the strays (all 0 after other kernels)."""
+        """What the last substep's tiled passes did: tiles, strays, tiles streamed through
+        LDS in more than one group of rows, and tiles handed over to the strays (all 0 after
+        other kernels)."""
         out = (C.c_uint32 * 4)()
         check(self._lib.sph_world_tile_stats(self._h, out))
-        return dict(zip(("tiles", "strays", "chunked", "handed_over"), (int(v) for v in out)))
+        return dict(zip(("tiles", "strays", "streamed", "handed_over"), (int(v) for v in out)))
 
     def set_precision(self, mode: str) -> None:
         """"exact" (bit-identical to the reference's float order, the default) or

@@ -1,11 +1,12 @@
 """The tiled neighbour passes (SPH_KERNEL_TILE: cell tiles staged in LDS, sph_tiles.hip)
 against the FP32 oracle, bit for bit: every exact tile configuration on the dam break (one
 substep, every stage, and ten frames), the 1M scene over three frames (its first clamp
-piles particles onto the margin planes, so tiles split into chunks), lists pinned far
-below the neighbour counts (the force pass walks the cells instead), scenes whose local
-density is far above the mean the tiles were sized for (chunks along x; a single column
-too dense for LDS hands its particles to the stray kernels), the near-pressure term, and
-the tolerance configuration within 1e-5 of the FP64 oracle.  Configurations 0 and 1 pair
+piles particles onto the margin planes, so some tiles' regions exceed the LDS and stream
+through it in groups of rows), lists pinned far below the neighbour counts (the force pass
+walks the cells instead), scenes whose local density is far above the mean the tiles were
+sized for (row groups; a single row too dense for LDS hands the tile's particles to the
+stray kernels), the near-pressure term, and the tolerance configuration within 1e-5 of
+the FP64 oracle.  Configurations 0 and 1 pair
 the tiled density pass with the per-lane force pass over slot lists, 2 and 3 with the tiled
 force pass over staged-index lists; the cases that stress the force pass run both."""
 import numpy as np
@@ -57,7 +58,9 @@ def test_tiles_dam_break_every_stage_and_ten_frames(gpu, orc, config):
 @pytest.mark.parametrize("config", [0, 2])
 def test_tiles_uniform_1m_three_frames(gpu, orc, config):
     """Three frames of the 1M scene (T = 2^21) on tiles: the margin piles the first clamp
-    forms overflow their tiles' LDS, which then split into chunks; bit-exact throughout."""
+    forms overflow their tiles' LDS; with slot lists such a region streams through LDS in
+    groups of rows, with the tiled force pass the tile is handed over; bit-exact
+    throughout."""
     from sph_hip.workloads import UNIFORM_1M as W
     params = W.params(table_cells=1 << 21)
     pos, vel = W.state()
@@ -68,7 +71,7 @@ def test_tiles_uniform_1m_three_frames(gpu, orc, config):
         gp, gv = w.download()
         stats = w.tile_stats()
     print("tile stats after 3 frames:", stats, flush=True)
-    assert stats["chunked"] > 0, stats
+    assert stats["streamed" if config == 0 else "handed_over"] > 0, stats
     rp, rv = orc.step(params, pos, vel, 3)
     _assert_bitexact("pos", gp, rp)
     _assert_bitexact("vel", gv, rv)
@@ -78,7 +81,7 @@ def test_tiles_uniform_1m_three_frames(gpu, orc, config):
 @pytest.mark.parametrize("entries", [1, 16])
 def test_tiles_short_lists(gpu, orc, entries, config):
     """Lists pinned far below the dam break's neighbour counts: the force pass walks the
-    27 cells instead (through global memory, or of the staged chunk); ten frames
+    27 cells instead (through global memory, or of the staged region); ten frames
     bit-exact."""
     from sph_hip.workloads import DAM_BREAK_16K as W
     params = W.params()
@@ -108,8 +111,9 @@ def _background_and_block(gpu, n_bg, n_block, lo, hi):
 @pytest.mark.parametrize("case", ["chunks", "hand_over"])
 def test_tiles_dense_blocks(gpu, orc, case, config):
     """A uniform background plus a block far denser than the mean the tiles are sized for:
-    ~6 particles per cell (tiles crossing it split into chunks along x) or ~60 (one column's
-    region exceeds the LDS: its particles are handed over to the stray kernels).  Two
+    ~6 particles per cell (the regions of tiles crossing it stream through LDS in groups of
+    rows, or with the tiled force pass are handed over) or ~60 (a single row of the region
+    exceeds the LDS: the tile's particles are handed over to the stray kernels).  Two
     substeps bit-exact, and the statistics show the path taken."""
     if case == "chunks":
         params, pos, vel = _background_and_block(gpu, 8000, 4000, (4, 2, 1), (7, 5, 4))
@@ -124,7 +128,7 @@ def test_tiles_dense_blocks(gpu, orc, case, config):
         w.step(1)
         gp, gv = w.download()
     print("tile stats after one substep:", stats, flush=True)
-    assert stats["chunked"] > 0, stats
+    assert stats["streamed" if config == 0 else "handed_over"] > 0, stats
     if case == "hand_over":
         assert stats["handed_over"] > 0 and stats["strays"] > 0, stats
     rp, rv = orc.step(params, pos, vel, 2)
