@@ -694,7 +694,9 @@ void tile_dims_for(SceneConst &c, double ppc, int cfg) {
     const double occupied = std::min<double>((double)(c.p_end - c.p_begin), (double)c.gx * c.gy * c.gz);
     const double collide = 1.0 + 1.3 * occupied / ((double)c.mask + 1.0);
     const double stage_cap = 0.8 * tc.entries / collide;
-    static const uint32_t kShapes[3][2] = {{2, 2}, {2, 1}, {1, 1}};
+    // (th, td) in order of preference, the first that leaves tw >= 4: a taller cross-section
+    // stages fewer halo entries per owned particle (3 x 3 measured best on the 1M fill)
+    static const uint32_t kShapes[4][2] = {{3, 3}, {2, 2}, {2, 1}, {1, 1}};
     uint32_t tw = 1, th = 1, td = 1;
     for (const auto &sh : kShapes) {
         th = sh[0];
