@@ -410,7 +410,8 @@ __device__ __forceinline__ void walk_group(const SceneConst &c, const ChainBuffe
                     const int ec = q[k * kB];
                     const float4 p = s_pos[ec];
                     const float d2 = dist2(p, s);
-                    if (interior || (ec >= mlo && ec < mhi)) density_terms<kF>(c, d2, M, Mn);
+                    // without a branch: +0 outside the row's density cells or out of range
+                    density_terms_masked<kF>(c, d2, interior || (ec >= mlo && ec < mhi), M, Mn);
                     const uint32_t slot = __float_as_uint(p.w);
                     if (slot != j && !(d2 > c.R2)) {
                         if (nl < cap) {
