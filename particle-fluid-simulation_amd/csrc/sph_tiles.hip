@@ -687,9 +687,10 @@ uint32_t scene_tiles(const SceneConst &c) { return c.use_tiles ? c.ntx * c.nty *
 void tile_dims_for(SceneConst &c, double ppc, int cfg) {
     const TileConfig &tc = kTileConfigs[cfg];
     if (!(ppc > 0.0)) ppc = 1.0;
-    // owned particles per tile about 0.85 of the workgroup, staged entries within 3/4 of
-    // the LDS capacity, region cells within kTileRegionMax
-    const double own_target = 0.85 * tc.block;
+    // owned particles per tile about 0.75 of the workgroup (on the 1M fill, a tile owning
+    // more than a lane each walks twice; 0.75 measured best against 0.85, 0.65 and 0.55),
+    // staged entries within 0.8 of the LDS capacity, region cells within kTileRegionMax
+    const double own_target = 0.75 * tc.block;
     // a region cell stages its key's whole run, colliding cells' particles included: about
     // one more occupied cell per T / (occupied cells) keys
     const double occupied = std::min<double>((double)(c.p_end - c.p_begin), (double)c.gx * c.gy * c.gz);
