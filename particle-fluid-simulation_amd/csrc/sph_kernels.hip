@@ -170,7 +170,10 @@ __device__ __forceinline__ GridRuns grid_runs(const SceneConst &c, const ChainBu
 }
 
 // Workgroups are handed to the 8 XCDs round-robin; this renumbering gives each XCD one
-// contiguous eighth of the slots, so that neighbouring cells share that XCD's L2.
+// contiguous eighth of the slots, so that neighbouring cells share that XCD's L2.  The
+// force pass keeps it (its gathers gain from the shared L2: 121 against 125 us on the 1M
+// fill); the density pass runs its slots in launch order instead (203 against 210 us):
+// the contiguous eighths put the dense floor and wall layers on a few XCDs.
 __device__ __forceinline__ uint32_t xcd_block() {
     constexpr uint32_t kXcds = 8;
     const uint32_t b = blockIdx.x, nb = gridDim.x;
@@ -189,7 +192,7 @@ template <class Scenes, int kU, int kB, bool kL, bool kF>
 __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(std::is_same_v<Scenes, OneScene> ? 8 : 7)))
 void k_density(Scenes sc, const ChainBuffers b, uint32_t n) {
     __shared__ uint32_t s_list[kL ? kListLds * kB : 1];
-    const uint32_t j = xcd_block() * kB + threadIdx.x;
+    const uint32_t j = blockIdx.x * kB + threadIdx.x;
     if (j >= n) return;
     const SceneConst c = sc.scene(j);
     const float4 s = b.pred_pos[j];
