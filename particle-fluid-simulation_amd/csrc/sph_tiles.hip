@@ -343,6 +343,50 @@ __device__ __forceinline__ void hand_over(const ChainBuffers &b, uint32_t n, uin
     }
 }
 
+// A tile that cannot be staged, with slot lists: its workgroup walks its owned particles
+// itself, one lane each, through global memory, recording their lists as the per-lane
+// density pass does.  The region's entries are scanned in rounds of kRound; each round's
+// owned slots are collected in `s_g` and walked with their list entries staged in
+// `lane_list` (kListLds per lane).  The particles count among the strays (counters[7]).
+template <int kB, int kRound, bool kF, class Scenes>
+__device__ __forceinline__ void walk_tile_global(const Scenes &sc, const ChainBuffers &b, uint32_t n, uint32_t t,
+                                                 const TileRegion &rg, int total, TileLds &L, uint32_t *s_g,
+                                                 uint32_t *lane_list) {
+    const uint32_t tid = threadIdx.x;
+    for (int base = 0; base < total; base += kRound) {
+        if (tid == 0) L.s_count = 0u;
+        __syncthreads();
+        const int end = min(total, base + kRound);
+        for (int e0 = base; e0 < end; e0 += kB) {
+            const int e = e0 + (int)tid;
+            bool own = false;
+            uint32_t g = 0;
+            if (e < end) {
+                const uint32_t rc = region_cell_in(L.s_off, 0u, rg.cells, e);
+                g = (uint32_t)(L.s_beg[rc] + (e - L.s_off[rc]));
+                const uint2 ow = b.towner[g];
+                own = ow.x == t && ow.y == rc;
+            }
+            const uint64_t m = __ballot(own);
+            if (!m) continue;
+            const uint32_t lane = tid & 63u;
+            const uint32_t leader = (uint32_t)__ffsll((long long)m) - 1u;
+            uint32_t at = 0;
+            if (lane == leader) at = atomicAdd(&L.s_count, (uint32_t)__popcll(m));
+            at = __shfl(at, (int)leader, 64);
+            if (own) s_g[at + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = g;
+        }
+        __syncthreads();
+        const uint32_t cnt = L.s_count;
+        if (tid == 0 && cnt) atomicAdd(&b.counters[7], cnt);
+        for (uint32_t i = tid; i < cnt; i += kB) {
+            const uint32_t j = s_g[i];
+            density_listed<kF>(sc.scene(j), b, n, j, lane_list + tid, kB);
+        }
+        __syncthreads();  // s_g and lane_list are reused by the next round
+    }
+}
+
 // The density walk of one owned particle over the rows of group [r0, r1), staged from
 // region entry e0: S = (predicted position xyz, slot); M, Mn, nl carry its sums and list
 // length from group to group.
@@ -445,7 +489,7 @@ __global__ __launch_bounds__(kB) void k_density_tile(Scenes sc, const ChainBuffe
     __shared__ float4 s_pos[kE];  // (predicted position xyz, cell-ordered slot)
     __shared__ TileLds L;
     __shared__ uint16_t s_own[kOwnMax];
-    __shared__ uint16_t s_q[kTQ * kB];
+    __shared__ alignas(16) uint16_t s_q[kTQ * kB];  // also walk_tile_global's u32 round buffer
     const uint32_t tid = threadIdx.x;
     if (blockIdx.x < lead) {
         // the placement's strays, beside the tiles (their walks' latency overlaps them); with
@@ -473,9 +517,18 @@ __global__ __launch_bounds__(kB) void k_density_tile(Scenes sc, const ChainBuffe
     // 16-bit entries (s_own, the queue, 16-bit lists) cover the region; 16-bit lists index
     // a single staging
     const uint32_t ng = total > 65535 ? 0u : plan_groups<kB>(rg, total, emax, !kS, L);
+    // a tile that cannot be staged: with slot lists its workgroup walks its particles
+    // through global memory (round buffer: the queue; list staging: s_pos); else they go
+    // to the stray kernel after this launch
+    constexpr int kRound = kTQ * kB / 2;
+    static_assert(kE * 4 >= kListLds * kB, "the list staging of walk_tile_global lives in s_pos");
     if (ng == 0u) {
         if (tid == 0) atomicAdd(&b.counters[6], 1u);
-        hand_over<kB>(b, n, t, rg, total, L);
+        if (kS)
+            walk_tile_global<kB, kRound, kF>(sc, b, n, t, rg, total, L, reinterpret_cast<uint32_t *>(s_q),
+                                             reinterpret_cast<uint32_t *>(s_pos));
+        else
+            hand_over<kB>(b, n, t, rg, total, L);
         return;
     }
     if (tid == 0 && ng > 1u) atomicAdd(&b.counters[5], 1u);
@@ -491,7 +544,12 @@ __global__ __launch_bounds__(kB) void k_density_tile(Scenes sc, const ChainBuffe
     const uint32_t owned = L.s_count;
     if (owned > (uint32_t)kOwnMax) {
         if (tid == 0) atomicAdd(&b.counters[6], 1u);
-        hand_over<kB>(b, n, t, rg, total, L);
+        __syncthreads();  // every lane has read the owned count before the rounds reset it
+        if (kS)
+            walk_tile_global<kB, kRound, kF>(sc, b, n, t, rg, total, L, reinterpret_cast<uint32_t *>(s_q),
+                                             reinterpret_cast<uint32_t *>(s_pos));
+        else
+            hand_over<kB>(b, n, t, rg, total, L);
         return;
     }
     // the lane's owned particles tid + q kB: (predicted position, slot) and running sums
@@ -735,8 +793,13 @@ hipError_t launch_density_pair(hipStream_t stream, const Scenes &sc, const Chain
     const uint32_t lead = lead_blocks<kB>(stray_blocks);
     hipLaunchKernelGGL((k_density_tile<Scenes, kB, kE, kF, kS>), dim3(lead + tile_total), dim3(kB), 0, stream, sc, b,
                        n, lead, tile_total);
-    if (hipPeekAtLastError() != hipSuccess) return hipGetLastError();
-    hipLaunchKernelGGL((k_density_stray<Scenes, kF, kS>), dim3(stray_blocks), dim3(kStrayBlock), 0, stream, sc, b, n);
+    // tiles handed over to the stray list: only without slot lists (with them, a tile's
+    // workgroup walks its own particles)
+    if (!kS) {
+        if (hipPeekAtLastError() != hipSuccess) return hipGetLastError();
+        hipLaunchKernelGGL((k_density_stray<Scenes, kF, kS>), dim3(stray_blocks), dim3(kStrayBlock), 0, stream, sc, b,
+                           n);
+    }
     return hipGetLastError();
 }
 
