@@ -197,7 +197,8 @@ int sph_lane_configs(void);
  * 4 = configuration 0 with the approximate square root and reciprocal (needs
  * SPH_PRECISION_TOLERANCE).  The tiled kernels stage each tile's cells and their
  * neighbours in LDS and give bit-identical results to every other variant; the tile shape
- * follows the particles per cell of the last uploaded state.  Needs a box of at most
+ * follows the particles per cell of the last uploaded state.  AUTO measures 0 and 1 (and 4
+ * in the tolerance mode); 2 and 3 run only when selected here.  Needs a box of at most
  * 64n + 65536 cells.  No counterpart in the reference (tuning). */
 int sph_world_set_tile_config(sph_world *w, int config);
 int sph_tile_configs(void);
@@ -313,8 +314,8 @@ int sph_batch_scene_uses_bricks(sph_batch *b, uint32_t scene);  /* 1, 0, or < 0 
 /* TILE at one fixed exact tile configuration (0 to 3, see sph_world_set_tile_config); every
  * scene whose box has at most 64 cells per particle (+ 65536) gets tiles sized for its last
  * uploaded state, the others' particles take the tiled launches' stray kernels.  AUTO
- * measures the tile configurations beside the per-lane and brick kernels.  No counterpart in
- * the reference (tuning). */
+ * measures tile configurations 0 and 1 beside the per-lane and brick kernels.  No
+ * counterpart in the reference (tuning). */
 int sph_batch_set_tile_config(sph_batch *b, int config);
 /* The kernels the next step uses once tuned (DIRECT, BRICK or TILE; AUTO before its
  * measurement), or < 0 on error. */

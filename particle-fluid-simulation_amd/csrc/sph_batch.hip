@@ -258,7 +258,8 @@ static int batch_autotune_once(sph_batch *b, uint32_t substeps) {
     std::vector<int> cands;
     for (int c = 0; c < kNumExactLaneConfigs; ++c) cands.push_back(c);
     if (b->b_total) cands.push_back(-1);
-    for (int c = 0; c < kNumExactTileConfigs; ++c) cands.push_back(kBatchCodeTile + c);
+    for (int c = 0; c < kNumExactTileConfigs; ++c)
+        if (tile_config_tuned(c, false)) cands.push_back(kBatchCodeTile + c);
     b->chain.set_radix_bits(8u);
     const size_t n_kernel_cands = cands.size();
     for (size_t ci = 0;; ++ci) {

@@ -149,6 +149,10 @@ struct TileConfig {
 };
 constexpr int kNumTileConfigs = 5;
 constexpr int kNumExactTileConfigs = 4;
+// The configurations the autotuners measure: the tiled force pass (2, 3) lost wherever it
+// was measured, and its 16-bit lists hand dense tiles to global walks in both passes, so
+// it is only run when pinned.
+bool tile_config_tuned(int cfg, bool tolerance);
 extern const TileConfig kTileConfigs[kNumTileConfigs];
 // Region cells a tile may have ((tw + 2) (th + 2) (td + 2)).
 constexpr uint32_t kTileRegionMax = 768;

@@ -38,6 +38,11 @@ namespace {
 constexpr int kTQ = 24;     // density candidate queue entries per lane
 constexpr int kTFlush = 16; // drain once some lane holds this many
 
+// Lanes of a tiled workgroup that walk particles through global memory (strays, tiles that
+// cannot be staged): their list entries are staged in the tile's LDS copy of positions.
+template <int kB>
+constexpr int kGlobalLanes = kB < 256 ? kB : 256;
+
 __device__ __forceinline__ GridRuns tile_grid_runs(const SceneConst &c, const ChainBuffers &b) {
     return GridRuns{b.grid, c.grid_base, (uint32_t)c.gx + 2u, (uint32_t)c.gy + 2u,
                     c.grid_cells ? (uint32_t)c.gz + 2u : 0u, c.table_base, c.mask, b.crun};
@@ -379,10 +384,12 @@ __device__ __forceinline__ void walk_tile_global(const Scenes &sc, const ChainBu
         __syncthreads();
         const uint32_t cnt = L.s_count;
         if (tid == 0 && cnt) atomicAdd(&b.counters[7], cnt);
-        for (uint32_t i = tid; i < cnt; i += kB) {
-            const uint32_t j = s_g[i];
-            density_listed<kF>(sc.scene(j), b, n, j, lane_list + tid, kB);
-        }
+        constexpr int kGL = kGlobalLanes<kB>;
+        if (tid < (uint32_t)kGL)
+            for (uint32_t i = tid; i < cnt; i += kGL) {
+                const uint32_t j = s_g[i];
+                density_listed<kF>(sc.scene(j), b, n, j, lane_list + tid, kGL);
+            }
         __syncthreads();  // s_g and lane_list are reused by the next round
     }
 }
@@ -494,13 +501,15 @@ __global__ __launch_bounds__(kB) void k_density_tile(Scenes sc, const ChainBuffe
     if (blockIdx.x < lead) {
         // the placement's strays, beside the tiles (their walks' latency overlaps them); with
         // slot lists they record theirs, their list entries staged in s_pos
-        static_assert(!kS || kE * 4 >= kListLds * kB, "the strays' list staging lives in s_pos");
+        constexpr int kGL = kGlobalLanes<kB>;
+        static_assert(!kS || kE * 4 >= kListLds * kGL, "the strays' list staging lives in s_pos");
         const uint32_t count = b.counters[1];
-        for (uint32_t i = blockIdx.x * kB + tid; i < count; i += lead * kB) {
-            const uint32_t j = b.stray[i];
-            if (kS) density_listed<kF>(sc.scene(j), b, n, j, reinterpret_cast<uint32_t *>(s_pos) + tid, kB);
-            else density_global<kF>(sc.scene(j), b, j);
-        }
+        if (tid < (uint32_t)kGL)
+            for (uint32_t i = blockIdx.x * kGL + tid; i < count; i += lead * kGL) {
+                const uint32_t j = b.stray[i];
+                if (kS) density_listed<kF>(sc.scene(j), b, n, j, reinterpret_cast<uint32_t *>(s_pos) + tid, kGL);
+                else density_global<kF>(sc.scene(j), b, j);
+            }
         return;
     }
     // Tiles in launch order, handed to the 8 XCDs round-robin: neighbouring tiles run on
@@ -521,7 +530,7 @@ __global__ __launch_bounds__(kB) void k_density_tile(Scenes sc, const ChainBuffe
     // through global memory (round buffer: the queue; list staging: s_pos); else they go
     // to the stray kernel after this launch
     constexpr int kRound = kTQ * kB / 2;
-    static_assert(kE * 4 >= kListLds * kB, "the list staging of walk_tile_global lives in s_pos");
+    static_assert(kE * 4 >= kListLds * kGlobalLanes<kB>, "the list staging of walk_tile_global lives in s_pos");
     if (ng == 0u) {
         if (tid == 0) atomicAdd(&b.counters[6], 1u);
         if (kS)
@@ -737,6 +746,11 @@ const TileConfig kTileConfigs[kNumTileConfigs] = {
 
 uint32_t scene_tiles(const SceneConst &c) { return c.use_tiles ? c.ntx * c.nty * c.ntz : 0u; }
 
+bool tile_config_tuned(int cfg, bool tolerance) {
+    return cfg >= 0 && cfg < kNumTileConfigs && kTileConfigs[cfg].force_lane >= 0 &&
+           (tolerance || !kTileConfigs[cfg].tolerance);
+}
+
 void tile_dims_for(SceneConst &c, double ppc, int cfg) {
     const TileConfig &tc = kTileConfigs[cfg];
     if (!(ppc > 0.0)) ppc = 1.0;
@@ -781,10 +795,11 @@ void tile_dims_for(SceneConst &c, double ppc, int cfg) {
 
 namespace {
 
-// Leading stray workgroups of a tiled launch (stray_blocks counts kStrayBlock lanes each).
+// Leading stray workgroups of a tiled launch (stray_blocks counts kStrayBlock lanes each;
+// a leading workgroup walks with kGlobalLanes of its lanes).
 template <int kB>
 uint32_t lead_blocks(uint32_t stray_blocks) {
-    return stray_blocks * (uint32_t)(kStrayBlock / kB);
+    return stray_blocks * (uint32_t)(kStrayBlock / kGlobalLanes<kB>);
 }
 
 template <class Scenes, int kB, int kE, bool kF, bool kS>

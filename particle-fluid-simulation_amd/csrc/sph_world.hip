@@ -646,7 +646,8 @@ int world_autotune(sph_world *w, uint32_t substeps, int *chosen, double *ms_out)
     for (int c = 0; c < lane_configs_for(w); ++c) cands.push_back(c);
     if (w->chain.brick_total) cands.push_back(kCodeBrick);
     if (world_tiles_possible(w))
-        for (int c = 0; c < tile_configs_for(w); ++c) cands.push_back(kCodeTile + c);
+        for (int c = 0; c < tile_configs_for(w); ++c)
+            if (tile_config_tuned(c, w->tolerance)) cands.push_back(kCodeTile + c);
     int best = kDefaultLaneConfig;
     double best_ms = 1e30;
     hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -727,7 +728,8 @@ int online_start(sph_world *w) {
     for (int c = 0; c < lane_configs_for(w); ++c) codes.push_back(c);
     if (w->chain.brick_total) codes.push_back(kCodeBrick);
     if (world_tiles_possible(w))
-        for (int c = 0; c < tile_configs_for(w); ++c) codes.push_back(kCodeTile + c);
+        for (int c = 0; c < tile_configs_for(w); ++c)
+            if (tile_config_tuned(c, w->tolerance)) codes.push_back(kCodeTile + c);
     w->online = sph_world::OnlineTune{};
     w->online.active = true;
     return online_begin_phase(w, std::move(codes));
