@@ -9,6 +9,7 @@
 #include "sph_kernels.hpp"
 #include "sph_pairs.hpp"
 
+
 namespace sph {
 
 namespace {
@@ -169,11 +170,25 @@ __device__ __forceinline__ GridRuns grid_runs(const SceneConst &c, const ChainBu
                     c.grid_cells ? (uint32_t)c.gz + 2u : 0u, c.table_base, c.mask, b.crun};
 }
 
-// Workgroups are handed to the 8 XCDs round-robin; this renumbering gives each XCD one
-// contiguous eighth of the slots, so that neighbouring cells share that XCD's L2.  The
-// force pass keeps it (its gathers gain from the shared L2: 121 against 125 us on the 1M
-// fill); the density pass runs its slots in launch order instead (203 against 210 us):
-// the contiguous eighths put the dense floor and wall layers on a few XCDs.
+// Workgroups are handed to the 8 XCDs round-robin.  xcd_chunked deals them in chunks of
+// kChunk consecutive blocks instead: XCD x runs chunks x, x + 8, x + 16, ... (each chunk a
+// contiguous slot range), the blocks of a last partial round of chunks in launch order.
+// xcd_block gives each XCD one contiguous eighth of the slots, so that neighbouring cells
+// share that XCD's L2.  The force pass uses xcd_block (on the 1M fill 122.6 us and 124 MiB
+// of HBM reads per launch, against 120.3 us and 182 MiB in chunks of 16).  The density pass
+// uses chunks of kDensityChunk: contiguous eighths put the dense floor and wall layers on a
+// few XCDs (210 us), launch order lost the L2 reuse (203 us, 176 MiB read), chunks of 16
+// keep both (202 us, 107 MiB).
+constexpr uint32_t kDensityChunk = 16;
+template <uint32_t kChunk>
+__device__ __forceinline__ uint32_t xcd_chunked() {
+    const uint32_t b = blockIdx.x, nb = gridDim.x;
+    constexpr uint32_t kSpan = 8u * kChunk;
+    if (b >= nb / kSpan * kSpan) return b;
+    const uint32_t i = b / 8u;
+    return ((i / kChunk) * 8u + b % 8u) * kChunk + i % kChunk;
+}
+
 __device__ __forceinline__ uint32_t xcd_block() {
     constexpr uint32_t kXcds = 8;
     const uint32_t b = blockIdx.x, nb = gridDim.x;
@@ -192,7 +207,7 @@ template <class Scenes, int kU, int kB, bool kL, bool kF>
 __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(std::is_same_v<Scenes, OneScene> ? 8 : 7)))
 void k_density(Scenes sc, const ChainBuffers b, uint32_t n) {
     __shared__ uint32_t s_list[kL ? kListLds * kB : 1];
-    const uint32_t j = blockIdx.x * kB + threadIdx.x;
+    const uint32_t j = xcd_chunked<kDensityChunk>() * kB + threadIdx.x;
     if (j >= n) return;
     const SceneConst c = sc.scene(j);
     const float4 s = b.pred_pos[j];

@@ -43,6 +43,19 @@ constexpr int kTFlush = 16; // drain once some lane holds this many
 template <int kB>
 constexpr int kGlobalLanes = kB < 256 ? kB : 256;
 
+// Tile of the r-th tile workgroup (r counted from a multiple of 8): tiles dealt to the 8
+// XCDs in chunks of kTileChunk consecutive tiles, XCD x taking chunks x, x + 8, ...  On the
+// 1M fill, one contiguous range of tiles per XCD put the dense floor and wall tiles on a
+// few XCDs (241 us); launch order balanced them (193 us) but read 87 MiB per launch from
+// HBM; chunks of 4 keep the time with 70 MiB (16: 195 us, 66 MiB; 64: 218 us, 47 MiB).
+constexpr uint32_t kTileChunk = 4;
+__device__ __forceinline__ uint32_t tile_of_block(uint32_t r, uint32_t nt) {
+    constexpr uint32_t kChunk = kTileChunk, kSpan = 8u * kChunk;
+    if (r >= nt / kSpan * kSpan) return r;
+    const uint32_t i = r / 8u;
+    return ((i / kChunk) * 8u + r % 8u) * kChunk + i % kChunk;
+}
+
 __device__ __forceinline__ GridRuns tile_grid_runs(const SceneConst &c, const ChainBuffers &b) {
     return GridRuns{b.grid, c.grid_base, (uint32_t)c.gx + 2u, (uint32_t)c.gy + 2u,
                     c.grid_cells ? (uint32_t)c.gz + 2u : 0u, c.table_base, c.mask, b.crun};
@@ -512,11 +525,7 @@ __global__ __launch_bounds__(kB) void k_density_tile(Scenes sc, const ChainBuffe
             }
         return;
     }
-    // Tiles in launch order, handed to the 8 XCDs round-robin: neighbouring tiles run on
-    // different XCDs.  Giving each XCD one contiguous range of tiles instead (for L2 reuse of
-    // the runs neighbouring tiles share) put the dense floor and wall tiles on a few XCDs:
-    // on the 1M fill the density pass took 241 us instead of 194.
-    const uint32_t t = blockIdx.x - lead;
+    const uint32_t t = tile_of_block(blockIdx.x - lead, tile_total);
     const SceneConst c = sc.scene_of_tile(t);
     if (!c.use_tiles) return;
     const TileRegion rg = tile_region(c, t - c.tile_base);
@@ -638,7 +647,7 @@ __global__ __launch_bounds__(kB) void k_force_tile(Scenes sc, const ChainBuffers
         }
         return;
     }
-    const uint32_t t = blockIdx.x - lead;
+    const uint32_t t = tile_of_block(blockIdx.x - lead, tile_total);
     const SceneConst c = sc.scene_of_tile(t);
     if (!c.use_tiles) return;
     const TileRegion rg = tile_region(c, t - c.tile_base);
@@ -796,10 +805,11 @@ void tile_dims_for(SceneConst &c, double ppc, int cfg) {
 namespace {
 
 // Leading stray workgroups of a tiled launch (stray_blocks counts kStrayBlock lanes each;
-// a leading workgroup walks with kGlobalLanes of its lanes).
+// a leading workgroup walks with kGlobalLanes of its lanes), a multiple of 8 so that the
+// tiles' chunks keep their XCDs.
 template <int kB>
 uint32_t lead_blocks(uint32_t stray_blocks) {
-    return stray_blocks * (uint32_t)(kStrayBlock / kGlobalLanes<kB>);
+    return (stray_blocks * (uint32_t)(kStrayBlock / kGlobalLanes<kB>) + 7u) / 8u * 8u;
 }
 
 template <class Scenes, int kB, int kE, bool kF, bool kS>
