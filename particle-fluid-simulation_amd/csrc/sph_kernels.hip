@@ -15,13 +15,15 @@ namespace sph {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr uint32_t kHashBlock = 1024;      // lanes of a hashing / clearing workgroup
+constexpr uint32_t kHashKeys = 4 * 1024;  // keys per hashing workgroup
 
 // ---------------------------------------------------------------------------------------
 // Hash: updateCellEntries.comp:69-80 (key of the CURRENT position, value = slot index,
 // implicit here), plus the digit histograms of the radix passes and the clears that
 // clearStartIndices.comp:47-52 performs, moved to a point where the table is dead.
 template <class Scenes, int kW>
-__global__ __launch_bounds__(kBlock) void k_hash(
+__global__ __launch_bounds__(kHashBlock) void k_hash(
     Scenes sc, const float4 *__restrict__ pos, uint32_t *__restrict__ keys, uint32_t n,
     uint32_t passes, uint32_t *__restrict__ ghist, uint32_t hash_blocks,
     int2 *__restrict__ table, int2 *__restrict__ crun, uint32_t table_total,
@@ -34,23 +36,43 @@ __global__ __launch_bounds__(kBlock) void k_hash(
     __shared__ uint32_t s_hist[kMaxP][kBins];
     const uint32_t tid = threadIdx.x;
     if (blockIdx.x < hash_blocks) {
-        for (uint32_t i = tid; i < passes * kBins; i += kBlock) (&s_hist[0][0])[i] = 0u;
+        for (uint32_t i = tid; i < passes * kBins; i += kHashBlock) (&s_hist[0][0])[i] = 0u;
         __syncthreads();
-        const uint32_t per_block = ((n + hash_blocks - 1u) / hash_blocks + kBlock - 1u) / kBlock * kBlock;
+        const uint32_t per_block = ((n + hash_blocks - 1u) / hash_blocks + kHashBlock - 1u) / kHashBlock * kHashBlock;
         const uint32_t begin = blockIdx.x * per_block;
         const uint32_t end = min(n, begin + per_block);
-        for (uint32_t i = begin + tid; i < end; i += kBlock) {
-            const SceneConst c = sc.scene(i);
-            const float4 p = pos[i];
-            const uint32_t cx = (uint32_t)cell_of(p.x, c.minx, c.R);
-            const uint32_t cy = (uint32_t)cell_of(p.y, c.miny, c.R);
-            const uint32_t cz = (uint32_t)cell_of(p.z, c.minz, c.R);
-            const uint32_t key = c.table_base + (cell_hash(cx, cy, cz) & c.mask);
-            keys[i] = key;
-            for (uint32_t q = 0; q < passes; ++q) atomicAdd(&s_hist[q][(key >> (kW * q)) & (kBins - 1u)], 1u);
+        // The state arrives in the previous substep's key order, so a wave's digits come in
+        // runs of one value: each run adds its length with one LDS atomic from its first
+        // lane, where one atomic per key would queue the workgroup's 16 waves on one address.
+        const uint32_t lane = tid & 63u;
+        for (uint32_t i0 = begin; i0 < end; i0 += kHashBlock) {
+            const uint32_t i = i0 + tid;
+            const bool valid = i < end;
+            uint32_t key = 0u;
+            if (valid) {
+                const SceneConst c = sc.scene(i);
+                const float4 p = pos[i];
+                const uint32_t cx = (uint32_t)cell_of(p.x, c.minx, c.R);
+                const uint32_t cy = (uint32_t)cell_of(p.y, c.miny, c.R);
+                const uint32_t cz = (uint32_t)cell_of(p.z, c.minz, c.R);
+                key = c.table_base + (cell_hash(cx, cy, cz) & c.mask);
+                keys[i] = key;
+            }
+            for (uint32_t q = 0; q < passes; ++q) {
+                // lanes past the end carry a value no digit has
+                const uint32_t d = valid ? (key >> (kW * q)) & (kBins - 1u) : 0xFFFFFFFFu;
+                const uint32_t up = __shfl_up(d, 1u, 64);
+                const bool head = lane == 0u || up != d;
+                const uint64_t heads = __ballot(head);
+                if (head && valid) {
+                    const uint64_t later = lane == 63u ? 0ull : heads >> (lane + 1u);
+                    const uint32_t len = later ? (uint32_t)__builtin_ctzll(later) + 1u : 64u - lane;
+                    atomicAdd(&s_hist[q][d], len);
+                }
+            }
         }
         __syncthreads();
-        for (uint32_t i = tid; i < passes * kBins; i += kBlock) {
+        for (uint32_t i = tid; i < passes * kBins; i += kHashBlock) {
             const uint32_t v = (&s_hist[0][0])[i];
             if (v) atomicAdd(&ghist[i], v);
         }
@@ -61,13 +83,13 @@ __global__ __launch_bounds__(kBlock) void k_hash(
         // [0] queue, [1] strays, [5] streamed tiles, [6] handed-over tiles, [7] their
         // particles (not [4]: the host clears the list record when it reads it)
         if (b == 0 && counters && (tid < 2 || (tid >= 5 && tid < 8))) counters[tid] = 0u;
-        for (uint32_t i = b * kBlock + tid; i < table_total; i += nb * kBlock) {
+        for (uint32_t i = b * kHashBlock + tid; i < table_total; i += nb * kHashBlock) {
             table[i] = make_int2(-1, -1);
             if (crun) crun[i] = make_int2(-1, -1);
         }
-        for (uint32_t i = b * kBlock + tid; i < lookback_words; i += nb * kBlock) lookback[i] = 0u;
-        for (uint32_t i = b * kBlock + tid; i < bucket_total; i += nb * kBlock) bcount[i] = 0u;
-        for (uint32_t i = b * kBlock + tid; i < blookback_words; i += nb * kBlock) blookback[i] = 0u;
+        for (uint32_t i = b * kHashBlock + tid; i < lookback_words; i += nb * kHashBlock) lookback[i] = 0u;
+        for (uint32_t i = b * kHashBlock + tid; i < bucket_total; i += nb * kHashBlock) bcount[i] = 0u;
+        for (uint32_t i = b * kHashBlock + tid; i < blookback_words; i += nb * kHashBlock) blookback[i] = 0u;
     }
 }
 
@@ -294,17 +316,20 @@ template <class Scenes>
 hipError_t launch_hash(hipStream_t stream, const Scenes &sc, const ChainBuffers &b, uint32_t n,
                        uint32_t table_total, uint32_t passes, uint32_t lookback_words,
                        uint32_t bucket_total, uint32_t digit_bits) {
-    // 2K keys per hashing workgroup: enough workgroups to fill the chip, few enough global
-    // histogram atomics; the clearing workgroups run beside them.
-    const uint32_t hash_blocks = (n + 2047u) / 2048u;
+    // 4 keys per lane of 1024-lane hashing workgroups: enough waves to fill the chip, few
+    // global histogram atomics (every workgroup adds to the same bins, and those additions
+    // are serialised; each lane's keys come one after another): at 1M keys, 512 workgroups
+    // of 256 lanes over 2K keys each took 20 us, 256 of 1024 lanes over 4K 14 us, 128 over
+    // 8K 20 us.  The clearing workgroups run beside them.
+    const uint32_t hash_blocks = (n + kHashKeys - 1u) / kHashKeys;
     const uint32_t btiles = (bucket_total + kBucketTile - 1u) / kBucketTile;
     uint32_t clear_work = table_total > lookback_words ? table_total : lookback_words;
     if (bucket_total > clear_work) clear_work = bucket_total;
-    uint32_t clear_blocks = (clear_work + 4u * kBlock - 1u) / (4u * kBlock);
+    uint32_t clear_blocks = (clear_work + 4u * kHashBlock - 1u) / (4u * kHashBlock);
     if (clear_blocks < 1u) clear_blocks = 1u;
-    if (clear_blocks > 2048u) clear_blocks = 2048u;
+    if (clear_blocks > 512u) clear_blocks = 512u;
     auto kernel = digit_bits == 12 ? k_hash<Scenes, 12> : digit_bits == 11 ? k_hash<Scenes, 11> : k_hash<Scenes, 8>;
-    hipLaunchKernelGGL(kernel, dim3(hash_blocks + clear_blocks), dim3(kBlock), 0, stream,
+    hipLaunchKernelGGL(kernel, dim3(hash_blocks + clear_blocks), dim3(kHashBlock), 0, stream,
                        sc, b.pos, b.keys, n, passes, b.ghist, hash_blocks, b.table,
                        bucket_total ? b.crun : nullptr, table_total,
                        b.lookback, lookback_words, b.tile_ctr, b.counters, b.bcount, bucket_total,
