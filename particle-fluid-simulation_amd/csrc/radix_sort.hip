@@ -61,7 +61,7 @@ template <int kBlk, int kW>
 __global__ __launch_bounds__(kBlk) void k_radix_pass(
     const uint32_t *__restrict__ kin, const uint32_t *__restrict__ vin,
     uint32_t *__restrict__ kout, uint32_t *__restrict__ vout, uint32_t n, uint32_t shift,
-    uint32_t bits, const uint32_t *__restrict__ ghist, uint32_t *lookback,
+    uint32_t bits, const uint32_t *__restrict__ ghist, uint32_t ghist_stride, uint32_t *lookback,
     uint32_t *tile_counter) {
     constexpr int kWaves = kBlk / 64;
     constexpr int kItems = (int)kTileKeys / kBlk;
@@ -124,7 +124,7 @@ __global__ __launch_bounds__(kBlk) void k_radix_pass(
 
     // Per owned digit: the tile's count, with each wave's count turned into the exclusive
     // count of the waves before it.
-    uint32_t tcount[kDPT];
+    uint32_t tcount[kDPT], gcount[kDPT];
     uint32_t tsum = 0, gsum = 0;
     if (owner) {
 #pragma unroll
@@ -139,7 +139,11 @@ __global__ __launch_bounds__(kBlk) void k_radix_pass(
             }
             tcount[q] = sum;
             tsum += sum;
-            gsum += ghist[d];
+            uint32_t gc = 0;
+#pragma unroll
+            for (uint32_t r = 0; r < kGhistCopies; ++r) gc += ghist[r * ghist_stride + (uint32_t)d];
+            gcount[q] = gc;
+            gsum += gc;
         }
     }
     const uint32_t gbase = block_exclusive_scan<kBlk>(gsum, s_tot);
@@ -158,7 +162,7 @@ __global__ __launch_bounds__(kBlk) void k_radix_pass(
             loff[q] = l;
             l += tcount[q];
             const uint32_t gd = g;
-            g += ghist[d];
+            g += gcount[q];
             uint32_t *mine = lookback + (size_t)tile * kBins + (uint32_t)d;
             if (tile == 0) {
                 excl[q] = gd;
@@ -257,21 +261,21 @@ uint32_t radix_tile_size() { return kTileKeys; }
 
 hipError_t radix_pass(hipStream_t stream, const uint32_t *kin, const uint32_t *vin,
                       uint32_t *kout, uint32_t *vout, uint32_t n, uint32_t shift,
-                      uint32_t bits, const uint32_t *ghist, uint32_t *lookback,
+                      uint32_t bits, const uint32_t *ghist, uint32_t ghist_stride, uint32_t *lookback,
                       uint32_t *tile_counter, uint32_t digit_bits) {
     const uint32_t tiles = (n + kTileKeys - 1u) / kTileKeys;
     switch (digit_bits) {
         case 11:
             hipLaunchKernelGGL((k_radix_pass<kBlockWide, 11>), dim3(tiles), dim3(kBlockWide), 0, stream, kin,
-                               vin, kout, vout, n, shift, bits, ghist, lookback, tile_counter);
+                               vin, kout, vout, n, shift, bits, ghist, ghist_stride, lookback, tile_counter);
             break;
         case 12:
             hipLaunchKernelGGL((k_radix_pass<kBlockWide, 12>), dim3(tiles), dim3(kBlockWide), 0, stream, kin,
-                               vin, kout, vout, n, shift, bits, ghist, lookback, tile_counter);
+                               vin, kout, vout, n, shift, bits, ghist, ghist_stride, lookback, tile_counter);
             break;
         default:
             hipLaunchKernelGGL((k_radix_pass<kBlock8, 8>), dim3(tiles), dim3(kBlock8), 0, stream, kin,
-                               vin, kout, vout, n, shift, bits, ghist, lookback, tile_counter);
+                               vin, kout, vout, n, shift, bits, ghist, ghist_stride, lookback, tile_counter);
     }
     return hipGetLastError();
 }

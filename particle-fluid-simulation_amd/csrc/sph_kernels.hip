@@ -74,7 +74,7 @@ __global__ __launch_bounds__(kHashBlock) void k_hash(
         __syncthreads();
         for (uint32_t i = tid; i < passes * kBins; i += kHashBlock) {
             const uint32_t v = (&s_hist[0][0])[i];
-            if (v) atomicAdd(&ghist[i], v);
+            if (v) atomicAdd(&ghist[(blockIdx.x % kGhistCopies) * passes * kBins + i], v);
         }
     } else {
         const uint32_t b = blockIdx.x - hash_blocks;
@@ -108,7 +108,7 @@ __global__ __launch_bounds__(kBlock) void k_reorder(
     uint2 *__restrict__ run_info, uint32_t ghist_words) {
     const uint32_t tid = threadIdx.x;
     if (blockIdx.x == 0)
-        for (uint32_t i = tid; i < ghist_words; i += kBlock) ghist[i] = 0u;
+        for (uint32_t i = tid; i < kGhistCopies * ghist_words; i += kBlock) ghist[i] = 0u;
     const uint32_t j = blockIdx.x * kBlock + tid;
     bool to_stray = false;
     if (j < n) {

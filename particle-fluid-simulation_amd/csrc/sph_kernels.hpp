@@ -33,7 +33,7 @@ struct ChainBuffers {
                          // density pass has run)
     float2 *dens;        // (density, near density), sorted order
     int2 *table;         // (start, end) per cell, -1 = empty
-    uint32_t *ghist;     // [passes][bins] digit histograms (kGhistWords)
+    uint32_t *ghist;     // [kGhistCopies][passes][bins] digit histograms (kGhistWords each)
     uint32_t *lookback;  // [passes][tiles][bins] look-back words
     uint32_t *tile_ctr;  // [4] tile counters
     float4 *dbg_pred_vel;   // optional (debug capture): predicted velocities, sorted
@@ -74,13 +74,17 @@ uint32_t radix_tile_size();
 // tile_ctr[0..3]: radix passes; [4]: bucket scan.
 constexpr uint32_t kTileCounters = 8;
 // Radix digit widths: 8, 11 or 12 bits (256, 2048 or 4096 bins), at most 4 passes; the
-// digit histograms of all passes, [pass][bins], fit kGhistWords.
+// digit histograms of all passes, [pass][bins], fit kGhistWords.  The hash pass adds into
+// kGhistCopies copies of them (one per XCD, [copy][pass][bins] with a stride of the passes'
+// words), which the radix passes sum: each copy takes an eighth of the hashing workgroups'
+// additions to one bin, which device-wide atomics would otherwise serialise.
 constexpr uint32_t kGhistWords = 3u * 4096u;
+constexpr uint32_t kGhistCopies = 8u;
 
 // One stable LSD pass over `bits` (<= digit_bits) bits at `shift`, digit_bits = 8, 11, 12.
 hipError_t radix_pass(hipStream_t stream, const uint32_t *kin, const uint32_t *vin,
                       uint32_t *kout, uint32_t *vout, uint32_t n, uint32_t shift,
-                      uint32_t bits, const uint32_t *ghist, uint32_t *lookback,
+                      uint32_t bits, const uint32_t *ghist, uint32_t ghist_stride, uint32_t *lookback,
                       uint32_t *tile_counter, uint32_t digit_bits);
 
 // Hash pass: keys of the current positions, digit histograms for `passes` passes, and

@@ -27,7 +27,7 @@ constexpr uint64_t align_up(uint64_t x) { return (x + kAlign - 1) / kAlign * kAl
 constexpr uint64_t scratch_need(uint64_t n, uint64_t T, uint64_t passes, uint64_t buckets, uint64_t grid) {
     const uint64_t tiles = (n + 4095) / 4096;
     return align_up(4 * n) * 5 + align_up(16 * n) * 3 + align_up(8 * n) + align_up(4 * n) +
-           align_up(8 * T) * 3 + align_up(4 * 256 * 4) + align_up(4 * passes * tiles * 256) +
+           align_up(8 * T) * 3 + align_up(4 * kGhistCopies * 256 * 4) + align_up(4 * passes * tiles * 256) +
            align_up(4 * kTileCounters) + align_up(4 * 8) + align_up(4 * buckets) +
            align_up(4 * ((buckets + kBucketTile - 1) / kBucketTile)) + align_up(8 * grid);
 }
@@ -143,7 +143,7 @@ extern "C" int sph_step_noalloc(const sph_params *params, uint32_t n, float *pos
     b.table = cv.take<int2>(8ull * T);
     b.run_info = cv.take<uint2>(8ull * T);
     b.crun = cv.take<int2>(8ull * T);
-    b.ghist = cv.take<uint32_t>(4 * 256 * 4);
+    b.ghist = cv.take<uint32_t>(4 * kGhistCopies * 256 * 4);
     const uint64_t tiles = (N + 4095) / 4096;
     b.lookback = cv.take<uint32_t>(4 * (uint64_t)chain.passes * tiles * 256);
     b.tile_ctr = cv.take<uint32_t>(4 * kTileCounters);
@@ -160,7 +160,7 @@ extern "C" int sph_step_noalloc(const sph_params *params, uint32_t n, float *pos
     // The array keeps the previous call's contents: the digit histograms must start at zero
     // (each substep's reorder clears them for the next); a kernel, not a memset, so that a
     // caller may capture this call into a graph.  Everything else is cleared by the chain.
-    hipLaunchKernelGGL(k_zero_u32, dim3(4), dim3(256), 0, s, b.ghist, 4u * 256u);
+    hipLaunchKernelGGL(k_zero_u32, dim3(kGhistCopies * 4u), dim3(256), 0, s, b.ghist, kGhistCopies * 4u * 256u);
     hipError_t e = hipGetLastError();
     for (uint64_t k = 0; k < substeps && e == hipSuccess; ++k) e = chain.enqueue(s, scene, nullptr);
     return e == hipSuccess ? SPH_OK : hip_fail(e, "allocation-free substep launch");

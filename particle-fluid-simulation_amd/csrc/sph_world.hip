@@ -211,7 +211,7 @@ int Chain::allocate(uint32_t n_, uint32_t table_total_, uint32_t brick_total_,
         {(void **)&b.sort_k[1], N * 4}, {(void **)&b.sort_v[1], N * 4},
         {(void **)&b.pred_pos, N * 16}, {(void **)&b.pos_sorted, N * 16},
         {(void **)&b.vel_sorted, N * 16}, {(void **)&b.dens, N * 8},
-        {(void **)&b.table, (size_t)table_total * 8}, {(void **)&b.ghist, kGhistWords * 4},
+        {(void **)&b.table, (size_t)table_total * 8}, {(void **)&b.ghist, kGhistCopies * kGhistWords * 4},
         {(void **)&b.lookback, (size_t)(lookback_cap ? lookback_cap : 1) * 4},
         {(void **)&b.tile_ctr, kTileCounters * 4}, {(void **)&b.hfull, N * 4}, {(void **)&b.stray, N * 4},
         {(void **)&b.bcount, (size_t)bucket_total * 4},
@@ -242,7 +242,7 @@ int Chain::allocate(uint32_t n_, uint32_t table_total_, uint32_t brick_total_,
         return fail(SPH_ERR_OUT_OF_MEMORY, "pinned host words");
     }
     h_list[0] = h_list[1] = 0u;
-    hipError_t e = hipMemset(b.ghist, 0, kGhistWords * 4);
+    hipError_t e = hipMemset(b.ghist, 0, kGhistCopies * kGhistWords * 4);
     if (e == hipSuccess) e = hipMemset(b.counters, 0, 8 * 4);
     if (e == hipSuccess) e = hipMemset(b.brick_flag, 0, (size_t)(brick_total ? brick_total : 1) * 4);
     if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -423,7 +423,8 @@ hipError_t Chain::enqueue(hipStream_t s, const Scenes &sc, PassEvents *ev) {
         const uint32_t shift = radix_bits * q;
         const uint32_t bits = key_bits - shift < radix_bits ? key_bits - shift : radix_bits;
         e = radix_pass(s, kin, vin, b.sort_k[q & 1u], b.sort_v[q & 1u], n, shift, bits,
-                       b.ghist + bins * q, b.lookback + (size_t)q * tiles * bins, b.tile_ctr + q, radix_bits);
+                       b.ghist + bins * q, ghist_words(), b.lookback + (size_t)q * tiles * bins, b.tile_ctr + q,
+                       radix_bits);
         if (e != hipSuccess) return e;
     }
     if (ev) hipEventRecord(ev->ev[2], s);
@@ -858,7 +859,7 @@ uint64_t sph_world_footprint(uint32_t n, uint32_t table_cells) {
     return (uint64_t)n * (16 * 2 + 4 + 16 + 16 * 3 + 8 + 8 + 4 + 4 * kList + 1) + T * 24 +
            (2 * (uint64_t)n + 1024) * 4 + (16 * (uint64_t)n + 4096) * 8 +
            std::max<uint64_t>(4 * tiles * 256 * 4, std::max<uint64_t>(16 * (uint64_t)n, 8ull << 20)) +
-           kGhistWords * 4 + 4096;
+           kGhistCopies * kGhistWords * 4 + 4096;
 }
 
 int sph_world_create(const sph_params *params, uint32_t n, sph_world **out) {
