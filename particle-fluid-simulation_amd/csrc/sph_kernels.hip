@@ -194,14 +194,14 @@ __device__ __forceinline__ GridRuns grid_runs(const SceneConst &c, const ChainBu
 
 // Workgroups are handed to the 8 XCDs round-robin.  xcd_chunked deals them in chunks of
 // kChunk consecutive blocks instead: XCD x runs chunks x, x + 8, x + 16, ... (each chunk a
-// contiguous slot range), the blocks of a last partial round of chunks in launch order.
-// xcd_block gives each XCD one contiguous eighth of the slots, so that neighbouring cells
-// share that XCD's L2.  The force pass uses xcd_block (on the 1M fill 122.6 us and 124 MiB
-// of HBM reads per launch, against 120.3 us and 182 MiB in chunks of 16).  The density pass
-// uses chunks of kDensityChunk: contiguous eighths put the dense floor and wall layers on a
-// few XCDs (210 us), launch order lost the L2 reuse (203 us, 176 MiB read), chunks of 16
-// keep both (202 us, 107 MiB).
+// contiguous slot range, whose neighbouring cells share that XCD's L2), the blocks of a
+// last partial round of chunks in launch order.  One contiguous eighth of the slots per
+// XCD puts the dense floor and wall layers on a few XCDs; launch order loses the L2 reuse.
+// Density pass (chunks of kDensityChunk): eighths 210 us, launch order 203 us and 176 MiB
+// read, chunks of 16 202 us and 107 MiB.  Force pass (chunks of kForceChunk), 1M fill
+// after 25 steps: eighths 117.2 us, chunks of 32 114.1 us, of 128 113.8 us.
 constexpr uint32_t kDensityChunk = 16;
+constexpr uint32_t kForceChunk = 128;
 template <uint32_t kChunk>
 __device__ __forceinline__ uint32_t xcd_chunked() {
     const uint32_t b = blockIdx.x, nb = gridDim.x;
@@ -209,13 +209,6 @@ __device__ __forceinline__ uint32_t xcd_chunked() {
     if (b >= nb / kSpan * kSpan) return b;
     const uint32_t i = b / 8u;
     return ((i / kChunk) * 8u + b % 8u) * kChunk + i % kChunk;
-}
-
-__device__ __forceinline__ uint32_t xcd_block() {
-    constexpr uint32_t kXcds = 8;
-    const uint32_t b = blockIdx.x, nb = gridDim.x;
-    const uint32_t q = nb / kXcds, r = nb % kXcds, x = b % kXcds;
-    return x * q + min(x, r) + b / kXcds;
 }
 
 // Occupancy floors.  The single-scene density pass at 8 waves per SIMD (64 VGPRs, 2
@@ -278,7 +271,7 @@ template <class Scenes, int kU, int kB, bool kL, bool kF>
 __global__ __launch_bounds__(kB) __attribute__((amdgpu_waves_per_eu(4))) void k_force(Scenes sc, const ChainBuffers b,
                                                                                         uint32_t n) {
     __shared__ uint32_t s_queue[kQueue * kB];
-    const uint32_t j = xcd_block() * kB + threadIdx.x;
+    const uint32_t j = xcd_chunked<kForceChunk>() * kB + threadIdx.x;
     if (j >= n) return;
     const SceneConst c = sc.scene(j);
     const float4 s = b.pred_pos[j];
