@@ -184,8 +184,8 @@ TRAFFIC_PROFILE = "profiles/pmc_uniform_1m_traffic.md"
 TRAFFIC_KERNELS = ("per-lane, 4 candidates per scan step, 128-lane workgroups, force neighbour lists, "
                    "8-bit radix digits (3 passes)")
 TRAFFIC_BYTES_PER_LAUNCH = {
-    (5, 20): {"density": (2 * 109308 + 119027) * 1024, "force": (2 * 128089 + 42449) * 1024},
-    (5, 100): {"density": (2 * 116641 + 196183) * 1024, "force": (2 * 146168 + 42466) * 1024},
+    (5, 20): {"density": (2 * 109309 + 119029) * 1024, "force": (2 * 173590 + 42403) * 1024},
+    (5, 100): {"density": (2 * 116664 + 196226) * 1024, "force": (2 * 182953 + 42441) * 1024},
 }
 
 
