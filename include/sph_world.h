@@ -217,7 +217,7 @@ int sph_world_tile_stats(sph_world *w, uint32_t out[4]);
  * configurations 10 and 11, which evaluate the pair terms with the hardware's approximate
  * square root and reciprocal; cell keys, the sorted pairs and the table stay exact, and one
  * substep's densities, forces, positions and velocities agree with the reference's
- * equations within 1e-5 relative (north_star's stated tolerance).  Selecting a precision
+ * equations within 1e-5 relative (the single-step accuracy the drop-in must keep).  Selecting a precision
  * re-tunes an AUTO world; configurations 10 and 11 need TOLERANCE. */
 typedef enum sph_precision { SPH_PRECISION_EXACT = 0, SPH_PRECISION_TOLERANCE = 1 } sph_precision;
 int sph_world_set_precision(sph_world *w, int mode);
@@ -234,8 +234,10 @@ int sph_world_set_radix_bits(sph_world *w, int bits);
  * up to 254 at once), and the lists grow to hold the longest that did not fit (twice the
  * capacity, or that list plus a quarter, at most 254): sph_world_step looks between frames
  * without waiting (a frame or two after the overflow it synchronises once and grows), and
- * sph_world_sync / download_state and the tuning look as well.  A particle whose list does not fit walks its
- * cells in the force pass instead: results are the same bits at every capacity, only
+ * sph_world_sync / download_state and the tuning look as well.  Eight looks in a row with
+ * no list above half the capacity halve it (down to 64), at sph_world_sync /
+ * download_state only, where the stream is idle.  A particle whose list does not fit walks
+ * its cells in the force pass instead: results are the same bits at every capacity, only
  * speed and memory (4 B per entry per particle) change.
  *   sph_world_set_list_capacity: entries 1..254 fixes the capacity (reallocated now);
  *     0 returns to the adaptive sizing above.

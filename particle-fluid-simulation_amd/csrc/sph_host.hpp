@@ -76,17 +76,17 @@ struct Chain {
     // half the capacity since the record was last read and cleared): `list_target` gives the
     // capacity a record asks for (0: keep), twice
     // the capacity or the list plus a quarter (in steps of 32, at most kList) past a list that
-    // did not fit, and half the capacity (at least kListStart) after kListQuiet records in a
-    // row with no list above half of it.
-    uint32_t list_target(uint32_t record);
+    // did not fit, and, with `shrink`, half the capacity (at least kListStart) after
+    // kListQuiet records in a row with no list above half of it.
+    uint32_t list_target(uint32_t record, bool shrink);
     // Synchronises `s`; with `adapt`, first queues the record's copy and afterwards resizes
     // the lists as it asks.  Returns 1 when the capacity changed (a captured graph must be
     // dropped), 0, or an error.
     int sync_lists(hipStream_t s, bool adapt);
     // Between the frames of a step call, without waiting: looks at the record the copy
     // queued at the previous call left (if it has landed) and queues the next copy; when the
-    // record asks for another capacity it synchronises and resizes.  Returns 1 when the
-    // capacity changed, 0, or an error.
+    // record asks for longer lists it synchronises and grows them (shrinking waits for
+    // sync_lists).  Returns 1 when the capacity changed, 0, or an error.
     int poll_lists_async(hipStream_t s);
     uint32_t list_quiet = 0;        // records in a row with no list above half the capacity
     hipEvent_t list_ev = nullptr;   // marks the landing of the queued record copy
