@@ -14,6 +14,8 @@
 //     look-back over earlier tiles, then scatters through LDS so that each digit's run is
 //     written contiguously.
 //   * Only ceil(log2(T)/w) passes of w-bit digits run: keys are below the table size T.
+#include <type_traits>
+
 #include "sph_device.hpp"
 #include "sph_kernels.hpp"
 
@@ -70,7 +72,11 @@ __global__ __launch_bounds__(kBlk) void k_radix_pass(
     constexpr int kDPT = kBins >= kBlk ? kBins / kBlk : 1;
     constexpr int kOwners = kBins / kDPT;
     static_assert(kItems * kBlk == kTile, "tile must split evenly over the block");
-    __shared__ uint32_t s_whist[kWaves][kBins];  // per wave: count, then LDS start of the wave's run
+    // Per wave and digit: the count, then the LDS start of the wave's run (both at most
+    // kTile), in 16-bit words where 32-bit ones would not fit LDS beside the tile.
+    using WaveCount = std::conditional_t<(kWaves * kBins * 4 > 64 * 1024), uint16_t, uint32_t>;
+    static_assert(kTile < 65536, "16-bit wave counts hold every LDS position of a tile");
+    __shared__ WaveCount s_whist[kWaves][kBins];
     __shared__ uint32_t s_keys[kTile];
     __shared__ uint32_t s_vals[kTile];
     __shared__ int32_t s_delta[kBins];           // global start minus LDS start of each digit
@@ -116,7 +122,7 @@ __global__ __launch_bounds__(kBlk) void k_radix_pass(
             const uint32_t below = (uint32_t)__popcll(peers & lt_mask);
             const uint32_t prior = s_whist[w][d];
             r = prior + below;
-            if (below == 0u) s_whist[w][d] = prior + (uint32_t)__popcll(peers);
+            if (below == 0u) s_whist[w][d] = (WaveCount)(prior + (uint32_t)__popcll(peers));
         }
         rank[k] = r;
     }
@@ -134,7 +140,7 @@ __global__ __launch_bounds__(kBlk) void k_radix_pass(
 #pragma unroll
             for (int ww = 0; ww < kWaves; ++ww) {
                 const uint32_t c = s_whist[ww][d];
-                s_whist[ww][d] = sum;
+                s_whist[ww][d] = (WaveCount)sum;
                 sum += c;
             }
             tcount[q] = sum;
@@ -180,7 +186,7 @@ __global__ __launch_bounds__(kBlk) void k_radix_pass(
     // memory round trip.  A predecessor that has not published yet ends the step there; the
     // next step resumes at it.  Tile 0 always publishes a prefix, so no step passes it.
     // Measured on the 1M-key sort: 4 reads beat 1, 2 and 8; 16 and more congest the L2.
-    constexpr int kLook = kDPT == 1 ? 4 : 1;
+    constexpr int kLook = kDPT <= 2 ? 4 : 1;
     while (pending) {
         bool waited = false;
 #pragma unroll
@@ -221,7 +227,7 @@ __global__ __launch_bounds__(kBlk) void k_radix_pass(
             const int d = tid * kDPT + q;
             s_delta[d] = (int32_t)excl[q] - (int32_t)loff[q];
 #pragma unroll
-            for (int ww = 0; ww < kWaves; ++ww) s_whist[ww][d] += loff[q];
+            for (int ww = 0; ww < kWaves; ++ww) s_whist[ww][d] = (WaveCount)(s_whist[ww][d] + loff[q]);
         }
     }
     __syncthreads();
@@ -249,10 +255,12 @@ __global__ __launch_bounds__(kBlk) void k_radix_pass(
     }
 }
 
-// Threads per tile for each digit width.  8-bit digits: 1024 threads (16 waves, four per
-// SIMD) rank the 4096 keys with 4 items each.  The 11- and 12-bit passes keep 256 threads:
-// their per-wave histograms (2048 and 4096 bins) would not fit LDS for 16 waves.
+// Threads per tile for each digit width.  8- and 11-bit digits: 1024 threads (16 waves, four
+// per SIMD) rank the 4096 keys with 4 items each (the 11-bit per-wave counts in 16-bit
+// words: 64 KB).  The 12-bit pass keeps 256 threads: 16 waves of 4096 16-bit counts would
+// not fit LDS beside the tile.
 constexpr int kBlock8 = 1024;
+constexpr int kBlock11 = 1024;
 constexpr int kBlockWide = 256;
 
 }  // namespace
@@ -266,7 +274,7 @@ hipError_t radix_pass(hipStream_t stream, const uint32_t *kin, const uint32_t *v
     const uint32_t tiles = (n + kTileKeys - 1u) / kTileKeys;
     switch (digit_bits) {
         case 11:
-            hipLaunchKernelGGL((k_radix_pass<kBlockWide, 11>), dim3(tiles), dim3(kBlockWide), 0, stream, kin,
+            hipLaunchKernelGGL((k_radix_pass<kBlock11, 11>), dim3(tiles), dim3(kBlock11), 0, stream, kin,
                                vin, kout, vout, n, shift, bits, ghist, ghist_stride, lookback, tile_counter);
             break;
         case 12:
