@@ -1,9 +1,11 @@
 """The bench's closed-form figures and its one-GPU contract, checked without a device:
 the whole-chain algorithmic bytes per particle-substep (200 + 16 P + 8 T/N) for the three
 table sizes the measurements use, the candidate and neighbour counts of a uniform fill
-(n 27 R^3 and n 4/3 pi R^3), and the refusal of more than one GPU."""
+(n 27 R^3 and n 4/3 pi R^3), the HBM traffic figures it reports against the counter tables
+of the profile it names, and the refusal of more than one GPU."""
 import math
 import os
+import re
 import subprocess
 import sys
 
@@ -36,6 +38,24 @@ def test_uniform_fill_work(bench):
     # the dam break's particles fill a 4 x 6 x 4 block, not the 10 x 6 x 6 box
     d = bench.flops_per_particle_substep(DAM_BREAK_16K)
     assert d["candidates"] == pytest.approx(16384 / 96.0 * 27 * 0.35 ** 3)
+
+
+def test_traffic_matches_its_profile(bench):
+    """Every (warmup, steps) entry of the bench's per-launch HBM bytes is (2 FETCH_SIZE +
+    WRITE_SIZE) KiB of the density and force rows in the profile's table for that run."""
+    text = open(os.path.join(ROOT, bench.TRAFFIC_PROFILE)).read()
+    rows = {"density": "k_density<OneScene, 4, 128, true, false>",
+            "force": "k_force<OneScene, 4, 128, true, false>"}
+    assert bench.TRAFFIC_BYTES_PER_LAUNCH
+    for (warmup, steps), entry in bench.TRAFFIC_BYTES_PER_LAUNCH.items():
+        head = "## Per launch, after %d warmup + %d timed steps" % (warmup, steps)
+        assert head in text, head
+        section = text.split(head, 1)[1].split("\n## ", 1)[0]
+        for kind, kernel in rows.items():
+            m = re.search(r"\| `%s` \| (\d+) \| (\d+) \|" % re.escape(kernel), section)
+            assert m, (head, kernel)
+            fetch, write = int(m.group(1)), int(m.group(2))
+            assert entry[kind] == (2 * fetch + write) * 1024, (head, kind)
 
 
 def test_more_than_one_gpu_is_refused():
